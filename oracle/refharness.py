@@ -1,0 +1,61 @@
+"""Runs the golden-vector harness built from the reference's own serial sources
+(oracle/ref/, output in oracle/_ref/).  Test infrastructure only; it needs the
+reference tree and amdflang, so it runs in the build container, never on the GPU box."""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+HARNESS = os.path.join(REF_DIR, "mom6_ref_harness")
+DRIVER = os.path.join(REF_DIR, "rk2_serial_driver")
+REFERENCE = os.environ.get("MOM6_REFERENCE", "/root/reference")
+
+
+def available() -> bool:
+    return os.path.isdir(os.path.join(REFERENCE, "src", "serial"))
+
+
+def build():
+    """make -C oracle/ref (compiles the reference sources where they lie)."""
+    subprocess.check_call(["make", "-s", "-C", os.path.join(HERE, "ref"), f"REF={REFERENCE}"],
+                          stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    return HARNESS
+
+
+def run(cfg, case: dict, niter: int, dump_mode: int = 0) -> dict:
+    from hackathon_mom6_miniapp_amd.state import INPUT_ORDER
+
+    build()
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout, fcfg = (os.path.join(tmp, n) for n in ("in.bin", "out.bin", "cfg.txt"))
+        with open(fin, "wb") as f:
+            for name in INPUT_ORDER:
+                f.write(np.asarray(case[name], dtype="<f8").tobytes(order="F"))
+        with open(fcfg, "w") as f:
+            f.write(f"{cfg.ni} {cfg.nj} {cfg.nk} {niter} {cfg.bt_nsteps}\n")
+            f.write(f"{cfg.dt!r} {cfg.dx_km!r}\n{cfg.bc_x} {cfg.bc_y}\n")
+            f.write(f"{cfg.Kh!r} {cfg.Ah!r} {int(cfg.laplacian)} {int(cfg.biharmonic)} 0 0 0\n")
+            f.write(f"{cfg.Kv!r} {cfg.Kv_ml!r} {cfg.Kv_extra_bbl!r} {cfg.Hmix!r} {cfg.Hbbl!r} {cfg.ray_bottom!r}\n")
+            f.write(f"{cfg.coriolis_scheme} {int(cfg.monotonic)}\n{fin}\n{fout}\n{dump_mode}\n")
+        env = dict(os.environ)
+        subprocess.check_call(["bash", "-c", f'ulimit -s unlimited && exec "{HARNESS}" "{fcfg}"'], env=env)
+        return read_dump(fout, cfg)
+
+
+def read_dump(path: str, cfg) -> dict:
+    out = {}
+    raw = open(path, "rb").read()
+    pos, n2 = 0, cfg.n1 * cfg.n2
+    while pos < len(raw):
+        name = raw[pos:pos + 16].decode().strip()
+        cnt = int(np.frombuffer(raw, "<i8", 1, pos + 16)[0])
+        data = np.frombuffer(raw, "<f8", cnt, pos + 24).copy()
+        pos += 24 + 8 * cnt
+        shape = (cfg.n1, cfg.n2) if cnt == n2 else (cfg.n1, cfg.n2, cnt // n2)
+        out[name] = data.reshape(shape, order="F")
+    return out
