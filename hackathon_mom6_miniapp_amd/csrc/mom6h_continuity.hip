@@ -1,0 +1,456 @@
+// mom6h_continuity.hip -- zonal PPM continuity on gfx950.
+//
+// Arithmetic of src/serial/mom6_continuity_serial.F90 (the serial oracle, not
+// the cuda_c kernels): positive-definite limiter by default (monotonic=.false.,
+// :66-76), REAL(4) tolerance literals (:826-830, :856-866, :993-994, :1086,
+// :1105) and REAL(4) limiter locals (:1266-1268, :1311-1315), Newton
+// bookkeeping that keeps the last evaluated layer fluxes (:877-914).
+//
+// The driver passes visc_rem_u = por_face_areaU = 1 everywhere
+// (mom6_continuity_serial.F90:63-64).  Multiplying or dividing by 1.0 is exact
+// in IEEE arithmetic, so the kernels drop those factors and stay bit-identical.
+//
+// Kernels
+//   k_ppm_edges     (i,j,k) threads  -> h_W, h_E          zonal_edge_thickness
+//   k_mass_flux     one thread per u-column (I,j), k loops: layer fluxes, CFL
+//                   bracket, Newton to the barotropic transport, BT_cont and
+//                   h_u, u_cor/du_cor                       zonal_mass_flux
+//   k_convergence   (i,j,k) threads  -> h                  continuity_zonal_convergence
+#include "mom6h_internal.h"
+
+namespace mom6h {
+
+// ------------------------------------------------------------ PPM edges ---
+// PPM_reconstruction_x (:1124-1247) slope at one point.
+__device__ inline double ppm_slope(const Dims &d, const double *__restrict__ h,
+                                   const double *__restrict__ mT, int i, int j, int k)
+{
+    if ((mT[IX2(d, i - 1, j)] * mT[IX2(d, i, j)] * mT[IX2(d, i + 1, j)]) == 0.0) return 0.0;
+    const double hp = h[IX3(d, i + 1, j, k)], hm = h[IX3(d, i - 1, j, k)], h0 = h[IX3(d, i, j, k)];
+    double s = 0.5 * (hp - hm);
+    const double dMx = dmax(dmax(hp, hm), h0) - h0;
+    const double dMn = h0 - dmin(dmin(hp, hm), h0);
+    return copysign(1.0, s) * dmin(fabs(s), 2.0 * dmin(dMx, dMn));
+}
+
+__global__ void k_ppm_edges(Dims d, const double *__restrict__ h, const double *__restrict__ mT,
+                            double *__restrict__ hW, double *__restrict__ hE, int upwind_1st,
+                            int simple_2nd, int monotonic, double h_min)
+{
+    const int i = d.isc - 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.jsc + blockIdx.y;
+    const int k = 1 + blockIdx.z;
+    if (i > d.iec + 1) return;
+    const long long c = IX3(d, i, j, k);
+    const double hi = h[c];
+    if (upwind_1st) {  // zonal_edge_thickness (:199-207): no reconstruction, no limiter
+        hW[c] = hi;
+        hE[c] = hi;
+        return;
+    }
+    const double mm = mT[IX2(d, i - 1, j)], mp = mT[IX2(d, i + 1, j)];
+    const double h_im1 = mm * h[IX3(d, i - 1, j, k)] + (1.0 - mm) * hi;
+    const double h_ip1 = mp * h[IX3(d, i + 1, j, k)] + (1.0 - mp) * hi;
+    double hl, hr;
+    if (simple_2nd) {
+        hl = 0.5 * (h_im1 + hi);
+        hr = 0.5 * (h_ip1 + hi);
+    } else {
+        const double oneSixth = 1.0 / 6.0;
+        const double sm = ppm_slope(d, h, mT, i - 1, j, k);
+        const double s0 = ppm_slope(d, h, mT, i, j, k);
+        const double sp = ppm_slope(d, h, mT, i + 1, j, k);
+        hl = 0.5 * (h_im1 + hi) + oneSixth * (sm - s0);
+        hr = 0.5 * (h_ip1 + hi) + oneSixth * (s0 - sp);
+    }
+    if (monotonic) {  // PPM_limit_CW84 (:1298-1338), REAL(4) locals
+        const float h_i = (float)hi;
+        if ((hr - (double)h_i) * ((double)h_i - hl) <= 0.0) {
+            hl = (double)h_i;
+            hr = (double)h_i;
+        } else {
+            const float RLdiff = (float)(hr - hl);
+            const float RLmean = (float)(0.5 * (hr + hl));
+            const float FunFac = 6.0f * RLdiff * (h_i - RLmean);
+            const float RLdiff2 = RLdiff * RLdiff;
+            if (FunFac > RLdiff2) hl = (double)(3.0f * h_i) - 2.0 * hr;
+            if (FunFac < -RLdiff2) hr = (double)(3.0f * h_i) - 2.0 * hl;
+        }
+    } else {  // PPM_limit_pos (:1253-1294), REAL(4) curv, dh, scale
+        const float curv = (float)(3.0 * ((hl + hr) - 2.0 * hi));
+        if ((double)curv > 0.0) {
+            const float dh = (float)(hr - hl);
+            if (fabsf(dh) < curv) {
+                if (hi <= h_min) {
+                    hl = hi;
+                    hr = hi;
+                } else {
+                    const double lhs = (double)(12.0f * curv) * (hi - h_min);
+                    const float rhs = curv * curv + 3.0f * (dh * dh);
+                    if (lhs < (double)rhs) {
+                        const float scale = (float)(lhs / (double)rhs);
+                        hl = hi + (double)scale * (hl - hi);
+                        hr = hi + (double)scale * (hr - hi);
+                    }
+                }
+            }
+        }
+    }
+    hW[c] = hl;
+    hE[c] = hr;
+}
+
+// ------------------------------------------------------- layer flux ---
+// zonal_flux_layer (:516-598) for one face with visc_rem = por = 1:
+// uh = (dy*u)*(...), h_marg; duhdu = dy*h_marg.
+__device__ inline void zflux(double u, double dt, double dy, double IdxT0, double IdxT1, double hW0,
+                             double hE0, double h0, double hW1, double hE1, double h1, double &uh,
+                             double &hmarg)
+{
+    if (u > 0.0) {
+        const double CFL = u * dt * IdxT0;
+        const double curv_3 = (hW0 + hE0) - 2.0 * h0;
+        uh = (dy * u) * (hE0 + CFL * (0.5 * (hW0 - hE0) + curv_3 * (CFL - 1.5)));
+        hmarg = hE0 + CFL * ((hW0 - hE0) + 3.0 * curv_3 * (CFL - 1.0));
+    } else if (u < 0.0) {
+        const double CFL = -(u * dt * IdxT1);
+        const double curv_3 = (hW1 + hE1) - 2.0 * h1;
+        uh = (dy * u) * (hW1 + CFL * (0.5 * (hE1 - hW1) + curv_3 * (CFL - 1.5)));
+        hmarg = hW1 + CFL * ((hE1 - hW1) + 3.0 * curv_3 * (CFL - 1.0));
+    } else {
+        uh = 0.0;
+        hmarg = 0.5 * (hW1 + hE0);
+    }
+}
+
+// Per-column view of the six thickness values a flux at face I needs.
+struct Column {
+    const double *u, *hW, *hE, *h;  // base pointers
+    long long ci, ci1;              // offset of cell i and i+1 at k=1
+    long long cu;                   // offset of face I at k=1
+    long long s2;                   // k stride
+    double dt, dy, IdxT0, IdxT1;
+    __device__ inline void eval(int k, double un, double &uh, double &hm) const
+    {
+        const long long o = (long long)(k - 1) * s2;
+        zflux(un, dt, dy, IdxT0, IdxT1, hW[ci + o], hE[ci + o], h[ci + o], hW[ci1 + o], hE[ci1 + o],
+              h[ci1 + o], uh, hm);
+    }
+    __device__ inline double uk(int k) const { return u[cu + (long long)(k - 1) * s2]; }
+};
+
+// zonal_flux_adjust (:739-916) for one column.  uh3 != nullptr: uh_3d present
+// (layer fluxes written back, evaluated also at itt = max_itts).
+__device__ double flux_adjust(const Column &col, int nk, double uhbt, double uh_tot_0,
+                              double duhdu_tot_0, double du_max_CFL, double du_min_CFL,
+                              double dtIarea, double tol_eta0, double tol_vel, double *uh3)
+{
+    const int max_itts = 20;
+    double du = 0.0, du_max = du_max_CFL, du_min = du_min_CFL;
+    double uh_err = uh_tot_0 - uhbt, duhdu_tot = duhdu_tot_0;
+    double uh_err_best = fabs(uh_err);
+    bool do_I = true;
+    for (int itt = 1; itt <= max_itts; ++itt) {
+        double tol_eta;
+        if (itt <= 1) tol_eta = (double)1e-6f * tol_eta0;
+        else if (itt == 2) tol_eta = (double)1e-4f * tol_eta0;
+        else if (itt == 3) tol_eta = (double)1e-2f * tol_eta0;
+        else tol_eta = tol_eta0;
+        if (uh_err > 0.0) du_max = du;
+        else if (uh_err < 0.0) du_min = du;
+        else do_I = false;
+        if (do_I) {
+            const double aerr = fabs(uh_err);
+            if ((dtIarea * aerr > tol_eta) || ((aerr > tol_vel * duhdu_tot) || (aerr > uh_err_best))) {
+                const double ddu = -uh_err / duhdu_tot;
+                const double du_prev = du;
+                du = du + ddu;
+                if (fabs(ddu) < (double)1.0e-15f * fabs(du)) {
+                    do_I = false;
+                } else if (ddu > 0.0) {
+                    if (du >= du_max) {
+                        du = 0.5 * (du_prev + du_max);
+                        if (du_max - du_prev < (double)1.0e-15f * fabs(du)) do_I = false;
+                    }
+                } else {
+                    if (du <= du_min) {
+                        du = 0.5 * (du_prev + du_min);
+                        if (du_prev - du_min < (double)1.0e-15f * fabs(du)) do_I = false;
+                    }
+                }
+            } else {
+                do_I = false;
+            }
+        }
+        // A column that is done leaves its du untouched for the rest of the
+        // row's iterations in the reference, so it can stop here.
+        if (!do_I) break;
+        if ((itt < max_itts) || uh3) {
+            double se = -uhbt, sd = 0.0;
+            for (int k = 1; k <= nk; ++k) {
+                double uh, hm;
+                col.eval(k, col.uk(k) + du, uh, hm);
+                if (uh3) uh3[col.cu + (long long)(k - 1) * col.s2] = uh;
+                se = se + uh;
+                sd = sd + col.dy * hm;
+            }
+            if (itt < max_itts) {
+                uh_err = se;
+                duhdu_tot = sd;
+                uh_err_best = dmin(uh_err_best, fabs(uh_err));
+            }
+        }
+    }
+    return du;
+}
+
+__global__ void k_mass_flux(Dims d, const double *__restrict__ u, const double *__restrict__ hin,
+                            const double *__restrict__ hW, const double *__restrict__ hE,
+                            double *__restrict__ uh, double dt, const double *__restrict__ uhbt,
+                            const double *__restrict__ IareaT, const double *__restrict__ IdxT,
+                            const double *__restrict__ dy_Cu, const double *__restrict__ dxT,
+                            const double *__restrict__ dxCu, const double *__restrict__ mask2dCu,
+                            double tol_eta0, double tol_vel, double CFL_limit_adjust,
+                            double *__restrict__ u_cor, double *__restrict__ du_cor,
+                            double *__restrict__ FA_u_EE, double *__restrict__ FA_u_E0,
+                            double *__restrict__ FA_u_W0, double *__restrict__ FA_u_WW,
+                            double *__restrict__ uBT_WW, double *__restrict__ uBT_EE,
+                            double *__restrict__ bt_h_u, int set_bt_cont)
+{
+    const int I = d.isc - 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.jsc + blockIdx.y;
+    if (I > d.iec) return;
+    const int i = I;
+    const int nk = d.nk;
+    Column col;
+    col.u = u; col.hW = hW; col.hE = hE; col.h = hin;
+    col.ci = IX3(d, i, j, 1); col.ci1 = IX3(d, i + 1, j, 1); col.cu = IX3(d, I, j, 1);
+    col.s2 = d.s2; col.dt = dt;
+    const long long c2 = IX2(d, I, j);
+    col.dy = dy_Cu[c2];
+    col.IdxT0 = IdxT[IX2(d, i, j)];
+    col.IdxT1 = IdxT[IX2(d, i + 1, j)];
+    const double mCu = mask2dCu[c2];
+
+    // layer fluxes and their sums (zonal_mass_flux :300-355)
+    const double CFL_dt = CFL_limit_adjust / dt;
+    const double dx_W = dxT[IX2(d, i, j)], dx_E = dxT[IX2(d, i + 1, j)];
+    double du_max_CFL = 2.0 * (CFL_dt * dx_W) * 1.0;     // I_vrm = 1/visc_rem_max = 1
+    double du_min_CFL = -(2.0 * (CFL_dt * dx_E) * 1.0);
+    double uh_tot_0 = 0.0, duhdu_tot_0 = 0.0;
+    for (int k = 1; k <= nk; ++k) {
+        const double uu = col.uk(k);
+        double f, hm;
+        col.eval(k, uu, f, hm);
+        uh[col.cu + (long long)(k - 1) * d.s2] = f;
+        duhdu_tot_0 = duhdu_tot_0 + col.dy * hm;
+        uh_tot_0 = uh_tot_0 + f;
+        // CFL bracket, use_visc_rem branch without aggress_adjust (:378-393)
+        if (du_max_CFL > dx_W * CFL_dt - uu * mCu) du_max_CFL = (dx_W * CFL_dt - uu);
+        if (du_min_CFL < -dx_E * CFL_dt - uu * mCu) du_min_CFL = -(dx_E * CFL_dt + uu);
+    }
+    du_max_CFL = dmax(du_max_CFL, 0.0);
+    du_min_CFL = dmin(du_min_CFL, 0.0);
+
+    const double dtIarea = dt * dmin(IareaT[IX2(d, i, j)], IareaT[IX2(d, i + 1, j)]);
+    const double bt = uhbt ? uhbt[c2] : 0.0;
+    const double du = flux_adjust(col, nk, bt, uh_tot_0, duhdu_tot_0, du_max_CFL, du_min_CFL,
+                                  dtIarea, tol_eta0, tol_vel, uh);
+    if (du_cor) du_cor[c2] = du;
+    if (u_cor)
+        for (int k = 1; k <= nk; ++k) u_cor[col.cu + (long long)(k - 1) * d.s2] = col.uk(k) + du;
+    if (!set_bt_cont) return;
+
+    // set_zonal_BT_cont (:920-1121); the zero-transport correction equals du
+    // whenever the target transport is zero (identical Newton iterations).
+    const double du0 = (bt == 0.0) ? du
+                                   : flux_adjust(col, nk, 0.0, uh_tot_0, duhdu_tot_0, du_max_CFL,
+                                                 du_min_CFL, dtIarea, tol_eta0, tol_vel, nullptr);
+    const double Idt = 1.0 / dt;
+    const double CFL_min = (double)1e-6f;
+    const double du_CFL = (CFL_min * Idt) * dxCu[c2];
+    double duR = dmin(0.0, du0 - du_CFL);
+    double duL = dmax(0.0, du0 + du_CFL);
+    for (int k = 1; k <= nk; ++k) {   // visc_rem_lim = max(1, 0.1) = 1
+        const double uu = col.uk(k);
+        if (uu + duR > -du_CFL) duR = -(uu + du_CFL);
+        if (uu + duL < du_CFL) duL = -(uu - du_CFL);
+    }
+    double FAmt_0 = 0.0, FAmt_L = 0.0, FAmt_R = 0.0, uhtot_L = 0.0, uhtot_R = 0.0;
+    for (int k = 1; k <= nk; ++k) {
+        const double uu = col.uk(k);
+        double f0, h0m, fL, hLm, fR, hRm;
+        col.eval(k, uu + du0, f0, h0m);
+        col.eval(k, uu + duL, fL, hLm);
+        col.eval(k, uu + duR, fR, hRm);
+        FAmt_0 = FAmt_0 + col.dy * h0m;
+        FAmt_L = FAmt_L + col.dy * hLm;
+        FAmt_R = FAmt_R + col.dy * hRm;
+        uhtot_L = uhtot_L + fL;
+        uhtot_R = uhtot_R + fR;
+        // zonal_flux_thickness (:602-735) at u_cor = u + du, marginal faces
+        double hm_cor = h0m;
+        if (du0 != du) {
+            double fc;
+            col.eval(k, uu + du, fc, hm_cor);
+        }
+        bt_h_u[col.cu + (long long)(k - 1) * d.s2] = hm_cor;
+    }
+    double FA_0 = FAmt_0, FA_avg = FAmt_0;
+    if ((duL - du0) != 0.0) FA_avg = uhtot_L / (duL - du0);
+    if (FA_avg > dmax(FA_0, FAmt_L)) FA_avg = dmax(FA_0, FAmt_L);
+    else if (FA_avg < dmin(FA_0, FAmt_L)) FA_0 = FA_avg;
+    FA_u_W0[c2] = FA_0;
+    FA_u_WW[c2] = FAmt_L;
+    uBT_WW[c2] = (fabs(FA_0 - FAmt_L) <= (double)1e-12f * FA_0)
+                     ? 0.0
+                     : (1.5 * (duL - du0)) * ((FAmt_L - FA_avg) / (FAmt_L - FA_0));
+    FA_0 = FAmt_0;
+    FA_avg = FAmt_0;
+    if ((duR - du0) != 0.0) FA_avg = uhtot_R / (duR - du0);
+    if (FA_avg > dmax(FA_0, FAmt_R)) FA_avg = dmax(FA_0, FAmt_R);
+    else if (FA_avg < dmin(FA_0, FAmt_R)) FA_0 = FA_avg;
+    FA_u_E0[c2] = FA_0;
+    FA_u_EE[c2] = FAmt_R;
+    uBT_EE[c2] = (fabs(FAmt_R - FA_0) <= (double)1e-12f * FA_0)
+                     ? 0.0
+                     : (1.5 * (duR - du0)) * ((FAmt_R - FA_avg) / (FAmt_R - FA_0));
+}
+
+// continuity_zonal_convergence (:148-176) with hin; optionally copies hin into
+// the halo of h (the driver's `h = h0` reset folded into the predictor call).
+__global__ void k_convergence(Dims d, const double *hin, const double *__restrict__ uh, double dt,
+                              const double *__restrict__ IareaT, double *h, int halo_copy)
+{
+    const int i = d.isd + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.jsd + blockIdx.y;
+    const int k = 1 + blockIdx.z;
+    if (i > d.ied) return;
+    const long long c = IX3(d, i, j, k);
+    const bool inside = i >= d.isc && i <= d.iec && j >= d.jsc && j <= d.jec;
+    if (inside) {
+        const double v = hin[c] - dt * IareaT[IX2(d, i, j)] * (uh[c] - uh[c - 1]);
+        h[c] = dmax(v, 0.0);
+    } else if (halo_copy) {
+        h[c] = hin[c];
+    }
+}
+
+}  // namespace mom6h
+
+using namespace mom6h;
+
+extern "C" {
+
+int mom6h_continuity_init(mom6h_continuity_cs **out, int isd, int ied, int jsd, int jed, int isc,
+                          int iec, int jsc, int jec, int nk, const double *IareaT,
+                          const double *IdxT, const double *dy_Cu, const double *mask2dT,
+                          int monotonic, const double *dxT, const double *dxCu,
+                          const double *mask2dCu)
+{
+    MOM6H_REQUIRE(out != nullptr, "mom6h_continuity_init: NULL cs");
+    MOM6H_REQUIRE(isd < isc - 2 && iec + 2 < ied && jsd <= jsc && jec <= jed && isc <= iec &&
+                      jsc <= jec && nk >= 1,
+                  "mom6h_continuity_init: PPM needs 3 halo columns (isd..isc-3, iec+3..ied)");
+    MOM6H_REQUIRE(IareaT && IdxT && dy_Cu && mask2dT && dxT && dxCu && mask2dCu,
+                  "mom6h_continuity_init: NULL metric");
+    auto *cs = new mom6h_continuity_cs();
+    cs->d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, nk);
+    // continuity_init (mom6_continuity_serial.F90:66-76)
+    cs->monotonic = monotonic;
+    cs->upwind_1st = 0;
+    cs->simple_2nd = 0;
+    cs->tol_eta = 1.0e-12;
+    cs->tol_vel = 3.0e8;
+    cs->CFL_limit_adjust = 0.5;
+    const size_t n2 = (size_t)cs->d.s2, n3 = n2 * nk;
+    int rc = 0;
+    rc = rc ? rc : dev_upload(&cs->IareaT, IareaT, n2);
+    rc = rc ? rc : dev_upload(&cs->IdxT, IdxT, n2);
+    rc = rc ? rc : dev_upload(&cs->dy_Cu, dy_Cu, n2);
+    rc = rc ? rc : dev_upload(&cs->mask2dT, mask2dT, n2);
+    rc = rc ? rc : dev_upload(&cs->dxT, dxT, n2);
+    rc = rc ? rc : dev_upload(&cs->dxCu, dxCu, n2);
+    rc = rc ? rc : dev_upload(&cs->mask2dCu, mask2dCu, n2);
+    rc = rc ? rc : dev_alloc(&cs->hW, n3, true);
+    rc = rc ? rc : dev_alloc(&cs->hE, n3, true);
+    if (rc) {
+        mom6h_continuity_end(cs);
+        return rc;
+    }
+    *out = cs;
+    return MOM6H_OK;
+}
+
+int mom6h_continuity_ppm(mom6h_continuity_cs *cs, const double *u, const double *hin, double *h,
+                         double *uh, double dt, const double *uhbt, const mom6h_bt_cont *bt,
+                         double *u_cor, double *du_cor, int hin_halo_to_h, void *stream)
+{
+    MOM6H_REQUIRE(cs != nullptr, "mom6h_continuity_ppm: NULL cs");
+    MOM6H_REQUIRE(u && hin && h && uh, "mom6h_continuity_ppm: NULL array");
+    MOM6H_REQUIRE(dt > 0.0, "mom6h_continuity_ppm: dt must be positive");
+    MOM6H_REQUIRE(!bt || (bt->FA_u_EE && bt->FA_u_E0 && bt->FA_u_W0 && bt->FA_u_WW && bt->uBT_WW &&
+                          bt->uBT_EE && bt->h_u),
+                  "mom6h_continuity_ppm: incomplete BT_cont");
+    const Dims &d = cs->d;
+    hipStream_t s = (hipStream_t)stream;
+    {
+        const int ni = (d.iec + 1) - (d.isc - 1) + 1;
+        dim3 b(64), g((ni + 63) / 64, d.jec - d.jsc + 1, d.nk);
+        k_ppm_edges<<<g, b, 0, s>>>(d, hin, cs->mask2dT, cs->hW, cs->hE, cs->upwind_1st,
+                                    cs->simple_2nd, cs->monotonic, 2.0 * 1.0e-10);
+        MOM6H_LAUNCH_CHECK();
+    }
+    {
+        const int ni = d.iec - (d.isc - 1) + 1;
+        dim3 b(64), g((ni + 63) / 64, d.jec - d.jsc + 1);
+        k_mass_flux<<<g, b, 0, s>>>(d, u, hin, cs->hW, cs->hE, uh, dt, uhbt, cs->IareaT, cs->IdxT,
+                                    cs->dy_Cu, cs->dxT, cs->dxCu, cs->mask2dCu, cs->tol_eta,
+                                    cs->tol_vel, cs->CFL_limit_adjust, u_cor, du_cor,
+                                    bt ? bt->FA_u_EE : nullptr, bt ? bt->FA_u_E0 : nullptr,
+                                    bt ? bt->FA_u_W0 : nullptr, bt ? bt->FA_u_WW : nullptr,
+                                    bt ? bt->uBT_WW : nullptr, bt ? bt->uBT_EE : nullptr,
+                                    bt ? bt->h_u : nullptr, bt ? 1 : 0);
+        MOM6H_LAUNCH_CHECK();
+    }
+    {
+        dim3 b(128), g((d.n1 + 127) / 128, d.n2, d.nk);
+        k_convergence<<<g, b, 0, s>>>(d, hin, uh, dt, cs->IareaT, h, hin_halo_to_h);
+        MOM6H_LAUNCH_CHECK();
+    }
+    return MOM6H_OK;
+}
+
+int mom6h_continuity_end(mom6h_continuity_cs *cs)
+{
+    if (!cs) return MOM6H_OK;
+    for (double *p : {cs->IareaT, cs->IdxT, cs->dy_Cu, cs->mask2dT, cs->dxT, cs->dxCu,
+                      cs->mask2dCu, cs->hW, cs->hE})
+        dev_free(p);
+    delete cs;
+    return MOM6H_OK;
+}
+
+int mom6h_bt_cont_alloc(mom6h_bt_cont *bt, int isd, int ied, int jsd, int jed, int nk)
+{
+    MOM6H_REQUIRE(bt != nullptr && ied >= isd && jed >= jsd && nk >= 1, "mom6h_bt_cont_alloc: bad args");
+    *bt = mom6h_bt_cont{};
+    const size_t n2 = (size_t)(ied - isd + 1) * (jed - jsd + 1);
+    int rc = 0;
+    for (double **p : {&bt->FA_u_EE, &bt->FA_u_E0, &bt->FA_u_W0, &bt->FA_u_WW, &bt->uBT_WW, &bt->uBT_EE})
+        rc = rc ? rc : dev_alloc(p, n2, true);
+    rc = rc ? rc : dev_alloc(&bt->h_u, n2 * nk, true);
+    if (rc) mom6h_bt_cont_free(bt);
+    return rc;
+}
+
+int mom6h_bt_cont_free(mom6h_bt_cont *bt)
+{
+    if (!bt) return MOM6H_OK;
+    for (double *p : {bt->FA_u_EE, bt->FA_u_E0, bt->FA_u_W0, bt->FA_u_WW, bt->uBT_WW, bt->uBT_EE, bt->h_u})
+        dev_free(p);
+    *bt = mom6h_bt_cont{};
+    return MOM6H_OK;
+}
+
+}  // extern "C"

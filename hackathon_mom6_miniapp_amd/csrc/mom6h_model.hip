@@ -1,0 +1,476 @@
+// mom6h_model.hip -- one run of the split-RK2 driver on one GPU.
+//
+// Replays app/rk2_serial_driver.F90:218-399 with every module on the device:
+//   predictor  transports -> hor_visc -> CorAdCalc -> velocity update ->
+//              vert_visc -> [btstep] -> continuity
+//   corrector  the same on the predicted state, then the final continuity
+// The barotropic chain (btstep predictor -> corrector) reads only eta/ubt/vbt
+// and writes only ubt_av/vbt_av/eta_av/eta, none of which the 3D stages touch
+// (rk2_serial_driver.F90:285,365), so it runs on a second stream and joins the
+// main stream at the end of the step.
+//
+// Two copies of the reference driver are avoided without changing any value:
+//   * h = h0 at the top of the step (:222-228): the predictor stages read h0
+//     directly and the predictor continuity writes the whole h_pred field
+//     (compute domain updated, halo = h0);
+//   * htmp = h before the final continuity (:373-379): the predictor
+//     continuity writes straight into htmp and the final one writes h from it.
+// For periodic grids the 3-wide halos are refilled at the exchange points of
+// app/rk2_mpi_serial_driver.F90 (uh/vh after transports, up/vp before the
+// predictor continuity, h after it, u/v/h/eta/ubt/vbt at the end of the step).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mom6h_internal.h"
+
+using namespace mom6h;
+
+namespace {
+
+const double kG_EARTH = 9.80;     // src/common/mom6_types.F90:80
+const double kDepth = 4000.0;     // mom6_barotropic_serial.F90:76
+
+struct Field {
+    std::string name;
+    double **p;
+    size_t n;
+};
+
+enum Stage {
+    ST_TRANSPORTS, ST_HOR_VISC, ST_CORIOLIS, ST_VEL_UPDATE, ST_VERT_VISC, ST_CONTINUITY,
+    ST_HALO, ST_BTSTEP, ST_COUNT
+};
+const char *kStageNames[ST_COUNT] = {"transports", "hor_visc",   "coriolis", "vel_update",
+                                     "vert_visc",  "continuity", "halo",     "btstep"};
+
+}  // namespace
+
+struct mom6h_model {
+    mom6h_config c;
+    Dims d;
+    Halo halo;
+    bool prepared = false;
+    int use_graph = 0, profile = 0;
+    hipStream_t s0 = nullptr, s1 = nullptr;
+    hipEvent_t e_start = nullptr, e_bt = nullptr;
+    // state and driver arrays
+    double *u, *v, *h, *h0, *htmp, *uh, *vh, *CAu, *CAv, *up, *vp, *diffu, *diffv;
+    double *eta, *ubt, *vbt, *ubt_av, *vbt_av, *eta_av, *u_cor, *du_cor;
+    double *taux, *tauy, *ray_u, *ray_v, *CoriolisBu, *mask2dT, *mask2dCu, *mask2dCv, *mask2dBu;
+    mom6h_bt_cont bt{};
+    std::vector<Field> fields;
+    // host copies of the uniform metrics (init_ocean_grid)
+    std::vector<double> IareaT, areaT, dxT, dyT, IdxT, IdyT, dxCu, dyCu, dy_Cu, IdxCu, IdyCu;
+    std::vector<double> dxCv, dyCv, IdxCv, IdyCv, IareaBu, IareaCu, IareaCv, dxBu, dyBu, IdxBu, IdyBu;
+    // module control structures
+    mom6h_continuity_cs *cont = nullptr;
+    mom6h_coriolis_cs *cor = nullptr;
+    mom6h_barotropic_cs *bt_cs = nullptr;
+    mom6h_vert_visc_cs *vv = nullptr;
+    mom6h_hor_visc_cs *hv = nullptr;
+    // profiling
+    std::vector<hipEvent_t> ev;
+    double stage_ms[ST_COUNT] = {0};
+};
+
+namespace {
+
+int add_field(mom6h_model *m, const char *name, double **p, size_t n, bool zero)
+{
+    int rc = dev_alloc(p, n, zero);
+    if (rc) return rc;
+    m->fields.push_back(Field{name, p, n});
+    return MOM6H_OK;
+}
+
+// init_ocean_grid (src/common/mom6_types.F90:130-258): uniform spacing dx.
+void host_grid(mom6h_model *m)
+{
+    const size_t n2 = (size_t)m->d.s2;
+    const double dx_m = m->c.dx_km * 1000.0;
+    auto fill = [&](std::vector<double> &a, double v) { a.assign(n2, v); };
+    fill(m->areaT, dx_m * dx_m);
+    fill(m->IareaT, 1.0 / (dx_m * dx_m));
+    fill(m->dxT, dx_m); fill(m->dyT, dx_m); fill(m->IdxT, 1.0 / dx_m); fill(m->IdyT, 1.0 / dx_m);
+    fill(m->dxCu, dx_m); fill(m->dyCu, dx_m); fill(m->dy_Cu, dx_m);
+    fill(m->IdxCu, 1.0 / dx_m); fill(m->IdyCu, 1.0 / dx_m);
+    fill(m->dxCv, dx_m); fill(m->dyCv, dx_m); fill(m->IdxCv, 1.0 / dx_m); fill(m->IdyCv, 1.0 / dx_m);
+    fill(m->IareaBu, 1.0 / (dx_m * dx_m)); fill(m->IareaCu, 1.0 / (dx_m * dx_m));
+    fill(m->IareaCv, 1.0 / (dx_m * dx_m));
+    fill(m->dxBu, dx_m); fill(m->dyBu, dx_m); fill(m->IdxBu, 1.0 / dx_m); fill(m->IdyBu, 1.0 / dx_m);
+}
+
+std::vector<double> download(const double *d_p, size_t n)
+{
+    std::vector<double> h(n);
+    if (hipMemcpy(h.data(), d_p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) h.clear();
+    return h;
+}
+
+int fill_halos(mom6h_model *m, std::initializer_list<std::pair<double *, int>> arrays)
+{
+    if (!m->halo.px && !m->halo.py) return MOM6H_OK;
+    for (auto &a : arrays) {
+        int rc = halo_fill(m->d, m->halo, a.first, a.second, m->s0);
+        if (rc) return rc;
+    }
+    return MOM6H_OK;
+}
+
+#define RC(expr)               \
+    do {                       \
+        int rc_ = (expr);      \
+        if (rc_) return rc_;   \
+    } while (0)
+
+struct StageTimer {
+    mom6h_model *m;
+    int stage;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    StageTimer(mom6h_model *m_, int st, hipStream_t s_) : m(m_), stage(st), s(s_)
+    {
+        if (m->profile) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, s);
+        }
+    }
+    ~StageTimer()
+    {
+        if (m->profile) {
+            (void)hipEventRecord(b, s);
+            m->ev.push_back(a);
+            m->ev.push_back(b);
+            m->ev.push_back((hipEvent_t)(intptr_t)stage);
+        }
+    }
+};
+
+// One RK2 step (rk2_serial_driver.F90:218-399).
+int enqueue_step(mom6h_model *m)
+{
+    const double dt = m->c.dt;
+    const int nk = m->c.nk;
+    const Dims &d = m->d;
+    hipStream_t s0 = m->s0, s1 = m->s1;
+    const double *rayu = m->ray_u, *rayv = m->ray_v;
+
+    // barotropic chain on its own stream (:285, :365)
+    MOM6H_CHECK(hipEventRecord(m->e_start, s0));
+    MOM6H_CHECK(hipStreamWaitEvent(s1, m->e_start, 0));
+    {
+        StageTimer t(m, ST_BTSTEP, s1);
+        RC(mom6h_btstep(m->bt_cs, m->eta, m->ubt, m->vbt, m->ubt_av, m->vbt_av, m->eta_av, s1));
+        RC(mom6h_btstep(m->bt_cs, m->eta_av, m->ubt_av, m->vbt_av, m->ubt_av, m->vbt_av, m->eta, s1));
+    }
+    MOM6H_CHECK(hipEventRecord(m->e_bt, s1));
+
+    // ---- predictor (h = h0 throughout)
+    { StageTimer t(m, ST_TRANSPORTS, s0);
+      RC(mom6h_transports(d.isd, d.ied, d.jsd, d.jed, nk, m->u, m->v, m->h0, m->cor->dyCu, m->cor->dxCv, m->uh, m->vh, s0)); }
+    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->uh, nk}, {m->vh, nk}})); }
+    { StageTimer t(m, ST_HOR_VISC, s0); RC(mom6h_hor_visc(m->hv, m->u, m->v, m->h0, m->diffu, m->diffv, s0)); }
+    { StageTimer t(m, ST_CORIOLIS, s0); RC(mom6h_coradcalc(m->cor, m->u, m->v, m->h0, m->uh, m->vh, m->CAu, m->CAv, s0)); }
+    { StageTimer t(m, ST_VEL_UPDATE, s0);
+      RC(mom6h_vel_update(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk, dt, m->u, m->v, m->CAu,
+                          m->diffu, m->CAv, m->diffv, m->up, m->vp, s0)); }
+    { StageTimer t(m, ST_VERT_VISC, s0);
+      RC(mom6h_vert_visc(m->vv, m->up, m->vp, m->h0, dt, m->taux, m->tauy, rayu, rayv, s0)); }
+    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->up, nk}, {m->vp, nk}})); }
+    { StageTimer t(m, ST_CONTINUITY, s0);
+      RC(mom6h_continuity_ppm(m->cont, m->up, m->h0, m->htmp, m->uh, dt, nullptr, &m->bt, m->u_cor, m->du_cor, 1, s0)); }
+    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->htmp, nk}})); }
+
+    // ---- corrector (h = h_pred, held in htmp)
+    { StageTimer t(m, ST_TRANSPORTS, s0);
+      RC(mom6h_transports(d.isd, d.ied, d.jsd, d.jed, nk, m->up, m->vp, m->htmp, m->cor->dyCu, m->cor->dxCv, m->uh, m->vh, s0)); }
+    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->uh, nk}, {m->vh, nk}})); }
+    { StageTimer t(m, ST_HOR_VISC, s0); RC(mom6h_hor_visc(m->hv, m->up, m->vp, m->htmp, m->diffu, m->diffv, s0)); }
+    { StageTimer t(m, ST_CORIOLIS, s0); RC(mom6h_coradcalc(m->cor, m->up, m->vp, m->htmp, m->uh, m->vh, m->CAu, m->CAv, s0)); }
+    { StageTimer t(m, ST_VEL_UPDATE, s0);
+      RC(mom6h_vel_update(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk, 0.5 * dt, m->u, m->v, m->CAu,
+                          m->diffu, m->CAv, m->diffv, m->u, m->v, s0)); }
+    { StageTimer t(m, ST_VERT_VISC, s0);
+      RC(mom6h_vert_visc(m->vv, m->u, m->v, m->htmp, 0.5 * dt, m->taux, m->tauy, rayu, rayv, s0)); }
+    { StageTimer t(m, ST_CONTINUITY, s0);
+      RC(mom6h_continuity_ppm(m->cont, m->u, m->htmp, m->h, m->uh, 0.5 * dt, nullptr, &m->bt, m->u_cor, m->du_cor, 1, s0)); }
+    MOM6H_CHECK(hipStreamWaitEvent(s0, m->e_bt, 0));
+    { StageTimer t(m, ST_HALO, s0);
+      RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}, {m->eta, 1}, {m->ubt, 1}, {m->vbt, 1}})); }
+    return MOM6H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mom6h_model_create(mom6h_model **out, const mom6h_config *c)
+{
+    MOM6H_REQUIRE(out && c, "mom6h_model_create: NULL argument");
+    MOM6H_REQUIRE(c->ni >= 10 && c->nj >= 10 && c->nk >= 1, "mom6h_model_create: ni, nj >= 10 and nk >= 1 required");
+    MOM6H_REQUIRE(c->bt_nsteps >= 1 && c->dt > 0.0 && c->dx_km > 0.0, "mom6h_model_create: bad bt_nsteps/dt/dx");
+    MOM6H_REQUIRE(!(c->periodic_y && c->walls_y), "mom6h_model_create: periodic_y and walls_y exclude each other");
+    MOM6H_REQUIRE(c->coriolis_scheme >= 1 && c->coriolis_scheme <= 3, "mom6h_model_create: bad coriolis_scheme");
+    auto *m = new mom6h_model();
+    m->c = *c;
+    // mom6_types.F90:145-148
+    m->d = make_dims(1, c->ni + 2, 1, c->nj + 2, 4, c->ni - 1, 4, c->nj - 1, c->nk);
+    m->halo = Halo{c->periodic_x, c->periodic_y};
+    const size_t n2 = (size_t)m->d.s2, n3 = n2 * c->nk;
+    int rc = 0;
+#define F3(x) rc = rc ? rc : add_field(m, #x, &m->x, n3, true)
+#define F2(x) rc = rc ? rc : add_field(m, #x, &m->x, n2, true)
+    F3(u); F3(v); F3(h); F3(h0); F3(htmp); F3(uh); F3(vh); F3(CAu); F3(CAv); F3(up); F3(vp);
+    F3(diffu); F3(diffv); F3(u_cor);
+    F2(eta); F2(ubt); F2(vbt); F2(ubt_av); F2(vbt_av); F2(eta_av); F2(du_cor);
+    F2(taux); F2(tauy); F2(ray_u); F2(ray_v); F2(CoriolisBu); F2(mask2dT); F2(mask2dCu);
+    F2(mask2dCv); F2(mask2dBu);
+#undef F3
+#undef F2
+    if (!rc) rc = mom6h_bt_cont_alloc(&m->bt, 1, c->ni + 2, 1, c->nj + 2, c->nk);
+    if (!rc) {
+        m->fields.push_back(Field{"FA_u_EE", &m->bt.FA_u_EE, n2});
+        m->fields.push_back(Field{"FA_u_E0", &m->bt.FA_u_E0, n2});
+        m->fields.push_back(Field{"FA_u_W0", &m->bt.FA_u_W0, n2});
+        m->fields.push_back(Field{"FA_u_WW", &m->bt.FA_u_WW, n2});
+        m->fields.push_back(Field{"uBT_WW", &m->bt.uBT_WW, n2});
+        m->fields.push_back(Field{"uBT_EE", &m->bt.uBT_EE, n2});
+        m->fields.push_back(Field{"BT_h_u", &m->bt.h_u, n3});
+    }
+    if (!rc && hipStreamCreateWithFlags(&m->s0, hipStreamNonBlocking) != hipSuccess)
+        rc = set_error(MOM6H_ENOMEM, "stream creation failed");
+    if (!rc && hipStreamCreateWithFlags(&m->s1, hipStreamNonBlocking) != hipSuccess)
+        rc = set_error(MOM6H_ENOMEM, "stream creation failed");
+    if (!rc && hipEventCreateWithFlags(&m->e_start, hipEventDisableTiming) != hipSuccess)
+        rc = set_error(MOM6H_ENOMEM, "event creation failed");
+    if (!rc && hipEventCreateWithFlags(&m->e_bt, hipEventDisableTiming) != hipSuccess)
+        rc = set_error(MOM6H_ENOMEM, "event creation failed");
+    if (rc) {
+        mom6h_model_destroy(m);
+        return rc;
+    }
+    host_grid(m);
+    // Rayleigh drag in the bottom layer (rk2_serial_driver.F90:534-539) and
+    // the reference's default grid inputs; callers overwrite them by upload.
+    {
+        std::vector<double> one(n2, 1.0), ray(n2, 1.0e-4);
+        (void)hipMemcpy(m->ray_u, ray.data(), n2 * sizeof(double), hipMemcpyHostToDevice);
+        (void)hipMemcpy(m->ray_v, ray.data(), n2 * sizeof(double), hipMemcpyHostToDevice);
+        for (double *p : {m->mask2dT, m->mask2dCu, m->mask2dCv, m->mask2dBu})
+            (void)hipMemcpy(p, one.data(), n2 * sizeof(double), hipMemcpyHostToDevice);
+    }
+    *out = m;
+    return MOM6H_OK;
+}
+
+int mom6h_model_field(mom6h_model *m, const char *name, double **d_ptr, long *count)
+{
+    MOM6H_REQUIRE(m && name, "mom6h_model_field: NULL argument");
+    for (auto &f : m->fields)
+        if (f.name == name) {
+            if (d_ptr) *d_ptr = *f.p;
+            if (count) *count = (long)f.n;
+            return MOM6H_OK;
+        }
+    if (std::string(name) == "vv_taux_bot" || std::string(name) == "vv_tauy_bot" ||
+        std::string(name) == "vv_a_u" || std::string(name) == "vv_h_u") {
+        MOM6H_REQUIRE(m->vv, "mom6h_model_field: %s exists after prepare", name);
+        const size_t n2 = (size_t)m->d.s2;
+        double *p = std::string(name) == "vv_taux_bot" ? m->vv->taux_bot
+                    : std::string(name) == "vv_tauy_bot" ? m->vv->tauy_bot
+                    : std::string(name) == "vv_a_u" ? m->vv->a_u : m->vv->h_u;
+        const size_t n = std::string(name) == "vv_a_u" ? n2 * (m->c.nk + 1)
+                         : std::string(name) == "vv_h_u" ? n2 * m->c.nk : n2;
+        if (d_ptr) *d_ptr = p;
+        if (count) *count = (long)n;
+        return MOM6H_OK;
+    }
+    return set_error(MOM6H_EINVAL, "mom6h_model_field: unknown field '%s'", name);
+}
+
+int mom6h_model_upload(mom6h_model *m, const char *name, const double *src, long count)
+{
+    double *p = nullptr;
+    long n = 0;
+    RC(mom6h_model_field(m, name, &p, &n));
+    MOM6H_REQUIRE(src && count == n, "mom6h_model_upload(%s): expected %ld values, got %ld", name, n, count);
+    MOM6H_CHECK(hipDeviceSynchronize());
+    MOM6H_CHECK(hipMemcpy(p, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    return MOM6H_OK;
+}
+
+int mom6h_model_download(mom6h_model *m, const char *name, double *dst, long count)
+{
+    double *p = nullptr;
+    long n = 0;
+    RC(mom6h_model_field(m, name, &p, &n));
+    MOM6H_REQUIRE(dst && count == n, "mom6h_model_download(%s): expected %ld values, got %ld", name, n, count);
+    MOM6H_CHECK(hipDeviceSynchronize());
+    MOM6H_CHECK(hipMemcpy(dst, p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return MOM6H_OK;
+}
+
+int mom6h_model_prepare(mom6h_model *m)
+{
+    MOM6H_REQUIRE(m != nullptr, "mom6h_model_prepare: NULL model");
+    MOM6H_REQUIRE(!m->prepared, "mom6h_model_prepare: already prepared");
+    MOM6H_CHECK(hipDeviceSynchronize());
+    const Dims &d = m->d;
+    const size_t n2 = (size_t)d.s2;
+    const int nk = m->c.nk;
+    std::vector<double> mT = download(m->mask2dT, n2), mCu = download(m->mask2dCu, n2);
+    std::vector<double> mCv = download(m->mask2dCv, n2), mBu = download(m->mask2dBu, n2);
+    std::vector<double> fq = download(m->CoriolisBu, n2);
+    MOM6H_REQUIRE(!mT.empty() && !mCu.empty() && !mCv.empty() && !mBu.empty() && !fq.empty(),
+                  "mom6h_model_prepare: download of grid inputs failed");
+    // continuity_init (mom6_continuity_serial.F90:53-78)
+    RC(mom6h_continuity_init(&m->cont, d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk,
+                             m->IareaT.data(), m->IdxT.data(), m->dy_Cu.data(), mT.data(),
+                             m->c.monotonic, m->dxT.data(), m->dxCu.data(), mCu.data()));
+    // coriolis_init (mom6_coriolis_serial.F90:48-84)
+    RC(mom6h_coriolis_init(&m->cor, d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk,
+                           m->areaT.data(), m->IareaBu.data(), fq.data(), mBu.data(), m->dyCv.data(),
+                           m->dxCu.data(), m->dyCu.data(), m->dxCv.data(), m->IdxCu.data(),
+                           m->IdyCv.data(), m->c.coriolis_scheme));
+    // barotropic_init (mom6_barotropic_serial.F90:63-137); closed v faces carry no transport
+    {
+        std::vector<double> Datu(n2), Datv(n2), g(n2, kG_EARTH), rem(n2, 0.999), f4(4 * n2);
+        for (size_t q = 0; q < n2; ++q) {
+            Datu[q] = kDepth * m->dyCu[q];
+            Datv[q] = kDepth * m->dxCv[q];
+            if (m->c.walls_y) Datv[q] = Datv[q] * mCv[q];
+            for (int cc = 0; cc < 4; ++cc) f4[4 * q + cc] = 0.25 * fq[q];
+        }
+        RC(mom6h_barotropic_init(&m->bt_cs, d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec,
+                                 m->c.bt_nsteps, m->c.dt, 0.2, 0, Datu.data(), Datv.data(), g.data(),
+                                 g.data(), g.data(), g.data(), f4.data(), f4.data(), rem.data(),
+                                 rem.data(), m->IareaT.data(), m->IdxCu.data(), m->IdyCv.data(),
+                                 m->c.periodic_x, m->c.periodic_y));
+    }
+    // vert_visc_init (mom6_vert_visc_serial.F90:60-112)
+    RC(mom6h_vert_visc_init(&m->vv, d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk,
+                            mCu.data(), mCv.data(), m->c.Kv, m->c.Kv_ml, m->c.Kv_extra_bbl,
+                            m->c.Hmix, m->c.Hbbl));
+    // hor_visc_init (mom6_hor_visc_serial.F90:160-286)
+    {
+        std::vector<double> dx2h(n2), dy2h(n2), dx2q(n2), dy2q(n2), DX_dyT(n2), DY_dxT(n2);
+        std::vector<double> DX_dyBu(n2), DY_dxBu(n2), I2u(n2), Iu2(n2), I2v(n2), Iv2(n2), one(n2, 1.0);
+        for (size_t q = 0; q < n2; ++q) {
+            dx2h[q] = m->dxT[q] * m->dxT[q];
+            dy2h[q] = m->dyT[q] * m->dyT[q];
+            dx2q[q] = m->dxBu[q] * m->dxBu[q];
+            dy2q[q] = m->dyBu[q] * m->dyBu[q];
+            DX_dyT[q] = m->dxT[q] * m->IdyT[q];
+            DY_dxT[q] = m->dyT[q] * m->IdxT[q];
+            DX_dyBu[q] = m->dxBu[q] * m->IdyBu[q];
+            DY_dxBu[q] = m->dyBu[q] * m->IdxBu[q];
+            I2u[q] = m->IdxCu[q] * m->IdxCu[q] * m->IdyCu[q];
+            Iu2[q] = m->IdxCu[q] * m->IdyCu[q] * m->IdyCu[q];
+            I2v[q] = m->IdxCv[q] * m->IdxCv[q] * m->IdyCv[q];
+            Iv2[q] = m->IdxCv[q] * m->IdyCv[q] * m->IdyCv[q];
+        }
+        const double h_neglect = 1.0e-3;  // max(Angstrom_H, 1e-3) (:203)
+        RC(mom6h_hor_visc_init(&m->hv, d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk,
+                               m->c.Kh, h_neglect, DY_dxT.data(), DX_dyT.data(), DY_dxBu.data(),
+                               DX_dyBu.data(), m->IdyCu.data(), m->IdxCu.data(), m->IdyCv.data(),
+                               m->IdxCv.data(), m->IareaCu.data(), m->IareaCv.data(), mT.data(),
+                               mBu.data(), one.data(), one.data(), dy2h.data(), dx2h.data(),
+                               dy2q.data(), dx2q.data()));
+        RC(mom6h_hor_visc_set_options(m->hv, m->c.laplacian, m->c.biharmonic, m->c.Ah, m->c.no_slip,
+                                      I2u.data(), Iu2.data(), I2v.data(), Iv2.data()));
+    }
+    // periodic fill of the initial state (rk2_mpi_serial_driver.F90:191-197)
+    RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}, {m->h0, nk}, {m->eta, 1}, {m->ubt, 1}, {m->vbt, 1}}));
+    MOM6H_CHECK(hipStreamSynchronize(m->s0));
+    m->prepared = true;
+    return MOM6H_OK;
+}
+
+int mom6h_model_step(mom6h_model *m, int niter)
+{
+    MOM6H_REQUIRE(m != nullptr, "mom6h_model_step: NULL model");
+    MOM6H_REQUIRE(m->prepared, "mom6h_model_step: call mom6h_model_prepare first");
+    MOM6H_REQUIRE(niter >= 0, "mom6h_model_step: niter >= 0 required");
+    for (int it = 0; it < niter; ++it) RC(enqueue_step(m));
+    return MOM6H_OK;
+}
+
+int mom6h_model_synchronize(mom6h_model *m)
+{
+    MOM6H_REQUIRE(m != nullptr, "mom6h_model_synchronize: NULL model");
+    MOM6H_CHECK(hipStreamSynchronize(m->s1));
+    MOM6H_CHECK(hipStreamSynchronize(m->s0));
+    if (m->profile && !m->ev.empty()) {
+        for (size_t q = 0; q + 2 < m->ev.size(); q += 3) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, m->ev[q], m->ev[q + 1]) == hipSuccess)
+                m->stage_ms[(int)(intptr_t)m->ev[q + 2]] += ms;
+            (void)hipEventDestroy(m->ev[q]);
+            (void)hipEventDestroy(m->ev[q + 1]);
+        }
+        m->ev.clear();
+    }
+    return MOM6H_OK;
+}
+
+int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
+{
+    MOM6H_REQUIRE(m && key, "mom6h_model_set_option: NULL argument");
+    const std::string k(key);
+    if (k == "graph") m->use_graph = value != 0.0;
+    else if (k == "profile") {
+        m->profile = value != 0.0;
+        for (double &x : m->stage_ms) x = 0.0;
+    } else return set_error(MOM6H_EINVAL, "mom6h_model_set_option: unknown option '%s'", key);
+    return MOM6H_OK;
+}
+
+int mom6h_model_stage_times(mom6h_model *m, double *ms, int max_stages, char *names, int nbytes)
+{
+    MOM6H_REQUIRE(m != nullptr, "mom6h_model_stage_times: NULL model");
+    std::string all;
+    for (int s = 0; s < ST_COUNT; ++s) {
+        if (ms && s < max_stages) ms[s] = m->stage_ms[s];
+        if (s) all += ",";
+        all += kStageNames[s];
+    }
+    if (names && nbytes > 0) {
+        strncpy(names, all.c_str(), (size_t)nbytes - 1);
+        names[nbytes - 1] = 0;
+    }
+    return ST_COUNT;
+}
+
+int mom6h_model_main_stream(mom6h_model *m, void **stream)
+{
+    MOM6H_REQUIRE(m && stream, "mom6h_model_main_stream: NULL argument");
+    *stream = (void *)m->s0;
+    return MOM6H_OK;
+}
+
+int mom6h_model_destroy(mom6h_model *m)
+{
+    if (!m) return MOM6H_OK;
+    if (m->s0) (void)hipStreamSynchronize(m->s0);
+    if (m->s1) (void)hipStreamSynchronize(m->s1);
+    mom6h_continuity_end(m->cont);
+    mom6h_coriolis_end(m->cor);
+    mom6h_barotropic_end(m->bt_cs);
+    mom6h_vert_visc_end(m->vv);
+    mom6h_hor_visc_end(m->hv);
+    for (auto &f : m->fields)
+        if (f.name.rfind("FA_", 0) != 0 && f.name.rfind("uBT", 0) != 0 && f.name != "BT_h_u") dev_free(*f.p);
+    mom6h_bt_cont_free(&m->bt);
+    if (m->e_start) (void)hipEventDestroy(m->e_start);
+    if (m->e_bt) (void)hipEventDestroy(m->e_bt);
+    if (m->s0) (void)hipStreamDestroy(m->s0);
+    if (m->s1) (void)hipStreamDestroy(m->s1);
+    delete m;
+    return MOM6H_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,256 @@
+// mom6h_runtime.hip -- runtime helpers, error reporting and the driver's
+// inline loops (transports, velocity updates, periodic halo refill).
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "mom6h_internal.h"
+
+namespace mom6h {
+
+static thread_local char g_err[1024] = "";
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int dev_alloc(double **p, size_t count, bool zero)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(double));
+    if (e != hipSuccess)
+        return set_error(MOM6H_ENOMEM, "hipMalloc(%zu doubles): %s", count, hipGetErrorString(e));
+    if (zero) MOM6H_CHECK(hipMemset(*p, 0, count * sizeof(double)));
+    return MOM6H_OK;
+}
+
+int dev_upload(double **p, const double *host, size_t count)
+{
+    MOM6H_REQUIRE(host != nullptr, "missing host array");
+    int rc = dev_alloc(p, count, false);
+    if (rc) return rc;
+    MOM6H_CHECK(hipMemcpy(*p, host, count * sizeof(double), hipMemcpyHostToDevice));
+    return MOM6H_OK;
+}
+
+void dev_free(double *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------- halo fill
+// One thread per (halo cell, k).  x-fill: columns isd..isc-1 and iec+1..ied of
+// every row; y-fill: rows jsd..jsc-1 and jec+1..jed of every column, run after
+// the x-fill so corners take their x-filled source.
+__global__ void k_halo_x(Dims d, double *a, int nk)
+{
+    const int hw_w = d.isc - d.isd, hw_e = d.ied - d.iec, hw = hw_w + hw_e;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per_k = (long long)hw * d.n2;
+    if (t >= per_k * nk) return;
+    const int k = (int)(t / per_k) + 1;
+    const int r = (int)(t % per_k);
+    const int j = d.jsd + r / hw;
+    const int c = r % hw;
+    const int P = d.iec - d.isc + 1;
+    const int i = (c < hw_w) ? d.isd + c : d.iec + 1 + (c - hw_w);
+    const int src = (c < hw_w) ? i + P : i - P;
+    a[IX3(d, i, j, k)] = a[IX3(d, src, j, k)];
+}
+
+__global__ void k_halo_y(Dims d, double *a, int nk)
+{
+    const int hs = d.jsc - d.jsd, hn = d.jed - d.jec, hw = hs + hn;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per_k = (long long)hw * d.n1;
+    if (t >= per_k * nk) return;
+    const int k = (int)(t / per_k) + 1;
+    const int r = (int)(t % per_k);
+    const int i = d.isd + r % d.n1;
+    const int c = r / d.n1;
+    const int Q = d.jec - d.jsc + 1;
+    const int j = (c < hs) ? d.jsd + c : d.jec + 1 + (c - hs);
+    const int src = (c < hs) ? j + Q : j - Q;
+    a[IX3(d, i, j, k)] = a[IX3(d, i, src, k)];
+}
+
+int halo_fill(const Dims &d, const Halo &hm, double *a, int nk, hipStream_t s)
+{
+    const int bs = 256;
+    if (hm.px) {
+        const long long n = (long long)((d.isc - d.isd) + (d.ied - d.iec)) * d.n2 * nk;
+        if (n > 0) {
+            k_halo_x<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(d, a, nk);
+            MOM6H_LAUNCH_CHECK();
+        }
+    }
+    if (hm.py) {
+        const long long n = (long long)((d.jsc - d.jsd) + (d.jed - d.jec)) * d.n1 * nk;
+        if (n > 0) {
+            k_halo_y<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(d, a, nk);
+            MOM6H_LAUNCH_CHECK();
+        }
+    }
+    return MOM6H_OK;
+}
+
+// --------------------------------------------------------------- transports
+// compute_transports, app/rk2_serial_driver.F90:544-561:
+//   uh = u*0.5*(h + h(min(i+1,ied)))*dyCu ; vh = v*0.5*(h + h(i,min(j+1,jed)))*dxCv
+__global__ void k_transports(Dims d, const double *__restrict__ u, const double *__restrict__ v,
+                             const double *__restrict__ h, const double *__restrict__ dyCu,
+                             const double *__restrict__ dxCv, double *__restrict__ uh,
+                             double *__restrict__ vh)
+{
+    const int i = d.isd + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.jsd + blockIdx.y;
+    const int k = 1 + blockIdx.z;
+    if (i > d.ied) return;
+    const int ip = i + 1 < d.ied ? i + 1 : d.ied;
+    const int jp = j + 1 < d.jed ? j + 1 : d.jed;
+    const long long c = IX3(d, i, j, k);
+    const double hc = h[c];
+    uh[c] = u[c] * 0.5 * (hc + h[IX3(d, ip, j, k)]) * dyCu[IX2(d, i, j)];
+    vh[c] = v[c] * 0.5 * (hc + h[IX3(d, i, jp, k)]) * dxCv[IX2(d, i, j)];
+}
+
+// ---------------------------------------------------------- velocity update
+// app/rk2_serial_driver.F90:257-270 (fac_dt = dt) and :337-350 (0.5*dt):
+// ou(i,j,k) = bu + fac_dt*(CAu + diffu) for i in isc..iec-1, j in jsc..jec;
+// ov likewise for i in isc..iec, j in jsc..jec-1.
+__global__ void k_vel_update(Dims d, double fdt, const double *bu, const double *bv,
+                             const double *__restrict__ CAu, const double *__restrict__ diffu,
+                             const double *__restrict__ CAv, const double *__restrict__ diffv,
+                             double *ou, double *ov)
+{
+    const int i = d.isc + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.jsc + blockIdx.y;
+    const int k = 1 + blockIdx.z;
+    if (i > d.iec) return;
+    const long long c = IX3(d, i, j, k);
+    if (i <= d.iec - 1) ou[c] = bu[c] + fdt * (CAu[c] + diffu[c]);
+    if (j <= d.jec - 1) ov[c] = bv[c] + fdt * (CAv[c] + diffv[c]);
+}
+
+}  // namespace mom6h
+
+using namespace mom6h;
+
+extern "C" {
+
+const char *mom6h_last_error(void) { return g_err; }
+const char *mom6h_version(void) { return "mom6hip 0.1 (gfx950)"; }
+
+int mom6h_set_device(int device)
+{
+    MOM6H_CHECK(hipSetDevice(device));
+    return MOM6H_OK;
+}
+int mom6h_device_synchronize(void)
+{
+    MOM6H_CHECK(hipDeviceSynchronize());
+    return MOM6H_OK;
+}
+int mom6h_malloc(void **p, size_t bytes)
+{
+    MOM6H_REQUIRE(p != nullptr, "mom6h_malloc: NULL out pointer");
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess)
+        return set_error(MOM6H_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    return MOM6H_OK;
+}
+int mom6h_free(void *p)
+{
+    if (p) MOM6H_CHECK(hipFree(p));
+    return MOM6H_OK;
+}
+int mom6h_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    MOM6H_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MOM6H_OK;
+}
+int mom6h_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    MOM6H_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return MOM6H_OK;
+}
+int mom6h_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    MOM6H_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return MOM6H_OK;
+}
+int mom6h_memset(void *p, int value, size_t bytes)
+{
+    MOM6H_CHECK(hipMemset(p, value, bytes));
+    return MOM6H_OK;
+}
+int mom6h_stream_create(void **stream)
+{
+    MOM6H_REQUIRE(stream != nullptr, "mom6h_stream_create: NULL out pointer");
+    hipStream_t s;
+    MOM6H_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void *)s;
+    return MOM6H_OK;
+}
+int mom6h_stream_destroy(void *stream)
+{
+    if (stream) MOM6H_CHECK(hipStreamDestroy((hipStream_t)stream));
+    return MOM6H_OK;
+}
+int mom6h_stream_synchronize(void *stream)
+{
+    MOM6H_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    return MOM6H_OK;
+}
+
+int mom6h_transports(int isd, int ied, int jsd, int jed, int nk, const double *u, const double *v,
+                     const double *h, const double *dyCu, const double *dxCv, double *uh,
+                     double *vh, void *stream)
+{
+    MOM6H_REQUIRE(ied >= isd && jed >= jsd && nk >= 1, "mom6h_transports: bad bounds");
+    MOM6H_REQUIRE(u && v && h && dyCu && dxCv && uh && vh, "mom6h_transports: NULL array");
+    Dims d = make_dims(isd, ied, jsd, jed, isd, ied, jsd, jed, nk);
+    dim3 b(128), g((d.n1 + 127) / 128, d.n2, nk);
+    k_transports<<<g, b, 0, (hipStream_t)stream>>>(d, u, v, h, dyCu, dxCv, uh, vh);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
+}
+
+int mom6h_vel_update(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk,
+                     double fac_dt, const double *bu, const double *bv, const double *CAu,
+                     const double *diffu, const double *CAv, const double *diffv, double *ou,
+                     double *ov, void *stream)
+{
+    MOM6H_REQUIRE(isd <= isc && isc <= iec && iec <= ied && jsd <= jsc && jsc <= jec && jec <= jed && nk >= 1,
+                  "mom6h_vel_update: bad bounds");
+    MOM6H_REQUIRE(bu && bv && CAu && diffu && CAv && diffv && ou && ov, "mom6h_vel_update: NULL array");
+    Dims d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, nk);
+    const int ni = iec - isc + 1, nj = jec - jsc + 1;
+    dim3 b(128), g((ni + 127) / 128, nj, nk);
+    k_vel_update<<<g, b, 0, (hipStream_t)stream>>>(d, fac_dt, bu, bv, CAu, diffu, CAv, diffv, ou, ov);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
+}
+
+int mom6h_halo_fill(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk,
+                    int periodic_x, int periodic_y, double *a, void *stream)
+{
+    MOM6H_REQUIRE(a != nullptr, "mom6h_halo_fill: NULL array");
+    MOM6H_REQUIRE(isd <= isc && isc <= iec && iec <= ied && jsd <= jsc && jsc <= jec && jec <= jed && nk >= 1,
+                  "mom6h_halo_fill: bad bounds");
+    MOM6H_REQUIRE(!periodic_x || (iec - isc + 1) >= (isc - isd) && (iec - isc + 1) >= (ied - iec),
+                  "mom6h_halo_fill: compute domain narrower than the x halo");
+    MOM6H_REQUIRE(!periodic_y || (jec - jsc + 1) >= (jsc - jsd) && (jec - jsc + 1) >= (jed - jec),
+                  "mom6h_halo_fill: compute domain narrower than the y halo");
+    Dims d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, nk);
+    Halo hm{periodic_x, periodic_y};
+    return halo_fill(d, hm, a, nk, (hipStream_t)stream);
+}
+
+}  // extern "C"

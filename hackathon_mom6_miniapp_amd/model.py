@@ -1,0 +1,100 @@
+"""Host-side driver of one split-RK2 run on the GPU (the Python counterpart of
+app/rk2_cuda_c_driver.F90): builds the device model from a Config and a case,
+advances it with mom6h_model_step and reads fields back."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import MConfig, check, lib
+from .state import BC_PERIODIC, BC_WALL, Config
+
+# inputs uploaded before mom6h_model_prepare
+_UPLOAD = ("u", "v", "h0", "eta", "ubt", "vbt", "taux", "tauy", "CoriolisBu",
+           "mask2dT", "mask2dCu", "mask2dCv", "mask2dBu")
+
+
+def to_mconfig(cfg: Config) -> MConfig:
+    c = MConfig()
+    c.ni, c.nj, c.nk, c.bt_nsteps = cfg.ni, cfg.nj, cfg.nk, cfg.bt_nsteps
+    c.dt, c.dx_km = cfg.dt, cfg.dx_km
+    c.periodic_x = int(cfg.bc_x == BC_PERIODIC)
+    c.periodic_y = int(cfg.bc_y == BC_PERIODIC)
+    c.walls_y = int(cfg.bc_y == BC_WALL)
+    c.Kh, c.Ah = cfg.Kh, cfg.Ah
+    c.laplacian, c.biharmonic, c.no_slip = int(cfg.laplacian), int(cfg.biharmonic), 0
+    c.Kv, c.Kv_ml, c.Kv_extra_bbl, c.Hmix, c.Hbbl = cfg.Kv, cfg.Kv_ml, cfg.Kv_extra_bbl, cfg.Hmix, cfg.Hbbl
+    c.coriolis_scheme, c.monotonic = cfg.coriolis_scheme, int(cfg.monotonic)
+    return c
+
+
+class Model:
+    def __init__(self, cfg: Config, case: dict, graph: bool = False):
+        self.cfg = cfg
+        self.L = lib()
+        self.m = C.c_void_p()
+        mc = to_mconfig(cfg)
+        check(self.L.mom6h_model_create(C.byref(self.m), C.byref(mc)), "mom6h_model_create")
+        for name in _UPLOAD:
+            self.upload(name, case[name])
+        self.upload("h", case["h0"])
+        ray = np.full((cfg.n1, cfg.n2), cfg.ray_bottom, order="F")
+        self.upload("ray_u", ray)
+        self.upload("ray_v", ray)
+        check(self.L.mom6h_model_set_option(self.m, b"graph", float(graph)), "set_option")
+        check(self.L.mom6h_model_prepare(self.m), "mom6h_model_prepare")
+
+    def close(self):
+        if self.m:
+            self.L.mom6h_model_destroy(self.m)
+            self.m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _count(self, name: str) -> int:
+        n = C.c_long()
+        check(self.L.mom6h_model_field(self.m, name.encode(), None, C.byref(n)), f"field {name}")
+        return n.value
+
+    def device_ptr(self, name: str) -> int:
+        p, n = C.c_void_p(), C.c_long()
+        check(self.L.mom6h_model_field(self.m, name.encode(), C.byref(p), C.byref(n)), f"field {name}")
+        return p.value
+
+    def upload(self, name: str, arr):
+        a = np.asfortranarray(np.asarray(arr, dtype=np.float64))
+        check(self.L.mom6h_model_upload(self.m, name.encode(), a.ctypes.data, a.size), f"upload {name}")
+
+    def get(self, name: str) -> np.ndarray:
+        n = self._count(name)
+        out = np.empty(n, dtype=np.float64)
+        check(self.L.mom6h_model_download(self.m, name.encode(), out.ctypes.data, n), f"download {name}")
+        n2 = self.cfg.n1 * self.cfg.n2
+        shape = (self.cfg.n1, self.cfg.n2) if n == n2 else (self.cfg.n1, self.cfg.n2, n // n2)
+        return out.reshape(shape, order="F")
+
+    def step(self, niter: int = 1):
+        check(self.L.mom6h_model_step(self.m, niter), "mom6h_model_step")
+
+    def synchronize(self):
+        check(self.L.mom6h_model_synchronize(self.m), "mom6h_model_synchronize")
+
+    def set_option(self, key: str, value: float):
+        check(self.L.mom6h_model_set_option(self.m, key.encode(), float(value)), f"option {key}")
+
+    def stage_times(self) -> dict:
+        ms = (C.c_double * 16)()
+        names = C.create_string_buffer(512)
+        n = self.L.mom6h_model_stage_times(self.m, ms, 16, names, 512)
+        keys = names.value.decode().split(",")
+        return {keys[i]: ms[i] for i in range(n)}
+
+    def main_stream(self) -> int:
+        s = C.c_void_p()
+        check(self.L.mom6h_model_main_stream(self.m, C.byref(s)), "main_stream")
+        return s.value or 0

@@ -1,0 +1,221 @@
+/*
+ * mom6hip.h -- C ABI of the gfx950 (MI355X) split-RK2 dynamical core.
+ *
+ * Drop-in replacement for the reference's src/cuda_c layer.  Each module keeps
+ * the control-structure pattern of the cuda_c Fortran wrappers (init copies the
+ * 2D metrics to the device, compute takes device pointers, end frees), so a
+ * Fortran iso_c_binding module can expose the same subroutines
+ * (continuity_init_cuda_c -> mom6h_continuity_init, ...).  The kernels follow
+ * the SERIAL reference semantics (the src/serial modules), not the cuda_c ones.
+ *
+ * Conventions
+ *  - Arrays: double, Fortran column-major over the data domain isd..ied x
+ *    jsd..jed (x 1..nk for 3D).  Bounds are 1-based and inclusive.
+ *  - Pointers named d_* (and every array argument of a compute call) are device
+ *    pointers; init calls take host arrays (h_*) and copy them.
+ *  - `stream` is a hipStream_t (NULL = default stream).  Compute calls are
+ *    asynchronous on it; they never synchronise the device.
+ *  - Every function returns 0 (MOM6H_OK) or an error code; the last error text
+ *    is available from mom6h_last_error().  The reference's launch_* entry
+ *    points return nothing and never check launches; these do.
+ */
+#ifndef MOM6HIP_H
+#define MOM6HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOM6H_OK 0
+#define MOM6H_EINVAL 1001      /* bad argument (bounds, NULL, unknown name) */
+#define MOM6H_ENOMEM 1002      /* device allocation failed */
+#define MOM6H_ESTATE 1003      /* call out of order (e.g. step before prepare) */
+#define MOM6H_EBARRIER 1004    /* persistent kernel grid barrier timed out */
+
+const char *mom6h_last_error(void);
+const char *mom6h_version(void);
+
+/* ---------------------------------------------------------------- runtime --
+ * Replace cuda_{set_device,device_synchronize,malloc,free,memcpy_*,memset}_c
+ * (src/cuda_c/cuda_helpers.h:10-17, cuda_helpers.cu:8-48). */
+int mom6h_set_device(int device);
+int mom6h_device_synchronize(void);
+int mom6h_malloc(void **d_ptr, size_t bytes);
+int mom6h_free(void *d_ptr);
+int mom6h_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
+int mom6h_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
+int mom6h_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
+int mom6h_memset(void *d_ptr, int value, size_t bytes);
+int mom6h_stream_create(void **stream);           /* default priority */
+int mom6h_stream_destroy(void *stream);
+int mom6h_stream_synchronize(void *stream);
+
+/* ------------------------------------------------------------ continuity --
+ * continuity_init_cuda_c (src/cuda_c/mom6_continuity_cuda_c.F90:269-340),
+ * continuity_PPM_cuda_c (:444-617), continuity_end_cuda_c (:345-384);
+ * arithmetic of src/serial/mom6_continuity_serial.F90:100-1352.
+ * dxT, dxCu and mask2dCu are required here (optional in cuda_c). */
+typedef struct mom6h_continuity_cs mom6h_continuity_cs;
+typedef struct mom6h_bt_cont {
+    /* BT_cont_type (src/common/mom6_types.F90:89-110): device arrays */
+    double *FA_u_EE, *FA_u_E0, *FA_u_W0, *FA_u_WW, *uBT_WW, *uBT_EE; /* 2D */
+    double *h_u;                                                       /* 3D */
+} mom6h_bt_cont;
+
+int mom6h_continuity_init(mom6h_continuity_cs **cs, int isd, int ied, int jsd, int jed,
+                          int isc, int iec, int jsc, int jec, int nk,
+                          const double *h_IareaT, const double *h_IdxT, const double *h_dy_Cu,
+                          const double *h_mask2dT, int monotonic, const double *h_dxT,
+                          const double *h_dxCu, const double *h_mask2dCu);
+/* u, hin -> h (compute domain), uh; uhbt (2D, NULL = zero transport target),
+ * bt_cont (NULL = skip), u_cor (3D) and du_cor (2D) optional (NULL = not stored).
+ * When hin_halo_to_h is non-zero, h outside the compute domain is set to hin
+ * (the driver's h reset fused into the call); otherwise it is left untouched. */
+int mom6h_continuity_ppm(mom6h_continuity_cs *cs, const double *u, const double *hin, double *h,
+                         double *uh, double dt, const double *uhbt, const mom6h_bt_cont *bt_cont,
+                         double *u_cor, double *du_cor, int hin_halo_to_h, void *stream);
+int mom6h_continuity_end(mom6h_continuity_cs *cs);
+/* alloc_BT_cont_type_cuda_c / dealloc (:389-435): zeroed device arrays */
+int mom6h_bt_cont_alloc(mom6h_bt_cont *bt, int isd, int ied, int jsd, int jed, int nk);
+int mom6h_bt_cont_free(mom6h_bt_cont *bt);
+
+/* -------------------------------------------------------------- coriolis --
+ * coriolis_init_cuda_c (src/cuda_c/mom6_coriolis_cuda_c.F90:164-227),
+ * CorAdCalc_cuda_c (:270-356); arithmetic of
+ * src/serial/mom6_coriolis_serial.F90:112-252.  scheme: 1 Sadourny75 energy,
+ * 2 Arakawa-Hsu90, 3 Arakawa-Lamb81. */
+typedef struct mom6h_coriolis_cs mom6h_coriolis_cs;
+int mom6h_coriolis_init(mom6h_coriolis_cs **cs, int isd, int ied, int jsd, int jed,
+                        int isc, int iec, int jsc, int jec, int nk,
+                        const double *h_areaT, const double *h_IareaBu, const double *h_CoriolisBu,
+                        const double *h_mask2dBu, const double *h_dyCv, const double *h_dxCu,
+                        const double *h_dyCu, const double *h_dxCv, const double *h_IdxCu,
+                        const double *h_IdyCv, int scheme);
+int mom6h_coradcalc(mom6h_coriolis_cs *cs, const double *u, const double *v, const double *h,
+                    const double *uh, const double *vh, double *CAu, double *CAv, void *stream);
+int mom6h_coriolis_end(mom6h_coriolis_cs *cs);
+
+/* ------------------------------------------------------------ barotropic --
+ * barotropic_init_cuda_c (src/cuda_c/mom6_barotropic_cuda_c.F90:226-300),
+ * btstep_cuda_c (:305-433), barotropic_end_cuda_c (:629-669); arithmetic of
+ * src/serial/mom6_barotropic_serial.F90:178-356.  The whole substep loop runs
+ * in ONE persistent kernel (grid-wide barriers between dependent phases)
+ * instead of 9 launches per substep.  periodic_x/periodic_y select the
+ * split-API ranges (btstep_do_step, :379-474) with a halo refill of
+ * ubt, vbt, eta after every substep but the last. */
+typedef struct mom6h_barotropic_cs mom6h_barotropic_cs;
+int mom6h_barotropic_init(mom6h_barotropic_cs **cs, int isd, int ied, int jsd, int jed,
+                          int isc, int iec, int jsc, int jec, int nstep, double dt, double bebt,
+                          int first_direction, const double *h_Datu, const double *h_Datv,
+                          const double *h_gtot_E, const double *h_gtot_W, const double *h_gtot_N,
+                          const double *h_gtot_S, const double *h_f_4_u, const double *h_f_4_v,
+                          const double *h_bt_rem_u, const double *h_bt_rem_v,
+                          const double *h_IareaT, const double *h_IdxCu, const double *h_IdyCv,
+                          int periodic_x, int periodic_y);
+/* eta_in/ubt_in/vbt_in may alias u_av/v_av/eta_av (the driver's corrector call). */
+int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ubt_in,
+                 const double *vbt_in, double *u_av, double *v_av, double *eta_av, void *stream);
+int mom6h_barotropic_end(mom6h_barotropic_cs *cs);
+
+/* ------------------------------------------------------------- vert_visc --
+ * vert_visc_init_cuda_c (src/cuda_c/mom6_vert_visc_cuda_c.F90:89-125),
+ * vert_visc_cra_cuda_c (:130-188); arithmetic of
+ * src/serial/mom6_vert_visc_serial.F90:138-287 (coef) and :394-506 (apply),
+ * including the Rayleigh drag the cuda_c path lacks.  ray_u_bot/ray_v_bot are
+ * 2D Rayleigh coefficients of the bottom layer (NULL = none). */
+typedef struct mom6h_vert_visc_cs mom6h_vert_visc_cs;
+int mom6h_vert_visc_init(mom6h_vert_visc_cs **cs, int isd, int ied, int jsd, int jed,
+                         int isc, int iec, int jsc, int jec, int nk, const double *h_mask2dCu,
+                         const double *h_mask2dCv, double Kv, double Kv_ml, double Kv_extra_bbl,
+                         double Hmix, double Hbbl);
+int mom6h_vert_visc(mom6h_vert_visc_cs *cs, double *u, double *v, const double *h, double dt,
+                    const double *taux, const double *tauy, const double *ray_u_bot,
+                    const double *ray_v_bot, void *stream);
+/* device pointers of the coefficient arrays (a_u/a_v: nk+1 levels) and bottom stress */
+int mom6h_vert_visc_arrays(mom6h_vert_visc_cs *cs, double **a_u, double **a_v, double **h_u,
+                           double **h_v, double **taux_bot, double **tauy_bot);
+int mom6h_vert_visc_end(mom6h_vert_visc_cs *cs);
+
+/* -------------------------------------------------------------- hor_visc --
+ * hor_visc_init_cuda_c (src/cuda_c/mom6_hor_visc_cuda_c.F90:117-190),
+ * hor_visc_cuda_c (:237-298); arithmetic of
+ * src/serial/mom6_hor_visc_serial.F90:464-1109 (Laplacian, biharmonic). */
+typedef struct mom6h_hor_visc_cs mom6h_hor_visc_cs;
+int mom6h_hor_visc_init(mom6h_hor_visc_cs **cs, int isd, int ied, int jsd, int jed,
+                        int isc, int iec, int jsc, int jec, int nz, double Kh, double h_neglect,
+                        const double *h_DY_dxT, const double *h_DX_dyT, const double *h_DY_dxBu,
+                        const double *h_DX_dyBu, const double *h_IdyCu, const double *h_IdxCu,
+                        const double *h_IdyCv, const double *h_IdxCv, const double *h_IareaCu,
+                        const double *h_IareaCv, const double *h_mask2dT, const double *h_mask2dBu,
+                        const double *h_reduction_xx, const double *h_reduction_xy,
+                        const double *h_dy2h, const double *h_dx2h, const double *h_dy2q,
+                        const double *h_dx2q);
+/* Enables the biharmonic term (hor_visc_init biharmonic=.true., Ah; :160-286)
+ * and selects free-slip (0) or no-slip (1) boundaries; laplacian=0 drops Kh. */
+int mom6h_hor_visc_set_options(mom6h_hor_visc_cs *cs, int laplacian, int biharmonic, double Ah,
+                               int no_slip, const double *h_Idx2dyCu, const double *h_Idxdy2u,
+                               const double *h_Idx2dyCv, const double *h_Idxdy2v);
+int mom6h_hor_visc(mom6h_hor_visc_cs *cs, const double *u, const double *v, const double *h,
+                   double *diffu, double *diffv, void *stream);
+int mom6h_hor_visc_end(mom6h_hor_visc_cs *cs);
+
+/* ------------------------------------------------------- driver kernels --
+ * The rk2 drivers' inline loops (app/rk2_serial_driver.F90). */
+/* compute_transports (:544-561), full data domain */
+int mom6h_transports(int isd, int ied, int jsd, int jed, int nk, const double *u, const double *v,
+                     const double *h, const double *dyCu, const double *dxCv, double *uh,
+                     double *vh, void *stream);
+/* VelUpdate (:257-270, :337-350): ou = bu + fac_dt*(CAu+diffu) on the
+ * reference ranges (ou may alias bu). */
+int mom6h_vel_update(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk,
+                     double fac_dt, const double *bu, const double *bv, const double *CAu,
+                     const double *diffu, const double *CAv, const double *diffv, double *ou,
+                     double *ov, void *stream);
+/* Periodic refill of the 3-wide halo (the on-device replacement of
+ * halo_exchange_3d/2d, src/common/mom6_mpi_halo.F90) */
+int mom6h_halo_fill(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk,
+                    int periodic_x, int periodic_y, double *a, void *stream);
+
+/* ------------------------------------------------------------ the model --
+ * A whole run of app/rk2_serial_driver.F90 on one GPU: grid, control
+ * structures and state on the device; mom6h_model_step advances niter RK2
+ * steps (:218-399) with every module on the GPU and the barotropic chain on a
+ * second stream.  Fields are addressed by the driver's names: u v h h0 htmp
+ * uh vh CAu CAv up vp diffu diffv eta ubt vbt ubt_av vbt_av eta_av u_cor du_cor
+ * taux tauy ray_u ray_v CoriolisBu mask2dT mask2dCu mask2dCv mask2dBu, and the
+ * BT_cont fields FA_u_EE FA_u_E0 FA_u_W0 FA_u_WW uBT_WW uBT_EE BT_h_u. */
+typedef struct mom6h_config {
+    int ni, nj, nk, bt_nsteps;
+    double dt, dx_km;
+    int periodic_x, periodic_y, walls_y;
+    double Kh, Ah;
+    int laplacian, biharmonic, no_slip;
+    double Kv, Kv_ml, Kv_extra_bbl, Hmix, Hbbl;
+    int coriolis_scheme, monotonic;
+} mom6h_config;
+
+typedef struct mom6h_model mom6h_model;
+int mom6h_model_create(mom6h_model **m, const mom6h_config *c);
+int mom6h_model_field(mom6h_model *m, const char *name, double **d_ptr, long *count);
+int mom6h_model_upload(mom6h_model *m, const char *name, const double *h_src, long count);
+int mom6h_model_download(mom6h_model *m, const char *name, double *h_dst, long count);
+/* after the inputs are uploaded: barotropic coefficients from CoriolisBu,
+ * wall masks, periodic fill of the initial state */
+int mom6h_model_prepare(mom6h_model *m);
+int mom6h_model_step(mom6h_model *m, int niter);
+int mom6h_model_synchronize(mom6h_model *m);
+/* Options: "graph" (1 = replay a captured hipGraph per step, default 1),
+ * "profile" (1 = record per-stage HIP events; disables the graph). */
+int mom6h_model_set_option(mom6h_model *m, const char *key, double value);
+/* Per-stage GPU time (ms) accumulated while "profile" is on; names are
+ * written comma-separated into names (size nbytes). Returns the stage count. */
+int mom6h_model_stage_times(mom6h_model *m, double *ms, int max_stages, char *names, int nbytes);
+int mom6h_model_main_stream(mom6h_model *m, void **stream);
+int mom6h_model_destroy(mom6h_model *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOM6HIP_H */

@@ -1,0 +1,26 @@
+"""Configurations shared by the parity tests (small enough for the oracle to
+finish in well under a second each)."""
+from hackathon_mom6_miniapp_amd.state import BC_PERIODIC, BC_STATIC, BC_WALL, Config
+
+# name -> (Config, init, niter)
+CASES = {
+    "static_ref_16": (Config(ni=16, nj=16, nk=5, bt_nsteps=10), "reference", 2),
+    "static_seeded_ragged": (Config(ni=22, nj=15, nk=6, bt_nsteps=12), "seeded", 3),
+    "doubly_periodic_fplane": (Config(ni=24, nj=24, nk=8, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_PERIODIC,
+                                      coriolis="fplane", f0=1e-4), "seeded", 3),
+    "zonal_periodic_walls_beta": (Config(ni=28, nj=20, nk=7, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                         coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3), "seeded", 3),
+    "biharmonic_walls": (Config(ni=24, nj=24, nk=6, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, biharmonic=True, Ah=1e11),
+                         "seeded", 2),
+    "cw84_limiter": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, monotonic=True), "seeded", 2),
+    "arakawa_hsu90": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, coriolis_scheme=2), "reference", 2),
+    "arakawa_lamb81": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, coriolis_scheme=3), "seeded", 2),
+    "deep_column_64": (Config(ni=64, nj=48, nk=30, bt_nsteps=30), "reference", 2),
+}
+
+# fields compared after the run: every array the reference driver holds
+COMPARE = ("u", "v", "h", "htmp", "up", "vp", "uh", "vh", "CAu", "CAv", "diffu", "diffv",
+           "eta", "ubt_av", "vbt_av", "eta_av", "u_cor", "du_cor",
+           "FA_u_EE", "FA_u_E0", "FA_u_W0", "FA_u_WW", "uBT_WW", "uBT_EE", "BT_h_u",
+           "vv_taux_bot", "vv_tauy_bot")
