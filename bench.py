@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the split-RK2 dynamical-core step on one MI355X.
+
+Workload (the FP64 drop-in configuration): 720x720x75 cells, bt_nsteps=50,
+dt=300 s, dx=10 km, beta-plane (f0=1e-4 s^-1, beta=2e-11 m^-1 s^-1), zonally
+periodic with solid walls at the southern and northern compute rows, Laplacian
+viscosity Kh=1e3 m^2/s, Kv=1e-4 m^2/s, linear bottom drag 1e-4 m/s, seeded
+state h = 4000/75 + N(0, 0.5) m, u, v ~ N(0, 0.05) m/s, eta = 0.
+
+A step is one RK2 step (app/rk2_serial_driver.F90:218-399) on device-resident
+state.  value = ni*nj*nk / t_step (cell-updates/s), t_step from the wall clock
+between device synchronisations around exactly --steps steps.
+
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0          # MI355X HBM3E
+METRIC = "RK2 cell-updates/s @720x720x75, FP64 vs mixed/batched; achieved HBM GB/s"
+
+
+def workload_config(ni=720, nj=720, nk=75, bt=50):
+    from hackathon_mom6_miniapp_amd.state import BC_PERIODIC, BC_WALL, Config
+    return Config(ni=ni, nj=nj, nk=nk, niter=10, bt_nsteps=bt, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                  coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, Kv=1e-4, ray_bottom=1e-4)
+
+
+def step_bytes(cfg) -> float:
+    """Algorithmic HBM bytes of one FP64 RK2 step: every module call reads its
+    3D/2D state inputs once and writes its outputs once (74 3D + 32 2D arrays
+    per step; static metrics excluded)."""
+    n2 = cfg.n1 * cfg.n2
+    return 8.0 * (74.0 * n2 * cfg.nk + 32.0 * n2)
+
+
+# Algorithmic bytes of ONE call of each stage (3D arrays x 8 B x n1*n2*nk +
+# 2D arrays x 8 B x n1*n2), following the per-stage counts of the step total.
+STAGE_ARRAYS = {            # (3D, 2D) per call
+    "transports": (5, 0),    # u, v, h -> uh, vh
+    "hor_visc": (5, 0),      # u, v, h -> diffu, diffv
+    "coriolis": (7, 0),      # u, v, h, uh, vh -> CAu, CAv
+    "vel_update": (8, 0),    # u, v, CAu, CAv, diffu, diffv -> up, vp
+    "vert_visc": (5, 4),     # u, v, h -> u, v ; taux, tauy, Ray_u, Ray_v
+    "continuity": (5, 6),    # u, hin -> h, uh, h_u ; BT_cont 2D
+    "btstep": (0, 6),        # eta, ubt, vbt -> u_av, v_av, eta_av
+    # fused transports+hor_visc+CorAdCalc+velocity update, compulsory traffic
+    # averaged over its two calls: predictor u,v,h0 -> up,vp (5), corrector
+    # up,vp,h,u,v -> u,v (7)
+    "momentum": (6, 0),
+}
+STAGE_CALLS = {"transports": 2, "hor_visc": 2, "coriolis": 2, "vel_update": 2, "vert_visc": 2,
+               "continuity": 2, "btstep": 2, "halo": 1, "momentum": 2}
+
+
+def cpu_baseline(cfg, steps_cap=1):
+    """The reference serial driver (built from its own sources into oracle/_ref)
+    timed on this host, one thread, for one RK2 step of the same grid."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "rk2_serial_driver")
+    sample = (f"{steps_cap} RK2 step of the reference serial driver rk2_serial_driver "
+              f"{cfg.ni} {cfg.nj} {cfg.nk} {steps_cap} {cfg.bt_nsteps} (its analytic state, static halo)")
+    if os.path.exists(exe):
+        cmd = f'ulimit -s unlimited 2>/dev/null || ulimit -s hard; exec "{exe}" {cfg.ni} {cfg.nj} {cfg.nk} {steps_cap} {cfg.bt_nsteps}'
+        try:
+            out = subprocess.run(["bash", "-c", cmd], capture_output=True, text=True, timeout=240)
+            t = [float(x) for x in re.findall(r"RK2 iteration\s+\d+:\s+([0-9.]+) s", out.stdout)]
+            if out.returncode == 0 and t:
+                t_step = sum(t) / len(t)
+                return {"value": cfg.cells / t_step, "unit": "cell-updates/s", "cores": 1,
+                        "kind": "reference", "sample": sample, "s_per_step": t_step}
+        except subprocess.TimeoutExpired:
+            pass
+    # fallback: the C restatement (oracle/mom6_oracle.c) on a 180x180x75 sample
+    from hackathon_mom6_miniapp_amd.state import make_case
+    from oracle.pyoracle import Oracle
+    small = workload_config(180, 180, cfg.nk, cfg.bt_nsteps)
+    o = Oracle(small, make_case(small, init="seeded", seed=1))
+    t0 = time.perf_counter()
+    o.step(2)
+    t_step = (time.perf_counter() - t0) / 2
+    return {"value": small.cells / t_step, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "2 RK2 steps of the C oracle at 180x180x75 (same per-cell work)",
+            "s_per_step": t_step}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ni", type=int, default=720)
+    ap.add_argument("--nj", type=int, default=720)
+    ap.add_argument("--nk", type=int, default=75)
+    ap.add_argument("--bt", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design, not code)", file=sys.stderr)
+        sys.exit(2)
+
+    from hackathon_mom6_miniapp_amd.model import Model
+    from hackathon_mom6_miniapp_amd.state import make_case
+    from hackathon_mom6_miniapp_amd._lib import lib, check
+
+    cfg = workload_config(args.ni, args.nj, args.nk, args.bt)
+    t0 = time.perf_counter()
+    case = make_case(cfg, init="seeded", seed=2024, lazy3d=True)
+    print(f"[bench] state generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    model = Model(cfg, case)
+    del case
+    L = lib()
+
+    model.step(args.warmup)
+    model.synchronize()
+    model.set_option("profile", 0.0 if args.no_profile else 1.0)
+    check(L.mom6h_device_synchronize())
+    t0 = time.perf_counter()
+    model.step(args.steps)
+    model.synchronize()
+    check(L.mom6h_device_synchronize())
+    elapsed = time.perf_counter() - t0
+    stages = model.stage_times() if not args.no_profile else {}
+    print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    t_step = elapsed / args.steps
+    value = cfg.cells / t_step
+    n3 = cfg.n1 * cfg.n2 * cfg.nk
+    n2 = cfg.n1 * cfg.n2
+    per_stage = {}
+    for name, ms in stages.items():
+        calls = STAGE_CALLS.get(name, 1) * args.steps
+        per_stage[name] = {"ms_per_step": ms / args.steps, "ms_per_call": ms / calls}
+    roofline = None
+    dominant = None
+    if per_stage:
+        cand = {k: v for k, v in per_stage.items() if k in STAGE_ARRAYS and v["ms_per_step"] > 0}
+        dominant = max(cand, key=lambda k: cand[k]["ms_per_step"])
+        a3, a2 = STAGE_ARRAYS[dominant]
+        traffic_alg = 8.0 * (a3 * n3 + a2 * n2)
+        ach = traffic_alg / (per_stage[dominant]["ms_per_call"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
+                    "algorithmic_bytes_per_launch": traffic_alg}
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64/Irwin-Hall state)",
+        "config": {"workload": f"split-RK2 step FP64 {cfg.ni}x{cfg.nj}x{cfg.nk} bt_nsteps={cfg.bt_nsteps}, "
+                               "beta-plane, zonally periodic, walls in y, Kh=1e3",
+                   "ni": cfg.ni, "nj": cfg.nj, "nk": cfg.nk, "bt_nsteps": cfg.bt_nsteps, "dt": cfg.dt,
+                   "parallelism": "single GPU"},
+        "step_algorithmic_GBps": step_bytes(cfg) / t_step / 1e9,
+        "roofline": roofline,
+        "stages": per_stage,
+    }
+    model.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

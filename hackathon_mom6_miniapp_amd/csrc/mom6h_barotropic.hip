@@ -1,26 +1,19 @@
-// mom6h_barotropic.hip -- the barotropic substep loop as ONE persistent kernel.
+// mom6h_barotropic.hip -- the barotropic substep loop on gfx950.
 //
 // Arithmetic of src/serial/mom6_barotropic_serial.F90:178-356 (btstep,
 // update_u, update_v) and, for periodic grids, the split API's extended
 // transport ranges (btstep_do_step, :379-474) with a refill of the ubt, vbt and
 // eta halos after every substep but the last (app/rk2_mpi_serial_driver.F90:
-// 297-306).  The reference CUDA-C path issues ~9 launches per substep from the
-// host; here a cooperative launch runs all substeps with grid-wide barriers
-// between the dependent phases of one substep:
-//   P1  store previous velocities, eta predictor
-//   P2  pressure forces, first velocity update (v on odd substeps)
-//   P3  second velocity update, transports uhbt/vhbt
-//   P4  free-surface update, time averages
-//   P5  (periodic only) halo refill
-// Every phase writes only cells that no other thread of the same phase reads.
-// State/work arrays persist in the control structure and start at zero, so
-// the cells the reference never writes (uhbt(is-1), uhbt(ie), vhbt(js-1),
-// vhbt(je) with the narrow ranges) read as zero, as they do there.
-#include <hip/hip_cooperative_groups.h>
+// 297-306).  The reference CUDA-C path issues ~9 dependent launches per
+// substep from the host; here each substep is ONE launch of an LDS-tiled
+// kernel that runs the whole substep (eta predictor, pressure forces, both
+// Coriolis/velocity updates in the alternating order, transports, free
+// surface, time averages, periodic refill) on its tile.  Transports that the
+// reference never writes (uhbt on I = is-1 and ie, vhbt on J = js-1 and je
+// with the narrow ranges) read as zero, as they do there.
+#include <utility>
 
 #include "mom6h_internal.h"
-
-namespace cg = cooperative_groups;
 
 namespace mom6h {
 
@@ -41,135 +34,192 @@ __device__ inline double F4(const double *f, const Dims &d, int c, int i, int j)
     return f[4 * IX2(d, i, j) + (c - 1)];
 }
 
-// update_u at one point (:307-330)
-__device__ inline void upd_u(const BT &p, int i, int j)
+// ---------------------------------------------------------------------------
+// One substep per launch, 2D tiles staged in LDS.  A block owns the output
+// tile T = [i0, i0+TX-1] x [j0, j0+TY-1] and recomputes the few cells around
+// it that its stencils need (radius 2 to the west/south, 1 to the east/north),
+// so no grid-wide barrier is needed: consecutive launches on one stream order
+// the substeps.  State ping-pongs between two buffer sets; the time averages
+// are accumulated in place.
+constexpr int BT_TX = 64, BT_TY = 8;
+constexpr int BT_W = BT_TX + 3, BT_H = BT_TY + 3;
+
+struct BTStep {
+    const double *eta_o, *ubt_o, *vbt_o;   // state before the substep
+    double *eta_n, *ubt_n, *vbt_n;         // state after it
+    double *out_u, *out_v, *out_eta;       // written on the last substep only
+    int n;                                 // substep number, 1-based
+};
+
+__global__ void k_bt_copy_in(Dims d, const double *eta_in, const double *ubt_in, const double *vbt_in,
+                             double *eta, double *ubt, double *vbt, double *ubt_av, double *vbt_av)
 {
-    const Dims &d = p.d;
-    const long long c = IX2(d, i, j);
-    const double Cor = (((F4(p.f_4_u, d, 4, i, j) * p.vbt[IX2(d, i + 1, j)]) + (F4(p.f_4_u, d, 1, i, j) * p.vbt[IX2(d, i, j - 1)])) +
-                        ((F4(p.f_4_u, d, 3, i, j) * p.vbt[c]) + (F4(p.f_4_u, d, 2, i, j) * p.vbt[IX2(d, i + 1, j - 1)])));
-    p.Cor_u[c] = Cor;
-    p.ubt[c] = p.bt_rem_u[c] * (p.ubt[c] + p.dtbt * (Cor + p.PFu[c]));
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.s2) return;
+    eta[q] = eta_in[q];
+    ubt[q] = ubt_in[q];
+    vbt[q] = vbt_in[q];
+    ubt_av[q] = 0.0;
+    vbt_av[q] = 0.0;
 }
 
-// update_v at one point (:333-356)
-__device__ inline void upd_v(const BT &p, int i, int j)
+__global__ void __launch_bounds__(256) k_bt_substep(BT p, BTStep st)
 {
-    const Dims &d = p.d;
-    const long long c = IX2(d, i, j);
-    const double Cor = -1.0 * (((F4(p.f_4_v, d, 1, i, j) * p.ubt[IX2(d, i - 1, j)]) + (F4(p.f_4_v, d, 4, i, j) * p.ubt[IX2(d, i, j + 1)])) +
-                               ((F4(p.f_4_v, d, 2, i, j) * p.ubt[c]) + (F4(p.f_4_v, d, 3, i, j) * p.ubt[IX2(d, i - 1, j + 1)])));
-    p.Cor_v[c] = Cor;
-    p.vbt[c] = p.bt_rem_v[c] * (p.vbt[c] + p.dtbt * (Cor + p.PFv[c]));
-}
-
-__global__ void __launch_bounds__(256) k_btstep(BT p)
-{
-    cg::grid_group grid = cg::this_grid();
+    __shared__ double s_eta[BT_H][BT_W], s_u[BT_H][BT_W], s_v[BT_H][BT_W];
+    __shared__ double s_ep[BT_H][BT_W], s_un[BT_H][BT_W], s_vn[BT_H][BT_W];
+    __shared__ double s_PFu[BT_H][BT_W], s_PFv[BT_H][BT_W];
     const Dims &d = p.d;
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nth = (long long)gridDim.x * blockDim.x;
-    const long long nall = d.s2;
-    // points of the box is-1..ie+1 x js-1..je+1, enumerated row-major
-    const int bw = (ie + 1) - (is - 1) + 1, bh = (je + 1) - (js - 1) + 1;
-    const long long nbox = (long long)bw * bh;
+    const int i0 = d.isd + blockIdx.x * BT_TX, j0 = d.jsd + blockIdx.y * BT_TY;
+    const int i1 = i0 + BT_TX - 1, j1 = j0 + BT_TY - 1;
+    const int bi = i0 - 2, bj = j0 - 2;  // LDS origin
+    const int tid = threadIdx.x;
+    const bool v_first = ((st.n + p.first_direction) % 2) == 1;
+    const double dtbt = p.dtbt;
 
-    for (long long q = tid; q < nall; q += nth) {  // initialise from input (:199-207)
-        p.eta[q] = p.eta_in[q];
-        p.ubt[q] = p.ubt_in[q];
-        p.vbt[q] = p.vbt_in[q];
-        p.ubt_av[q] = 0.0;
-        p.vbt_av[q] = 0.0;
+    // L: old state on [i0-2, i1+1] x [j0-2, j1+1]
+    for (int t = tid; t < BT_W * BT_H; t += blockDim.x) {
+        const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
+        double e = 0.0, uu = 0.0, vv = 0.0;
+        if (i >= d.isd && i <= d.ied && j >= d.jsd && j <= d.jed) {
+            const long long c = IX2(d, i, j);
+            e = st.eta_o[c]; uu = st.ubt_o[c]; vv = st.vbt_o[c];
+        }
+        s_eta[lj][li] = e; s_u[lj][li] = uu; s_v[lj][li] = vv;
     }
-    grid.sync();
-
+    __syncthreads();
+    // P1: eta predictor (:224-231) on [i0-1, i1+1] x [j0-1, j1+1]
+    for (int t = tid; t < BT_W * BT_H; t += blockDim.x) {
+        const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
+        if (i < i0 - 1 || j < j0 - 1 || i < is || i > ie || j < js || j > je) continue;
+        const long long c = IX2(d, i, j);
+        s_ep[lj][li] = s_eta[lj][li] + (dtbt * p.IareaT[c]) *
+            (((p.Datu[IX2(d, i - 1, j)] * s_u[lj][li - 1]) - (p.Datu[c] * s_u[lj][li])) +
+             ((p.Datv[IX2(d, i, j - 1)] * s_v[lj - 1][li]) - (p.Datv[c] * s_v[lj][li])));
+    }
+    __syncthreads();
+    // P2: pressure forces (:234-247) and the first velocity update (:250-260)
+    for (int t = tid; t < BT_W * BT_H; t += blockDim.x) {
+        const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
+        if (i < i0 - 1 || j < j0 - 1) continue;
+        const long long c = IX2(d, i, j);
+        const bool in_u = i >= is && i <= ie - 1 && j >= js && j <= je;
+        const bool in_v = i >= is && i <= ie && j >= js && j <= je - 1;
+        double PFu = 0.0, PFv = 0.0;
+        if (in_u && i <= i1) {
+            PFu = ((s_ep[lj][li] * p.gtot_E[c]) - (s_ep[lj][li + 1] * p.gtot_W[IX2(d, i + 1, j)])) * p.dgeo_de * p.IdxCu[c];
+            s_PFu[lj][li] = PFu;
+        }
+        if (in_v && j <= j1) {
+            PFv = ((s_ep[lj][li] * p.gtot_N[c]) - (s_ep[lj + 1][li] * p.gtot_S[IX2(d, i, j + 1)])) * p.dgeo_de * p.IdyCv[c];
+            s_PFv[lj][li] = PFv;
+        }
+        if (v_first) {  // vbt on [i0-1, i1+1] x [j0-1, j1]
+            if (j <= j1) {
+                double vn = s_v[lj][li];
+                if (in_v) {
+                    const double Cor = -1.0 * (((F4(p.f_4_v, d, 1, i, j) * s_u[lj][li - 1]) + (F4(p.f_4_v, d, 4, i, j) * s_u[lj + 1][li])) +
+                                               ((F4(p.f_4_v, d, 2, i, j) * s_u[lj][li]) + (F4(p.f_4_v, d, 3, i, j) * s_u[lj + 1][li - 1])));
+                    vn = p.bt_rem_v[c] * (vn + dtbt * (Cor + PFv));
+                }
+                s_vn[lj][li] = vn;
+            }
+        } else {        // ubt on [i0-1, i1] x [j0-1, j1+1]
+            if (i <= i1) {
+                double un = s_u[lj][li];
+                if (in_u) {
+                    const double Cor = (((F4(p.f_4_u, d, 4, i, j) * s_v[lj][li + 1]) + (F4(p.f_4_u, d, 1, i, j) * s_v[lj - 1][li])) +
+                                        ((F4(p.f_4_u, d, 3, i, j) * s_v[lj][li]) + (F4(p.f_4_u, d, 2, i, j) * s_v[lj - 1][li + 1])));
+                    un = p.bt_rem_u[c] * (un + dtbt * (Cor + PFu));
+                }
+                s_un[lj][li] = un;
+            }
+        }
+    }
+    __syncthreads();
+    // P3: the second velocity update
+    for (int t = tid; t < BT_W * BT_H; t += blockDim.x) {
+        const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
+        if (i < i0 - 1 || j < j0 - 1 || i > i1 || j > j1) continue;
+        const long long c = IX2(d, i, j);
+        if (v_first) {  // ubt on [i0-1, i1] x [j0, j1]
+            if (j < j0) continue;
+            double un = s_u[lj][li];
+            if (i >= is && i <= ie - 1 && j >= js && j <= je) {
+                const double Cor = (((F4(p.f_4_u, d, 4, i, j) * s_vn[lj][li + 1]) + (F4(p.f_4_u, d, 1, i, j) * s_vn[lj - 1][li])) +
+                                    ((F4(p.f_4_u, d, 3, i, j) * s_vn[lj][li]) + (F4(p.f_4_u, d, 2, i, j) * s_vn[lj - 1][li + 1])));
+                un = p.bt_rem_u[c] * (un + dtbt * (Cor + s_PFu[lj][li]));
+            }
+            s_un[lj][li] = un;
+        } else {        // vbt on [i0, i1] x [j0-1, j1]
+            if (i < i0) continue;
+            double vn = s_v[lj][li];
+            if (i >= is && i <= ie && j >= js && j <= je - 1) {
+                const double Cor = -1.0 * (((F4(p.f_4_v, d, 1, i, j) * s_un[lj][li - 1]) + (F4(p.f_4_v, d, 4, i, j) * s_un[lj + 1][li])) +
+                                           ((F4(p.f_4_v, d, 2, i, j) * s_un[lj][li]) + (F4(p.f_4_v, d, 3, i, j) * s_un[lj + 1][li - 1])));
+                vn = p.bt_rem_v[c] * (vn + dtbt * (Cor + s_PFv[lj][li]));
+            }
+            s_vn[lj][li] = vn;
+        }
+    }
+    __syncthreads();
+    // P4: transports (:262-272), free surface (:274-279), averages (:281-291), output
     const double trans_wt1 = 1.0 + p.bebt, trans_wt2 = -p.bebt;
     const double inv_nstep = 1.0 / (double)p.nstep;
     const int iu0 = p.extended ? is - 1 : is, iu1 = p.extended ? ie : ie - 1;
     const int jv0 = p.extended ? js - 1 : js, jv1 = p.extended ? je : je - 1;
+    const bool refill = (p.px || p.py) && st.n < p.nstep;
+    const bool last = st.n == p.nstep;
     const int P = ie - is + 1, Q = je - js + 1;
-
-    for (int n = 1; n <= p.nstep; ++n) {
-        const bool v_first = ((n + p.first_direction) % 2) == 1;
-        // P1 (:213-231)
-        for (long long q = tid; q < nbox; q += nth) {
-            const int i = (is - 1) + (int)(q % bw), j = (js - 1) + (int)(q / bw);
-            const long long c = IX2(d, i, j);
-            if (j >= js && j <= je) p.ubt_prev[c] = p.ubt[c];
-            if (i >= is && i <= ie) p.vbt_prev[c] = p.vbt[c];
-            if (i >= is && i <= ie && j >= js && j <= je)
-                p.eta_pred[c] = p.eta[c] + (p.dtbt * p.IareaT[c]) *
-                    (((p.Datu[IX2(d, i - 1, j)] * p.ubt[IX2(d, i - 1, j)]) - (p.Datu[c] * p.ubt[c])) +
-                     ((p.Datv[IX2(d, i, j - 1)] * p.vbt[IX2(d, i, j - 1)]) - (p.Datv[c] * p.vbt[c])));
+    auto uhbt = [&](int i, int j, int li, int lj) -> double {
+        if (i < iu0 || i > iu1 || j < js || j > je) return 0.0;
+        return p.Datu[IX2(d, i, j)] * (trans_wt1 * s_un[lj][li] + trans_wt2 * s_u[lj][li]);
+    };
+    auto vhbt = [&](int i, int j, int li, int lj) -> double {
+        if (i < is || i > ie || j < jv0 || j > jv1) return 0.0;
+        return p.Datv[IX2(d, i, j)] * (trans_wt1 * s_vn[lj][li] + trans_wt2 * s_v[lj][li]);
+    };
+    for (int t = tid; t < BT_TX * BT_TY; t += blockDim.x) {
+        const int i = i0 + t % BT_TX, j = j0 + t / BT_TX;
+        if (i > d.ied || j > d.jed) continue;
+        const int li = i - bi, lj = j - bj;
+        const long long c = IX2(d, i, j);
+        const bool in_eta = i >= is && i <= ie && j >= js && j <= je;
+        const bool in_u = i >= is && i <= ie - 1 && j >= js && j <= je;
+        const bool in_v = i >= is && i <= ie && j >= js && j <= je - 1;
+        double e = s_eta[lj][li];
+        if (in_eta)
+            e = e - dtbt * p.IareaT[c] * ((uhbt(i, j, li, lj) - uhbt(i - 1, j, li - 1, lj)) +
+                                          (vhbt(i, j, li, lj) - vhbt(i, j - 1, li, lj - 1)));
+        // time averages; ubt/vbt carry their new value only on the update ranges
+        const double un = in_u ? s_un[lj][li] : s_u[lj][li];
+        const double vn = in_v ? s_vn[lj][li] : s_v[lj][li];
+        double ua = p.ubt_av[c], va = p.vbt_av[c];
+        if (in_u) { ua = ua + un * inv_nstep; p.ubt_av[c] = ua; }
+        if (in_v) { va = va + vn * inv_nstep; p.vbt_av[c] = va; }
+        const bool halo_cell = i < is || i > ie || j < js || j > je;
+        if (!(refill && halo_cell && (p.px ? true : (i >= is && i <= ie)) && (p.py ? true : (j >= js && j <= je)) &&
+              ((p.px && (i < is || i > ie)) || (p.py && (j < js || j > je))))) {
+            st.eta_n[c] = e; st.ubt_n[c] = un; st.vbt_n[c] = vn;
         }
-        grid.sync();
-        // P2 (:234-259): pressure forces, then the first update using them
-        for (long long q = tid; q < nbox; q += nth) {
-            const int i = (is - 1) + (int)(q % bw), j = (js - 1) + (int)(q / bw);
-            const long long c = IX2(d, i, j);
-            const bool in_u = i >= is && i <= ie - 1 && j >= js && j <= je;
-            const bool in_v = i >= is && i <= ie && j >= js && j <= je - 1;
-            if (in_u)
-                p.PFu[c] = ((p.eta_pred[c] * p.gtot_E[c]) - (p.eta_pred[IX2(d, i + 1, j)] * p.gtot_W[IX2(d, i + 1, j)])) *
-                           p.dgeo_de * p.IdxCu[c];
-            if (in_v)
-                p.PFv[c] = ((p.eta_pred[c] * p.gtot_N[c]) - (p.eta_pred[IX2(d, i, j + 1)] * p.gtot_S[IX2(d, i, j + 1)])) *
-                           p.dgeo_de * p.IdyCv[c];
-            if (v_first) {
-                if (in_v) upd_v(p, i, j);
-            } else {
-                if (in_u) upd_u(p, i, j);
-            }
+        if (refill && !halo_cell) {
+            // write the periodic images of this compute cell (halo refill, :301-303)
+            const int ni_[3] = {i, (p.px && i > ie - (is - d.isd)) ? i - P : ((p.px && i < is + (d.ied - ie)) ? i + P : -1), 0};
+            const int nj_[3] = {j, (p.py && j > je - (js - d.jsd)) ? j - Q : ((p.py && j < js + (d.jed - je)) ? j + Q : -1), 0};
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    if (a == 0 && b == 0) continue;
+                    const int ti = ni_[a], tj = nj_[b];
+                    if (ti < d.isd || ti > d.ied || tj < d.jsd || tj > d.jed) continue;
+                    const long long ct = IX2(d, ti, tj);
+                    st.eta_n[ct] = e; st.ubt_n[ct] = un; st.vbt_n[ct] = vn;
+                }
         }
-        grid.sync();
-        // P3 (:252-272): second update, then this point's transports
-        for (long long q = tid; q < nbox; q += nth) {
-            const int i = (is - 1) + (int)(q % bw), j = (js - 1) + (int)(q / bw);
-            const long long c = IX2(d, i, j);
-            if (v_first) {
-                if (i >= is && i <= ie - 1 && j >= js && j <= je) upd_u(p, i, j);
-            } else {
-                if (i >= is && i <= ie && j >= js && j <= je - 1) upd_v(p, i, j);
-            }
-            if (i >= iu0 && i <= iu1 && j >= js && j <= je)
-                p.uhbt[c] = p.Datu[c] * (trans_wt1 * p.ubt[c] + trans_wt2 * p.ubt_prev[c]);
-            if (i >= is && i <= ie && j >= jv0 && j <= jv1)
-                p.vhbt[c] = p.Datv[c] * (trans_wt1 * p.vbt[c] + trans_wt2 * p.vbt_prev[c]);
+        if (last) {  // outputs (:296-302)
+            st.out_u[c] = p.ubt_av[c];
+            st.out_v[c] = p.vbt_av[c];
+            st.out_eta[c] = e;
         }
-        grid.sync();
-        // P4 (:274-291)
-        for (long long q = tid; q < nbox; q += nth) {
-            const int i = (is - 1) + (int)(q % bw), j = (js - 1) + (int)(q / bw);
-            const long long c = IX2(d, i, j);
-            if (i >= is && i <= ie && j >= js && j <= je)
-                p.eta[c] = p.eta[c] - p.dtbt * p.IareaT[c] *
-                    ((p.uhbt[c] - p.uhbt[IX2(d, i - 1, j)]) + (p.vhbt[c] - p.vhbt[IX2(d, i, j - 1)]));
-            if (i >= is && i <= ie - 1 && j >= js && j <= je) p.ubt_av[c] = p.ubt_av[c] + p.ubt[c] * inv_nstep;
-            if (i >= is && i <= ie && j >= js && j <= je - 1) p.vbt_av[c] = p.vbt_av[c] + p.vbt[c] * inv_nstep;
-        }
-        grid.sync();
-        if ((p.px || p.py) && n < p.nstep) {
-            // P5: halo refill; a halo cell copies its periodic image, which is
-            // always a compute-domain cell (same result as x-then-y passes)
-            for (long long q = tid; q < nall; q += nth) {
-                const int i = d.isd + (int)(q % d.n1), j = d.jsd + (int)(q / d.n1);
-                int si = i, sj = j;
-                if (p.px) si = (i < is) ? i + P : (i > ie ? i - P : i);
-                if (p.py) sj = (j < js) ? j + Q : (j > je ? j - Q : j);
-                if (si == i && sj == j) continue;
-                const long long s = IX2(d, si, sj);
-                p.ubt[q] = p.ubt[s];
-                p.vbt[q] = p.vbt[s];
-                p.eta[q] = p.eta[s];
-            }
-            grid.sync();
-        }
-    }
-    for (long long q = tid; q < nall; q += nth) {  // outputs (:296-302)
-        p.u_av[q] = p.ubt_av[q];
-        p.v_av[q] = p.vbt_av[q];
-        p.eta_av[q] = p.eta[q];
     }
 }
 
@@ -191,17 +241,13 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
     MOM6H_REQUIRE(isd < isc && iec < ied && jsd < jsc && jec < jed && isc < iec && jsc < jec,
                   "mom6h_barotropic_init: needs one halo cell on every side");
     MOM6H_REQUIRE(nstep >= 1 && dt > 0.0, "mom6h_barotropic_init: nstep >= 1 and dt > 0 required");
-    MOM6H_REQUIRE(!periodic_x || (iec - isc + 1) >= (isc - isd) && (iec - isc + 1) >= (ied - iec),
-                  "mom6h_barotropic_init: compute domain narrower than the x halo");
-    MOM6H_REQUIRE(!periodic_y || (jec - jsc + 1) >= (jsc - jsd) && (jec - jsc + 1) >= (jed - jec),
-                  "mom6h_barotropic_init: compute domain narrower than the y halo");
+    MOM6H_REQUIRE(!periodic_x || (iec - isc + 1) >= (isc - isd) + (ied - iec),
+                  "mom6h_barotropic_init: periodic x needs a compute width of at least both halos");
+    MOM6H_REQUIRE(!periodic_y || (jec - jsc + 1) >= (jsc - jsd) + (jed - jec),
+                  "mom6h_barotropic_init: periodic y needs a compute height of at least both halos");
     MOM6H_REQUIRE(Datu && Datv && gtot_E && gtot_W && gtot_N && gtot_S && f_4_u && f_4_v && bt_rem_u &&
                       bt_rem_v && IareaT && IdxCu && IdyCv,
                   "mom6h_barotropic_init: NULL array");
-    int coop = 0, dev = 0;
-    MOM6H_CHECK(hipGetDevice(&dev));
-    MOM6H_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-    MOM6H_REQUIRE(coop, "mom6h_barotropic_init: device lacks cooperative launch");
     auto *cs = new mom6h_barotropic_cs();
     cs->d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, 1);
     cs->halo = Halo{periodic_x, periodic_y};
@@ -226,17 +272,6 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
         mom6h_barotropic_end(cs);
         return rc;
     }
-    // persistent grid: as many resident 256-thread blocks as the device holds,
-    // capped by the work of one phase
-    int per_cu = 0, ncu = 0;
-    MOM6H_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_btstep, 256, 0));
-    MOM6H_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const long long box = (long long)(iec - isc + 3) * (jec - jsc + 3);
-    long long want = (box + 255) / 256;
-    long long cap = (long long)per_cu * ncu;
-    if (cap > (long long)ncu) cap = ncu;   // one block per CU leaves room for the 3D kernels
-    cs->grid_blocks = (int)(want < cap ? want : cap);
-    if (cs->grid_blocks < 1) cs->grid_blocks = 1;
     *out = cs;
     return MOM6H_OK;
 }
@@ -262,9 +297,27 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
     p.ubt_av = cs->ubt_av; p.vbt_av = cs->vbt_av;
     p.eta_in = eta_in; p.ubt_in = ubt_in; p.vbt_in = vbt_in;
     p.u_av = u_av; p.v_av = v_av; p.eta_av = eta_av;
-    void *args[] = {&p};
-    MOM6H_CHECK(hipLaunchCooperativeKernel((const void *)k_btstep, dim3(cs->grid_blocks), dim3(256),
-                                           args, 0, (hipStream_t)stream));
+    // copy-in, then one launch per substep ping-ponging between the two state sets
+    hipStream_t strm = (hipStream_t)stream;
+    const Dims &d = cs->d;
+    {
+        const int bs = 256;
+        k_bt_copy_in<<<(unsigned)((d.s2 + bs - 1) / bs), bs, 0, strm>>>(d, eta_in, ubt_in, vbt_in, cs->eta, cs->ubt,
+                                                                         cs->vbt, cs->ubt_av, cs->vbt_av);
+        MOM6H_LAUNCH_CHECK();
+    }
+    dim3 grid((d.n1 + BT_TX - 1) / BT_TX, (d.n2 + BT_TY - 1) / BT_TY), block(256);
+    double *A[3] = {cs->eta, cs->ubt, cs->vbt}, *B[3] = {cs->eta_pred, cs->ubt_prev, cs->vbt_prev};
+    for (int n = 1; n <= cs->nstep; ++n) {
+        BTStep st;
+        st.eta_o = A[0]; st.ubt_o = A[1]; st.vbt_o = A[2];
+        st.eta_n = B[0]; st.ubt_n = B[1]; st.vbt_n = B[2];
+        st.out_u = u_av; st.out_v = v_av; st.out_eta = eta_av;
+        st.n = n;
+        k_bt_substep<<<grid, block, 0, strm>>>(p, st);
+        MOM6H_LAUNCH_CHECK();
+        for (int q = 0; q < 3; ++q) std::swap(A[q], B[q]);
+    }
     return MOM6H_OK;
 }
 

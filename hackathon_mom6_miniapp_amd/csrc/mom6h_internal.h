@@ -112,6 +112,8 @@ struct mom6h_barotropic_cs {
 struct mom6h_vert_visc_cs {
     mom6h::Dims d;
     double Kv, Kv_ml, Kv_extra_bbl, Hmix, Hbbl;
+    int keep_coefs;          // store a_u/a_v/h_u/h_v (the reference's CS arrays)
+    long long max_lds;       // device limit on shared memory per block
     double *mask2dCu, *mask2dCv;
     double *a_u, *a_v;       // coupling coefficients, nk+1 levels
     double *h_u, *h_v;       // effective thicknesses at velocity points

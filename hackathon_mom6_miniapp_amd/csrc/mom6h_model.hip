@@ -24,7 +24,7 @@
 #include <string>
 #include <vector>
 
-#include "mom6h_internal.h"
+#include "mom6h_fused.h"
 
 using namespace mom6h;
 
@@ -41,10 +41,10 @@ struct Field {
 
 enum Stage {
     ST_TRANSPORTS, ST_HOR_VISC, ST_CORIOLIS, ST_VEL_UPDATE, ST_VERT_VISC, ST_CONTINUITY,
-    ST_HALO, ST_BTSTEP, ST_COUNT
+    ST_HALO, ST_BTSTEP, ST_MOMENTUM, ST_COUNT
 };
-const char *kStageNames[ST_COUNT] = {"transports", "hor_visc",   "coriolis", "vel_update",
-                                     "vert_visc",  "continuity", "halo",     "btstep"};
+const char *kStageNames[ST_COUNT] = {"transports", "hor_visc",   "coriolis", "vel_update", "vert_visc",
+                                     "continuity", "halo",       "btstep",   "momentum"};
 
 }  // namespace
 
@@ -54,6 +54,9 @@ struct mom6h_model {
     Halo halo;
     bool prepared = false;
     int use_graph = 0, profile = 0;
+    int fused = 1;           // fused momentum stencil (module-by-module when 0)
+    bool uniform = false;    // every metric array constant (fused kernels allowed)
+    UMetrics ug{};
     hipStream_t s0 = nullptr, s1 = nullptr;
     hipEvent_t e_start = nullptr, e_bt = nullptr;
     // state and driver arrays
@@ -150,34 +153,63 @@ struct StageTimer {
     }
 };
 
-// One RK2 step (rk2_serial_driver.F90:218-399).
-int enqueue_step(mom6h_model *m)
+// The barotropic chain of one step on stream s1 (:285, :365): predictor and
+// corrector btstep, then (periodic grids) the end-of-step refill of the 2D
+// fields it owns.  It never waits for the 3D stream.
+int enqueue_barotropic(mom6h_model *m)
 {
-    const double dt = m->c.dt;
-    const int nk = m->c.nk;
-    const Dims &d = m->d;
-    hipStream_t s0 = m->s0, s1 = m->s1;
-    const double *rayu = m->ray_u, *rayv = m->ray_v;
-
-    // barotropic chain on its own stream (:285, :365)
-    MOM6H_CHECK(hipEventRecord(m->e_start, s0));
-    MOM6H_CHECK(hipStreamWaitEvent(s1, m->e_start, 0));
+    hipStream_t s1 = m->s1;
     {
         StageTimer t(m, ST_BTSTEP, s1);
         RC(mom6h_btstep(m->bt_cs, m->eta, m->ubt, m->vbt, m->ubt_av, m->vbt_av, m->eta_av, s1));
         RC(mom6h_btstep(m->bt_cs, m->eta_av, m->ubt_av, m->vbt_av, m->ubt_av, m->vbt_av, m->eta, s1));
     }
-    MOM6H_CHECK(hipEventRecord(m->e_bt, s1));
+    if (m->halo.px || m->halo.py)
+        for (double *a : {m->eta, m->ubt, m->vbt}) RC(halo_fill(m->d, m->halo, a, 1, s1));
+    return MOM6H_OK;
+}
+
+bool fused_ok(const mom6h_model *m)
+{
+    return m->fused && m->uniform && m->c.laplacian && !m->c.biharmonic && m->c.coriolis_scheme == 1;
+}
+
+// transports + hor_visc + CorAdCalc + velocity update of one phase
+int momentum_phase(mom6h_model *m, const double *u, const double *v, const double *h, double *ou,
+                   double *ov, double fdt)
+{
+    const Dims &d = m->d;
+    const int nk = m->c.nk;
+    hipStream_t s0 = m->s0;
+    if (fused_ok(m)) {
+        StageTimer t(m, ST_MOMENTUM, s0);
+        MomArgs a;
+        a.d = d; a.g = m->ug; a.Kh = m->c.Kh; a.h_neglect = 1.0e-3; a.fdt = fdt; a.no_slip = m->c.no_slip;
+        a.u = u; a.v = v; a.h = h; a.bu = m->u; a.bv = m->v; a.ou = ou; a.ov = ov;
+        a.mask2dT = m->mask2dT; a.mask2dBu = m->mask2dBu; a.CoriolisBu = m->CoriolisBu;
+        return launch_momentum(a, s0);
+    }
+    { StageTimer t(m, ST_TRANSPORTS, s0);
+      RC(mom6h_transports(d.isd, d.ied, d.jsd, d.jed, nk, u, v, h, m->cor->dyCu, m->cor->dxCv, m->uh, m->vh, s0)); }
+    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->uh, nk}, {m->vh, nk}})); }
+    { StageTimer t(m, ST_HOR_VISC, s0); RC(mom6h_hor_visc(m->hv, u, v, h, m->diffu, m->diffv, s0)); }
+    { StageTimer t(m, ST_CORIOLIS, s0); RC(mom6h_coradcalc(m->cor, u, v, h, m->uh, m->vh, m->CAu, m->CAv, s0)); }
+    { StageTimer t(m, ST_VEL_UPDATE, s0);
+      RC(mom6h_vel_update(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk, fdt, m->u, m->v, m->CAu,
+                          m->diffu, m->CAv, m->diffv, ou, ov, s0)); }
+    return MOM6H_OK;
+}
+
+// The 3D part of one RK2 step (rk2_serial_driver.F90:218-399) on stream s0.
+int enqueue_step(mom6h_model *m)
+{
+    const double dt = m->c.dt;
+    const int nk = m->c.nk;
+    hipStream_t s0 = m->s0;
+    const double *rayu = m->ray_u, *rayv = m->ray_v;
 
     // ---- predictor (h = h0 throughout)
-    { StageTimer t(m, ST_TRANSPORTS, s0);
-      RC(mom6h_transports(d.isd, d.ied, d.jsd, d.jed, nk, m->u, m->v, m->h0, m->cor->dyCu, m->cor->dxCv, m->uh, m->vh, s0)); }
-    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->uh, nk}, {m->vh, nk}})); }
-    { StageTimer t(m, ST_HOR_VISC, s0); RC(mom6h_hor_visc(m->hv, m->u, m->v, m->h0, m->diffu, m->diffv, s0)); }
-    { StageTimer t(m, ST_CORIOLIS, s0); RC(mom6h_coradcalc(m->cor, m->u, m->v, m->h0, m->uh, m->vh, m->CAu, m->CAv, s0)); }
-    { StageTimer t(m, ST_VEL_UPDATE, s0);
-      RC(mom6h_vel_update(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk, dt, m->u, m->v, m->CAu,
-                          m->diffu, m->CAv, m->diffv, m->up, m->vp, s0)); }
+    RC(momentum_phase(m, m->u, m->v, m->h0, m->up, m->vp, dt));
     { StageTimer t(m, ST_VERT_VISC, s0);
       RC(mom6h_vert_visc(m->vv, m->up, m->vp, m->h0, dt, m->taux, m->tauy, rayu, rayv, s0)); }
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->up, nk}, {m->vp, nk}})); }
@@ -186,21 +218,12 @@ int enqueue_step(mom6h_model *m)
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->htmp, nk}})); }
 
     // ---- corrector (h = h_pred, held in htmp)
-    { StageTimer t(m, ST_TRANSPORTS, s0);
-      RC(mom6h_transports(d.isd, d.ied, d.jsd, d.jed, nk, m->up, m->vp, m->htmp, m->cor->dyCu, m->cor->dxCv, m->uh, m->vh, s0)); }
-    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->uh, nk}, {m->vh, nk}})); }
-    { StageTimer t(m, ST_HOR_VISC, s0); RC(mom6h_hor_visc(m->hv, m->up, m->vp, m->htmp, m->diffu, m->diffv, s0)); }
-    { StageTimer t(m, ST_CORIOLIS, s0); RC(mom6h_coradcalc(m->cor, m->up, m->vp, m->htmp, m->uh, m->vh, m->CAu, m->CAv, s0)); }
-    { StageTimer t(m, ST_VEL_UPDATE, s0);
-      RC(mom6h_vel_update(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk, 0.5 * dt, m->u, m->v, m->CAu,
-                          m->diffu, m->CAv, m->diffv, m->u, m->v, s0)); }
+    RC(momentum_phase(m, m->up, m->vp, m->htmp, m->u, m->v, 0.5 * dt));
     { StageTimer t(m, ST_VERT_VISC, s0);
       RC(mom6h_vert_visc(m->vv, m->u, m->v, m->htmp, 0.5 * dt, m->taux, m->tauy, rayu, rayv, s0)); }
     { StageTimer t(m, ST_CONTINUITY, s0);
       RC(mom6h_continuity_ppm(m->cont, m->u, m->htmp, m->h, m->uh, 0.5 * dt, nullptr, &m->bt, m->u_cor, m->du_cor, 1, s0)); }
-    MOM6H_CHECK(hipStreamWaitEvent(s0, m->e_bt, 0));
-    { StageTimer t(m, ST_HALO, s0);
-      RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}, {m->eta, 1}, {m->ubt, 1}, {m->vbt, 1}})); }
+    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}})); }
     return MOM6H_OK;
 }
 
@@ -383,6 +406,34 @@ int mom6h_model_prepare(mom6h_model *m)
         RC(mom6h_hor_visc_set_options(m->hv, m->c.laplacian, m->c.biharmonic, m->c.Ah, m->c.no_slip,
                                       I2u.data(), Iu2.data(), I2v.data(), Iv2.data()));
     }
+    // uniform-metric values for the fused kernels (exact array contents)
+    {
+        UMetrics &g = m->ug;
+        const size_t n = n2;
+        bool ok = true;
+        auto U = [&](const std::vector<double> &a, double &v) { ok = ok && uniform_value(a.data(), n, &v); };
+        U(m->areaT, g.areaT); U(m->IareaT, g.IareaT); U(m->dxT, g.dxT); U(m->IdxT, g.IdxT);
+        U(m->dyCu, g.dyCu); U(m->dy_Cu, g.dy_Cu); U(m->dxCu, g.dxCu); U(m->IdxCu, g.IdxCu); U(m->IdyCu, g.IdyCu);
+        U(m->dxCv, g.dxCv); U(m->dyCv, g.dyCv); U(m->IdxCv, g.IdxCv); U(m->IdyCv, g.IdyCv);
+        U(m->IareaBu, g.IareaBu); U(m->IareaCu, g.IareaCu); U(m->IareaCv, g.IareaCv);
+        U(m->dyT, g.dy2h); U(m->dxBu, g.dx2q); U(m->dyBu, g.dy2q); U(m->IdyT, g.DX_dyT);
+        U(m->IdxBu, g.DY_dxBu); U(m->IdyBu, g.DX_dyBu);
+        if (ok) {
+            // derived exactly as hor_visc_init / coriolis_init form them
+            const double dyT = m->dyT[0], dxBu = m->dxBu[0], dyBu = m->dyBu[0], IdyT = m->IdyT[0];
+            const double IdxBu = m->IdxBu[0], IdyBu = m->IdyBu[0];
+            g.dx2h = g.dxT * g.dxT;
+            g.dy2h = dyT * dyT;
+            g.dx2q = dxBu * dxBu;
+            g.dy2q = dyBu * dyBu;
+            g.DX_dyT = g.dxT * IdyT;
+            g.DY_dxT = dyT * g.IdxT;
+            g.DX_dyBu = dxBu * IdyBu;
+            g.DY_dxBu = dyBu * IdxBu;
+            g.Area_q = (g.areaT + g.areaT) + (g.areaT + g.areaT);
+        }
+        m->uniform = ok;
+    }
     // periodic fill of the initial state (rk2_mpi_serial_driver.F90:191-197)
     RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}, {m->h0, nk}, {m->eta, 1}, {m->ubt, 1}, {m->vbt, 1}}));
     MOM6H_CHECK(hipStreamSynchronize(m->s0));
@@ -395,7 +446,15 @@ int mom6h_model_step(mom6h_model *m, int niter)
     MOM6H_REQUIRE(m != nullptr, "mom6h_model_step: NULL model");
     MOM6H_REQUIRE(m->prepared, "mom6h_model_step: call mom6h_model_prepare first");
     MOM6H_REQUIRE(niter >= 0, "mom6h_model_step: niter >= 0 required");
+    // The barotropic chain and the 3D chain share no array (the 2D fields the
+    // barotropic solver reads and writes are never touched by the 3D stages),
+    // so s1 runs all its steps without waiting for s0; the streams join once.
+    MOM6H_CHECK(hipEventRecord(m->e_start, m->s0));
+    MOM6H_CHECK(hipStreamWaitEvent(m->s1, m->e_start, 0));
+    for (int it = 0; it < niter; ++it) RC(enqueue_barotropic(m));
     for (int it = 0; it < niter; ++it) RC(enqueue_step(m));
+    MOM6H_CHECK(hipEventRecord(m->e_bt, m->s1));
+    MOM6H_CHECK(hipStreamWaitEvent(m->s0, m->e_bt, 0));
     return MOM6H_OK;
 }
 
@@ -422,6 +481,7 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
     MOM6H_REQUIRE(m && key, "mom6h_model_set_option: NULL argument");
     const std::string k(key);
     if (k == "graph") m->use_graph = value != 0.0;
+    else if (k == "fused") m->fused = value != 0.0;
     else if (k == "profile") {
         m->profile = value != 0.0;
         for (double &x : m->stage_ms) x = 0.0;

@@ -138,6 +138,35 @@ __global__ void k_vel_update(Dims d, double fdt, const double *bu, const double 
     if (j <= d.jec - 1) ov[c] = bv[c] + fdt * (CAv[c] + diffv[c]);
 }
 
+
+// ------------------------------------------------------------ seeded state
+// Counter-based N(mean, sigma) samples: element e of field f sums 12 uniforms
+// U_r = (x_r >> 11) * 2^-53, x_r = SplitMix64(seed*2^32 + f*2^52 + 12e + r),
+// left to right from 0, then sigma*(sum - 6) [+ mean] [* mask2d].  Integer
+// arithmetic, exact conversions and IEEE adds: the host generator
+// (hackathon_mom6_miniapp_amd/state.py) produces the same bits.
+__global__ void k_seeded_normal(double *out, long long n, unsigned long long seed,
+                                unsigned long long field, int add_mean, double mean, double sigma,
+                                const double *mask2d, long long n2)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const unsigned long long base = (seed << 32) + (field << 52) + (unsigned long long)e * 12ull;
+    double acc = 0.0;
+    for (int r = 0; r < 12; ++r) {
+        unsigned long long z = base + (unsigned long long)r;
+        z += 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        acc = acc + (double)(z >> 11) * 0x1p-53;
+    }
+    double v = sigma * (acc - 6.0);
+    if (add_mean) v = mean + v;
+    if (mask2d) v = v * mask2d[e % n2];
+    out[e] = v;
+}
+
 }  // namespace mom6h
 
 using namespace mom6h;
@@ -251,6 +280,18 @@ int mom6h_halo_fill(int isd, int ied, int jsd, int jed, int isc, int iec, int js
     Dims d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, nk);
     Halo hm{periodic_x, periodic_y};
     return halo_fill(d, hm, a, nk, (hipStream_t)stream);
+}
+
+int mom6h_seeded_normal(double *out, long count, unsigned long long seed, int field, int add_mean,
+                        double mean, double sigma, const double *mask2d, long n2, void *stream)
+{
+    MOM6H_REQUIRE(out && count > 0 && field >= 0 && field < 4096, "mom6h_seeded_normal: bad arguments");
+    MOM6H_REQUIRE(!mask2d || (n2 > 0 && count % n2 == 0), "mom6h_seeded_normal: count must be a multiple of n2");
+    const int bs = 256;
+    k_seeded_normal<<<(unsigned)((count + bs - 1) / bs), bs, 0, (hipStream_t)stream>>>(
+        out, count, seed, (unsigned long long)field, add_mean, mean, sigma, mask2d, n2 > 0 ? n2 : 1);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
 }
 
 }  // extern "C"

@@ -8,11 +8,11 @@ import ctypes as C
 import numpy as np
 
 from ._lib import MConfig, check, lib
-from .state import BC_PERIODIC, BC_WALL, Config
+from .state import BC_PERIODIC, BC_WALL, Config, Seeded
 
-# inputs uploaded before mom6h_model_prepare
-_UPLOAD = ("u", "v", "h0", "eta", "ubt", "vbt", "taux", "tauy", "CoriolisBu",
-           "mask2dT", "mask2dCu", "mask2dCv", "mask2dBu")
+# inputs uploaded before mom6h_model_prepare (2D first: masks feed the 3D recipes)
+_UPLOAD = ("eta", "ubt", "vbt", "taux", "tauy", "CoriolisBu",
+           "mask2dT", "mask2dCu", "mask2dCv", "mask2dBu", "u", "v", "h0")
 
 
 def to_mconfig(cfg: Config) -> MConfig:
@@ -37,8 +37,14 @@ class Model:
         mc = to_mconfig(cfg)
         check(self.L.mom6h_model_create(C.byref(self.m), C.byref(mc)), "mom6h_model_create")
         for name in _UPLOAD:
-            self.upload(name, case[name])
-        self.upload("h", case["h0"])
+            val = case[name]
+            if isinstance(val, Seeded):
+                self.generate(name, val)
+            else:
+                self.upload(name, val)
+        check(self.L.mom6h_device_synchronize(), "synchronize")
+        check(self.L.mom6h_memcpy_d2d(self.device_ptr("h"), self.device_ptr("h0"),
+                                      8 * self._count("h0")), "h = h0")
         ray = np.full((cfg.n1, cfg.n2), cfg.ray_bottom, order="F")
         self.upload("ray_u", ray)
         self.upload("ray_v", ray)
@@ -69,6 +75,14 @@ class Model:
     def upload(self, name: str, arr):
         a = np.asfortranarray(np.asarray(arr, dtype=np.float64))
         check(self.L.mom6h_model_upload(self.m, name.encode(), a.ctypes.data, a.size), f"upload {name}")
+
+    def generate(self, name: str, rec: Seeded):
+        """Fill a 3D field from its seeded recipe on the device."""
+        n = self._count(name)
+        mask = self.device_ptr(rec.mask) if rec.mask else None
+        check(self.L.mom6h_seeded_normal(self.device_ptr(name), n, rec.seed, rec.field,
+                                         int(rec.mean is not None), rec.mean or 0.0, rec.sigma,
+                                         mask, self.cfg.n1 * self.cfg.n2, None), f"generate {name}")
 
     def get(self, name: str) -> np.ndarray:
         n = self._count(name)

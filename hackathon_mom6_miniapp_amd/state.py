@@ -158,7 +158,31 @@ def masks(cfg: Config):
 
 
 # ----------------------------------------------------------------- state ---
-def make_case(cfg: Config, init: str = "seeded", seed: int = 1234, bump: tuple | None = None) -> dict:
+@dataclasses.dataclass
+class Seeded:
+    """A 3D field of the seeded state, generated where it is used (host or GPU,
+    same bits): ``sigma*(sum of 12 uniforms - 6)``, plus ``mean`` if given,
+    times the 2D mask named by ``mask`` if given."""
+
+    seed: int
+    field: int
+    sigma: float
+    mean: float | None = None
+    mask: str | None = None
+
+    def host(self, cfg: Config, case: dict) -> np.ndarray:
+        n3 = cfg.n1 * cfg.n2 * cfg.nk
+        a = seeded_normal(self.seed, self.field, n3, self.sigma)
+        if self.mean is not None:
+            a = self.mean + a
+        a = a.reshape((cfg.n1, cfg.n2, cfg.nk), order="F")
+        if self.mask is not None:
+            a = a * case[self.mask][:, :, None]
+        return np.asfortranarray(a)
+
+
+def make_case(cfg: Config, init: str = "seeded", seed: int = 1234, bump: tuple | None = None,
+              lazy3d: bool = False) -> dict:
     """Inputs of one run: u, v, h0 (3D); eta, ubt, vbt, taux, tauy, CoriolisBu,
     mask2dT, mask2dCu, mask2dCv, mask2dBu (2D).  ``bump = (amplitude_m, radius_m)``
     adds a Gaussian sea-surface bump centred on the compute domain."""
@@ -177,10 +201,10 @@ def make_case(cfg: Config, init: str = "seeded", seed: int = 1234, bump: tuple |
         ubt = 0.05 * np.sin((j2 - 1) / cfg.nj * p) + 0.0 * i2
         vbt = 0.05 * np.cos((i2 - 1) / cfg.ni * p) + 0.0 * j2
     elif init == "seeded":
-        n3 = n1 * n2 * nk
-        h0 = (4000.0 / nk + seeded_normal(seed, 0, n3, 0.5)).reshape(shp3, order="F")
-        u = seeded_normal(seed, 1, n3, 0.05).reshape(shp3, order="F")
-        v = seeded_normal(seed, 2, n3, 0.05).reshape(shp3, order="F")
+        walls = cfg.bc_y == BC_WALL
+        h0 = Seeded(seed, 0, 0.5, mean=4000.0 / nk)
+        u = Seeded(seed, 1, 0.05, mask="mask2dCu" if walls else None)
+        v = Seeded(seed, 2, 0.05, mask="mask2dCv" if walls else None)
         eta = np.zeros(shp2)
         ubt = np.zeros(shp2)
         vbt = np.zeros(shp2)
@@ -199,13 +223,20 @@ def make_case(cfg: Config, init: str = "seeded", seed: int = 1234, bump: tuple |
     tauy = np.zeros(shp2)
     mT, mCu, mCv, mBu = masks(cfg)
     if cfg.bc_y == BC_WALL:  # no flow on land rows and through the closed faces
-        u = u * mCu[:, :, None]
-        v = v * mCv[:, :, None]
+        if init != "seeded":
+            u = u * mCu[:, :, None]
+            v = v * mCv[:, :, None]
         ubt = ubt * mCu
         vbt = vbt * mCv
     out = dict(u=u, v=v, h0=h0, eta=eta, ubt=ubt, vbt=vbt, taux=taux, tauy=tauy,
                CoriolisBu=coriolis_bu(cfg), mask2dT=mT, mask2dCu=mCu, mask2dCv=mCv, mask2dBu=mBu)
-    return {key: np.asfortranarray(np.asarray(val, dtype=np.float64)) for key, val in out.items()}
+    case = {key: (val if isinstance(val, Seeded) else np.asfortranarray(np.asarray(val, dtype=np.float64)))
+            for key, val in out.items()}
+    if not lazy3d:
+        for key in ("u", "v", "h0"):
+            if isinstance(case[key], Seeded):
+                case[key] = case[key].host(cfg, case)
+    return case
 
 
 INPUT_ORDER = ("u", "v", "h0", "eta", "ubt", "vbt", "taux", "tauy", "CoriolisBu",

@@ -133,6 +133,10 @@ int mom6h_vert_visc_init(mom6h_vert_visc_cs **cs, int isd, int ied, int jsd, int
 int mom6h_vert_visc(mom6h_vert_visc_cs *cs, double *u, double *v, const double *h, double dt,
                     const double *taux, const double *tauy, const double *ray_u_bot,
                     const double *ray_v_bot, void *stream);
+/* Whether mom6h_vert_visc stores the coupling coefficients and effective
+ * thicknesses (default 1).  With 0 they stay on chip: used when nothing
+ * reads them after the call. */
+int mom6h_vert_visc_keep_coefficients(mom6h_vert_visc_cs *cs, int keep);
 /* device pointers of the coefficient arrays (a_u/a_v: nk+1 levels) and bottom stress */
 int mom6h_vert_visc_arrays(mom6h_vert_visc_cs *cs, double **a_u, double **a_v, double **h_u,
                            double **h_v, double **taux_bot, double **tauy_bot);
@@ -177,6 +181,14 @@ int mom6h_vel_update(int isd, int ied, int jsd, int jed, int isc, int iec, int j
  * halo_exchange_3d/2d, src/common/mom6_mpi_halo.F90) */
 int mom6h_halo_fill(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk,
                     int periodic_x, int periodic_y, double *a, void *stream);
+
+/* Seeded initial state on the device (no counterpart in the reference, whose
+ * driver uses an analytic state): N(mean, sigma) samples from SplitMix64 and
+ * Irwin-Hall sums of 12 uniforms, bit-identical to the host generator.
+ * add_mean = 0 omits the mean; mask2d (n2 values, NULL = none) multiplies
+ * every layer. */
+int mom6h_seeded_normal(double *d_out, long count, unsigned long long seed, int field, int add_mean,
+                        double mean, double sigma, const double *d_mask2d, long n2, void *stream);
 
 /* ------------------------------------------------------------ the model --
  * A whole run of app/rk2_serial_driver.F90 on one GPU: grid, control

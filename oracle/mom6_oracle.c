@@ -211,6 +211,8 @@ static void halo_fill(om_model *m, double *a, int nk)
             }
     }
 }
+void om_halo_fill(om_model *m, double *a, int nk) { halo_fill(m, a, nk); }
+
 static int periodic(const om_model *m)
 {
     return m->c.bc_x == OM_BC_PERIODIC || m->c.bc_y == OM_BC_PERIODIC;

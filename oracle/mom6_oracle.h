@@ -84,6 +84,8 @@ void om_btstep(om_model *m, const double *eta_in, const double *ubt_in, const do
                double *u_av, double *v_av, double *eta_av);
 void om_continuity_ppm(om_model *m, const double *u, const double *hin, double *h, double *uh,
                        double dt);
+/* periodic refill of the 3-wide halo of a (nk layers; no-op for static BCs) */
+void om_halo_fill(om_model *m, double *a, int nk);
 /* verify_state (rk2_serial_driver.F90:563-623): out[7] = mass_init, mass_final,
  * rel_error, max_u, max_v, max_eta, ke_total. */
 void om_verify(om_model *m, double *out7);

@@ -56,6 +56,7 @@ def lib():
         _lib.om_vert_visc.argtypes = [C.c_void_p, d, d, d, C.c_double]
         _lib.om_btstep.argtypes = [C.c_void_p, d, d, d, d, d, d]
         _lib.om_continuity_ppm.argtypes = [C.c_void_p, d, d, d, d, C.c_double]
+        _lib.om_halo_fill.argtypes = [C.c_void_p, d, C.c_int]
     return _lib
 
 
@@ -118,6 +119,9 @@ class Oracle:
         out = (C.c_double * 7)()
         lib().om_verify(self.m, out)
         return list(out)
+
+    def halo_fill(self, a: np.ndarray):
+        lib().om_halo_fill(self.m, _ptr(a), a.shape[2] if a.ndim == 3 else 1)
 
     # module-level calls on explicit arrays
     def call(self, fn: str, *arrays, scalar=None):

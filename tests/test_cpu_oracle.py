@@ -1,0 +1,206 @@
+"""The oracle against the reference: committed golden vectors (written by the
+reference's own serial modules through oracle/ref/ref_harness.F90), the
+unmodified reference driver when its sources are present, and invariants that
+the mathematics fixes."""
+import glob
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from hackathon_mom6_miniapp_amd.state import BC_PERIODIC, BC_STATIC, BC_WALL, Config, make_case
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+
+
+def _load(path):
+    z = np.load(path, allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    cfg = Config(**meta["config"])
+    case = make_case(cfg, init=meta["init"], seed=meta["seed"])
+    for k in list(case):
+        if f"in_{k}" in z.files:
+            case[k] = np.asfortranarray(z[f"in_{k}"])
+    out = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
+    return cfg, case, meta, out
+
+
+def _oracle(cfg, case):
+    from oracle.pyoracle import Oracle
+    return Oracle(cfg, case)
+
+
+NAME = {"taux_bot": "vv_taux_bot", "tauy_bot": "vv_tauy_bot"}
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_end_state_matches_reference_golden(path):
+    cfg, case, meta, out = _load(path)
+    o = _oracle(cfg, case)
+    o.step(meta["niter"])
+    for name, ref in out.items():
+        if name.startswith("p_"):
+            continue
+        mine = o.field(NAME.get(name, name))
+        assert np.array_equal(mine, ref), f"{name}: max diff {np.abs(mine - ref).max():.3e}"
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_modules_match_reference_golden(path):
+    """Replays the first predictor with the oracle's module entry points and
+    compares every module's output with the reference module's."""
+    cfg, case, meta, out = _load(path)
+    o = _oracle(cfg, case)
+    f = o.field
+    u, v, h, h0 = f("u"), f("v"), f("h"), f("h0")
+    uh, vh, diffu, diffv, CAu, CAv = f("uh"), f("vh"), f("diffu"), f("diffv"), f("CAu"), f("CAv")
+    up, vp = f("up"), f("vp")
+    per = cfg.periodic
+    h[...] = h0
+    o.call("om_compute_transports", u, v, h, uh, vh)
+    if per:
+        o.halo_fill(uh)
+        o.halo_fill(vh)
+    assert np.array_equal(uh, out["p_uh"]) and np.array_equal(vh, out["p_vh"])
+    o.call("om_hor_visc", u, v, h, diffu, diffv)
+    assert np.array_equal(diffu, out["p_diffu"]) and np.array_equal(diffv, out["p_diffv"])
+    o.call("om_coradcalc", u, v, h, uh, vh, CAu, CAv)
+    isd, ied, jsd, jed, isc, iec, jsc, jec = cfg.bounds
+    cu = np.s_[isc - 1:iec - 1, jsc - 1:jec]
+    cv = np.s_[isc - 1:iec, jsc - 1:jec - 1]
+    assert np.array_equal(CAu[cu], out["p_CAu"][cu]) and np.array_equal(CAv[cv], out["p_CAv"][cv])
+    up[cu] = u[cu] + cfg.dt * (CAu[cu] + diffu[cu])      # rk2_serial_driver.F90:257-270
+    vp[cv] = v[cv] + cfg.dt * (CAv[cv] + diffv[cv])
+    assert np.array_equal(up, out["p_up0"]) and np.array_equal(vp, out["p_vp0"])
+    o.call("om_vert_visc", up, vp, h, scalar=cfg.dt)
+    assert np.array_equal(up, out["p_up"]) and np.array_equal(vp, out["p_vp"])
+    assert np.array_equal(f("vv_taux_bot"), out["p_taux_bot"])
+    o.call("om_btstep", f("eta"), f("ubt"), f("vbt"), f("ubt_av"), f("vbt_av"), f("eta_av"))
+    for k in ("ubt_av", "vbt_av", "eta_av"):
+        assert np.array_equal(f(k), out[f"p_{k}"]), k
+    if per:
+        o.halo_fill(up)
+        o.halo_fill(vp)
+    o.call("om_continuity_ppm", up, h0, h, uh, scalar=cfg.dt)
+    if per:
+        o.halo_fill(h)
+    assert np.array_equal(h, out["p_h"])
+    assert np.array_equal(uh, out["p_uhc"])
+    for k in ("u_cor", "du_cor", "FA_u_EE", "FA_u_E0", "FA_u_W0", "FA_u_WW", "uBT_WW", "uBT_EE", "BT_h_u"):
+        assert np.array_equal(f(k), out[f"p_{k}"]), k
+
+
+@pytest.mark.slow
+def test_oracle_matches_unmodified_reference_driver_printout():
+    """verify_state of app/rk2_serial_driver.F90 (printed to 9 digits) equals the
+    oracle's on the driver's own analytic state."""
+    from oracle import refharness
+    if not refharness.available():
+        pytest.skip("reference sources not present")
+    refharness.build()
+    res = subprocess.run(["bash", "-c", f'ulimit -s unlimited; "{refharness.DRIVER}" 40 40 10 3 20'],
+                         capture_output=True, text=True, check=True).stdout
+    cfg = Config(ni=40, nj=40, nk=10, bt_nsteps=20)
+    o = _oracle(cfg, make_case(cfg, init="reference"))
+    o.step(3)
+    mine = o.verify()
+    keys = ["Initial mass:", "Final mass:", "Rel. error:", "Max |u|:", "Max |v|:", "Max |eta|:", "Total KE:"]
+    for key, val in zip(keys, mine):
+        line = next(l for l in res.splitlines() if key in l)
+        assert line.split(key)[1].strip() == f"{val:.8E}".replace("E-0", "E-0"), (key, line, val)
+
+
+def test_rest_state_stays_at_rest():
+    """No velocity, flat sea surface, no wind: every field is unchanged bit for bit."""
+    cfg = Config(ni=20, nj=18, nk=5, bt_nsteps=10)
+    case = make_case(cfg, init="seeded", seed=3)
+    for k in ("u", "v", "eta", "ubt", "vbt", "taux"):
+        case[k] = np.zeros_like(case[k])
+    case["h0"] = np.full_like(case["h0"], 4000.0 / cfg.nk)
+    o = _oracle(cfg, case)
+    o.step(3)
+    assert not np.any(o.field("u")) and not np.any(o.field("v")) and not np.any(o.field("eta"))
+    assert np.array_equal(o.field("h"), case["h0"])
+
+
+@pytest.mark.parametrize("bc", [(BC_STATIC, BC_STATIC), (BC_PERIODIC, BC_PERIODIC), (BC_PERIODIC, BC_WALL)])
+def test_volume_conserved_to_round_off(bc):
+    """verify_state mass budget over the compute domain (PASS threshold 1e-10;
+    here tighter)."""
+    cfg = Config(ni=26, nj=22, nk=6, bt_nsteps=15, bc_x=bc[0], bc_y=bc[1])
+    o = _oracle(cfg, make_case(cfg, init="seeded", seed=5))
+    o.step(4)
+    assert o.verify()[2] < 1e-13
+
+
+def test_btstep_closed_box_conserves_eta_volume():
+    """The serial btstep's narrow transport ranges close the box: sum(eta*area) is
+    conserved to round-off (mom6_barotropic_serial.F90:263-279)."""
+    cfg = Config(ni=24, nj=20, nk=3, bt_nsteps=25)
+    case = make_case(cfg, init="reference")
+    o = _oracle(cfg, case)
+    isd, ied, jsd, jed, isc, iec, jsc, jec = cfg.bounds
+    eta0 = case["eta"][isc - 1:iec, jsc - 1:jec].sum()
+    f = o.field
+    o.call("om_btstep", f("eta"), f("ubt"), f("vbt"), f("ubt_av"), f("vbt_av"), f("eta_av"))
+    eta1 = f("eta_av")[isc - 1:iec, jsc - 1:jec].sum()
+    assert abs(eta1 - eta0) < 1e-11 * np.abs(case["eta"]).sum()
+    assert not np.array_equal(f("eta_av"), case["eta"])
+
+
+def test_vertical_viscosity_solves_its_tridiagonal_system():
+    """After vert_visc_apply, u_new satisfies the implicit system the Schopf &
+    Loughe sweep solves: (h_k + dt(a_k + a_{k+1} + Ray_k)) u_k - dt a_k u_{k-1}
+    - dt a_{k+1} u_{k+1} = h_k u_k_old (+ dt/rho0 taux at k = 1), with a_1 = 0;
+    checked against a dense solve of the same system."""
+    cfg = Config(ni=14, nj=12, nk=9, bt_nsteps=10)
+    case = make_case(cfg, init="seeded", seed=8)
+    o = _oracle(cfg, case)
+    u_old = o.field("u").copy()
+    u, v, h = o.field("u"), o.field("v"), o.field("h0")
+    dt = cfg.dt
+    o.call("om_vert_visc", u, v, h, scalar=dt)
+    a = o.field("vv_a_u")            # (n1, n2, nk+1)
+    hu = o.field("vv_h_u")
+    ray = o.field("Ray_u")
+    taux = o.field("taux")
+    isd, ied, jsd, jed, isc, iec, jsc, jec = cfg.bounds
+    nk = cfg.nk
+    for (i, j) in [(isc - 1, jsc), (isc + 3, jsc + 2), (iec, jec)]:
+        i0, j0 = i - 1, j - 1
+        A = np.zeros((nk, nk))
+        rhs = hu[i0, j0, :] * u_old[i0, j0, :]
+        rhs[0] += dt / 1035.0 * taux[i0, j0]
+        for k in range(nk):
+            A[k, k] = hu[i0, j0, k] + dt * (a[i0, j0, k] + a[i0, j0, k + 1] + ray[i0, j0, k])
+            if k > 0:
+                A[k, k - 1] = -dt * a[i0, j0, k]
+            if k < nk - 1:
+                A[k, k + 1] = -dt * a[i0, j0, k + 1]
+        assert a[i0, j0, 0] == 0.0
+        sol = np.linalg.solve(A, rhs)
+        np.testing.assert_allclose(u[i0, j0, :], sol, rtol=1e-12, atol=1e-15)
+
+
+def test_ppm_edges_reproduce_linear_profiles():
+    """PPM edge values of a linear thickness profile are exact interface values
+    (limiter inactive): h_E(i) = h_W(i+1) = (h_i + h_{i+1})/2 in the interior."""
+    cfg = Config(ni=20, nj=12, nk=2, bt_nsteps=5)
+    case = make_case(cfg, init="seeded", seed=1)
+    i = np.arange(1, cfg.n1 + 1, dtype=np.float64)[:, None, None]
+    case["h0"] = np.asfortranarray(np.broadcast_to(40.0 + 0.25 * i, case["h0"].shape).copy())
+    case["u"] = np.asfortranarray(np.zeros_like(case["u"]))
+    o = _oracle(cfg, case)
+    f = o.field
+    o.call("om_continuity_ppm", f("u"), f("h0"), f("h"), f("uh"), scalar=cfg.dt)
+    hW, hE = f("cont_hW"), f("cont_hE")
+    isd, ied, jsd, jed, isc, iec, jsc, jec = cfg.bounds
+    hh = case["h0"]
+    for ii in range(isc - 1, iec + 1):        # 1-based interior cells with full stencils
+        np.testing.assert_allclose(hE[ii - 1, jsc - 1:jec], 0.5 * (hh[ii - 1, jsc - 1:jec] + hh[ii, jsc - 1:jec]), rtol=1e-15)
+        np.testing.assert_allclose(hW[ii - 1, jsc - 1:jec], 0.5 * (hh[ii - 2, jsc - 1:jec] + hh[ii - 1, jsc - 1:jec]), rtol=1e-15)
+    # u = 0 everywhere: no flux, h unchanged exactly
+    assert np.array_equal(f("h")[isc - 1:iec, jsc - 1:jec], hh[isc - 1:iec, jsc - 1:jec])
