@@ -12,6 +12,7 @@
 // reference never writes (uhbt on I = is-1 and ie, vhbt on J = js-1 and je
 // with the narrow ranges) read as zero, as they do there.
 #include <utility>
+#include <vector>
 
 #include "mom6h_internal.h"
 
@@ -21,18 +22,49 @@ struct BT {
     Dims d;
     int nstep, first_direction, extended, px, py;
     double dtbt, bebt, dgeo_de;
-    const double *Datu, *Datv, *gtot_E, *gtot_W, *gtot_N, *gtot_S, *f_4_u, *f_4_v;
-    const double *bt_rem_u, *bt_rem_v, *IareaT, *IdxCu, *IdyCv;
-    double *eta, *eta_pred, *ubt, *vbt, *ubt_prev, *vbt_prev, *uhbt, *vhbt;
-    double *PFu, *PFv, *Cor_u, *Cor_v, *ubt_av, *vbt_av;
-    const double *eta_in, *ubt_in, *vbt_in;
-    double *u_av, *v_av, *eta_av;
+    double *ubt_av, *vbt_av;   // running time averages, accumulated in place
 };
 
-__device__ inline double F4(const double *f, const Dims &d, int c, int i, int j)
-{
-    return f[4 * IX2(d, i, j) + (c - 1)];
-}
+// Metric access.  BTArrays reads every coefficient from its 2D array (the
+// general grid); BTUniform holds the coefficients that are the same at every
+// point as values and keeps only Datv (closed faces of walls) and the
+// Coriolis factors (f_4 = one value per point, equal over the four
+// components) as arrays.  The arithmetic is the same for both.
+struct BTArrays {
+    const double *Datu, *Datv, *gtot_E, *gtot_W, *gtot_N, *gtot_S, *f_4_u, *f_4_v;
+    const double *bt_rem_u, *bt_rem_v, *IareaT, *IdxCu, *IdyCv;
+    __device__ double datu(long long c) const { return Datu[c]; }
+    __device__ double datv(long long c) const { return Datv[c]; }
+    __device__ double gE(long long c) const { return gtot_E[c]; }
+    __device__ double gW(long long c) const { return gtot_W[c]; }
+    __device__ double gN(long long c) const { return gtot_N[c]; }
+    __device__ double gS(long long c) const { return gtot_S[c]; }
+    __device__ double f4u(int comp, long long c) const { return f_4_u[4 * c + (comp - 1)]; }
+    __device__ double f4v(int comp, long long c) const { return f_4_v[4 * c + (comp - 1)]; }
+    __device__ double remu(long long c) const { return bt_rem_u[c]; }
+    __device__ double remv(long long c) const { return bt_rem_v[c]; }
+    __device__ double iareaT(long long c) const { return IareaT[c]; }
+    __device__ double idxCu(long long c) const { return IdxCu[c]; }
+    __device__ double idyCv(long long c) const { return IdyCv[c]; }
+};
+
+struct BTUniform {
+    double Datu, gtot, bt_rem_u, bt_rem_v, IareaT, IdxCu, IdyCv;
+    const double *Datv, *f4;
+    __device__ double datu(long long) const { return Datu; }
+    __device__ double datv(long long c) const { return Datv[c]; }
+    __device__ double gE(long long) const { return gtot; }
+    __device__ double gW(long long) const { return gtot; }
+    __device__ double gN(long long) const { return gtot; }
+    __device__ double gS(long long) const { return gtot; }
+    __device__ double f4u(int, long long c) const { return f4[c]; }
+    __device__ double f4v(int, long long c) const { return f4[c]; }
+    __device__ double remu(long long) const { return bt_rem_u; }
+    __device__ double remv(long long) const { return bt_rem_v; }
+    __device__ double iareaT(long long) const { return IareaT; }
+    __device__ double idxCu(long long) const { return IdxCu; }
+    __device__ double idyCv(long long) const { return IdyCv; }
+};
 
 // ---------------------------------------------------------------------------
 // One substep per launch, 2D tiles staged in LDS.  A block owns the output
@@ -63,7 +95,8 @@ __global__ void k_bt_copy_in(Dims d, const double *eta_in, const double *ubt_in,
     vbt_av[q] = 0.0;
 }
 
-__global__ void __launch_bounds__(256) k_bt_substep(BT p, BTStep st)
+template <typename M>
+__global__ void __launch_bounds__(256) k_bt_substep(BT p, M m, BTStep st)
 {
     __shared__ double s_eta[BT_H][BT_W], s_u[BT_H][BT_W], s_v[BT_H][BT_W];
     __shared__ double s_ep[BT_H][BT_W], s_un[BT_H][BT_W], s_vn[BT_H][BT_W];
@@ -93,9 +126,9 @@ __global__ void __launch_bounds__(256) k_bt_substep(BT p, BTStep st)
         const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
         if (i < i0 - 1 || j < j0 - 1 || i < is || i > ie || j < js || j > je) continue;
         const long long c = IX2(d, i, j);
-        s_ep[lj][li] = s_eta[lj][li] + (dtbt * p.IareaT[c]) *
-            (((p.Datu[IX2(d, i - 1, j)] * s_u[lj][li - 1]) - (p.Datu[c] * s_u[lj][li])) +
-             ((p.Datv[IX2(d, i, j - 1)] * s_v[lj - 1][li]) - (p.Datv[c] * s_v[lj][li])));
+        s_ep[lj][li] = s_eta[lj][li] + (dtbt * m.iareaT(c)) *
+            (((m.datu(IX2(d, i - 1, j)) * s_u[lj][li - 1]) - (m.datu(c) * s_u[lj][li])) +
+             ((m.datv(IX2(d, i, j - 1)) * s_v[lj - 1][li]) - (m.datv(c) * s_v[lj][li])));
     }
     __syncthreads();
     // P2: pressure forces (:234-247) and the first velocity update (:250-260)
@@ -107,20 +140,20 @@ __global__ void __launch_bounds__(256) k_bt_substep(BT p, BTStep st)
         const bool in_v = i >= is && i <= ie && j >= js && j <= je - 1;
         double PFu = 0.0, PFv = 0.0;
         if (in_u && i <= i1) {
-            PFu = ((s_ep[lj][li] * p.gtot_E[c]) - (s_ep[lj][li + 1] * p.gtot_W[IX2(d, i + 1, j)])) * p.dgeo_de * p.IdxCu[c];
+            PFu = ((s_ep[lj][li] * m.gE(c)) - (s_ep[lj][li + 1] * m.gW(IX2(d, i + 1, j)))) * p.dgeo_de * m.idxCu(c);
             s_PFu[lj][li] = PFu;
         }
         if (in_v && j <= j1) {
-            PFv = ((s_ep[lj][li] * p.gtot_N[c]) - (s_ep[lj + 1][li] * p.gtot_S[IX2(d, i, j + 1)])) * p.dgeo_de * p.IdyCv[c];
+            PFv = ((s_ep[lj][li] * m.gN(c)) - (s_ep[lj + 1][li] * m.gS(IX2(d, i, j + 1)))) * p.dgeo_de * m.idyCv(c);
             s_PFv[lj][li] = PFv;
         }
         if (v_first) {  // vbt on [i0-1, i1+1] x [j0-1, j1]
             if (j <= j1) {
                 double vn = s_v[lj][li];
                 if (in_v) {
-                    const double Cor = -1.0 * (((F4(p.f_4_v, d, 1, i, j) * s_u[lj][li - 1]) + (F4(p.f_4_v, d, 4, i, j) * s_u[lj + 1][li])) +
-                                               ((F4(p.f_4_v, d, 2, i, j) * s_u[lj][li]) + (F4(p.f_4_v, d, 3, i, j) * s_u[lj + 1][li - 1])));
-                    vn = p.bt_rem_v[c] * (vn + dtbt * (Cor + PFv));
+                    const double Cor = -1.0 * (((m.f4v(1, c) * s_u[lj][li - 1]) + (m.f4v(4, c) * s_u[lj + 1][li])) +
+                                               ((m.f4v(2, c) * s_u[lj][li]) + (m.f4v(3, c) * s_u[lj + 1][li - 1])));
+                    vn = m.remv(c) * (vn + dtbt * (Cor + PFv));
                 }
                 s_vn[lj][li] = vn;
             }
@@ -128,9 +161,9 @@ __global__ void __launch_bounds__(256) k_bt_substep(BT p, BTStep st)
             if (i <= i1) {
                 double un = s_u[lj][li];
                 if (in_u) {
-                    const double Cor = (((F4(p.f_4_u, d, 4, i, j) * s_v[lj][li + 1]) + (F4(p.f_4_u, d, 1, i, j) * s_v[lj - 1][li])) +
-                                        ((F4(p.f_4_u, d, 3, i, j) * s_v[lj][li]) + (F4(p.f_4_u, d, 2, i, j) * s_v[lj - 1][li + 1])));
-                    un = p.bt_rem_u[c] * (un + dtbt * (Cor + PFu));
+                    const double Cor = (((m.f4u(4, c) * s_v[lj][li + 1]) + (m.f4u(1, c) * s_v[lj - 1][li])) +
+                                        ((m.f4u(3, c) * s_v[lj][li]) + (m.f4u(2, c) * s_v[lj - 1][li + 1])));
+                    un = m.remu(c) * (un + dtbt * (Cor + PFu));
                 }
                 s_un[lj][li] = un;
             }
@@ -146,18 +179,18 @@ __global__ void __launch_bounds__(256) k_bt_substep(BT p, BTStep st)
             if (j < j0) continue;
             double un = s_u[lj][li];
             if (i >= is && i <= ie - 1 && j >= js && j <= je) {
-                const double Cor = (((F4(p.f_4_u, d, 4, i, j) * s_vn[lj][li + 1]) + (F4(p.f_4_u, d, 1, i, j) * s_vn[lj - 1][li])) +
-                                    ((F4(p.f_4_u, d, 3, i, j) * s_vn[lj][li]) + (F4(p.f_4_u, d, 2, i, j) * s_vn[lj - 1][li + 1])));
-                un = p.bt_rem_u[c] * (un + dtbt * (Cor + s_PFu[lj][li]));
+                const double Cor = (((m.f4u(4, c) * s_vn[lj][li + 1]) + (m.f4u(1, c) * s_vn[lj - 1][li])) +
+                                    ((m.f4u(3, c) * s_vn[lj][li]) + (m.f4u(2, c) * s_vn[lj - 1][li + 1])));
+                un = m.remu(c) * (un + dtbt * (Cor + s_PFu[lj][li]));
             }
             s_un[lj][li] = un;
         } else {        // vbt on [i0, i1] x [j0-1, j1]
             if (i < i0) continue;
             double vn = s_v[lj][li];
             if (i >= is && i <= ie && j >= js && j <= je - 1) {
-                const double Cor = -1.0 * (((F4(p.f_4_v, d, 1, i, j) * s_un[lj][li - 1]) + (F4(p.f_4_v, d, 4, i, j) * s_un[lj + 1][li])) +
-                                           ((F4(p.f_4_v, d, 2, i, j) * s_un[lj][li]) + (F4(p.f_4_v, d, 3, i, j) * s_un[lj + 1][li - 1])));
-                vn = p.bt_rem_v[c] * (vn + dtbt * (Cor + s_PFv[lj][li]));
+                const double Cor = -1.0 * (((m.f4v(1, c) * s_un[lj][li - 1]) + (m.f4v(4, c) * s_un[lj + 1][li])) +
+                                           ((m.f4v(2, c) * s_un[lj][li]) + (m.f4v(3, c) * s_un[lj + 1][li - 1])));
+                vn = m.remv(c) * (vn + dtbt * (Cor + s_PFv[lj][li]));
             }
             s_vn[lj][li] = vn;
         }
@@ -173,11 +206,11 @@ __global__ void __launch_bounds__(256) k_bt_substep(BT p, BTStep st)
     const int P = ie - is + 1, Q = je - js + 1;
     auto uhbt = [&](int i, int j, int li, int lj) -> double {
         if (i < iu0 || i > iu1 || j < js || j > je) return 0.0;
-        return p.Datu[IX2(d, i, j)] * (trans_wt1 * s_un[lj][li] + trans_wt2 * s_u[lj][li]);
+        return m.datu(IX2(d, i, j)) * (trans_wt1 * s_un[lj][li] + trans_wt2 * s_u[lj][li]);
     };
     auto vhbt = [&](int i, int j, int li, int lj) -> double {
         if (i < is || i > ie || j < jv0 || j > jv1) return 0.0;
-        return p.Datv[IX2(d, i, j)] * (trans_wt1 * s_vn[lj][li] + trans_wt2 * s_v[lj][li]);
+        return m.datv(IX2(d, i, j)) * (trans_wt1 * s_vn[lj][li] + trans_wt2 * s_v[lj][li]);
     };
     for (int t = tid; t < BT_TX * BT_TY; t += blockDim.x) {
         const int i = i0 + t % BT_TX, j = j0 + t / BT_TX;
@@ -189,7 +222,7 @@ __global__ void __launch_bounds__(256) k_bt_substep(BT p, BTStep st)
         const bool in_v = i >= is && i <= ie && j >= js && j <= je - 1;
         double e = s_eta[lj][li];
         if (in_eta)
-            e = e - dtbt * p.IareaT[c] * ((uhbt(i, j, li, lj) - uhbt(i - 1, j, li - 1, lj)) +
+            e = e - dtbt * m.iareaT(c) * ((uhbt(i, j, li, lj) - uhbt(i - 1, j, li - 1, lj)) +
                                           (vhbt(i, j, li, lj) - vhbt(i, j - 1, li, lj - 1)));
         // time averages; ubt/vbt carry their new value only on the update ranges
         const double un = in_u ? s_un[lj][li] : s_u[lj][li];
@@ -264,10 +297,33 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
     for (size_t q = 0; q < sizeof(src) / sizeof(src[0]); ++q) rc = rc ? rc : dev_upload(dst[q], src[q], n2);
     rc = rc ? rc : dev_upload(&cs->f_4_u, f_4_u, 4 * n2);
     rc = rc ? rc : dev_upload(&cs->f_4_v, f_4_v, 4 * n2);
-    for (double **pp : {&cs->eta, &cs->eta_pred, &cs->ubt, &cs->vbt, &cs->ubt_prev, &cs->vbt_prev,
-                        &cs->uhbt, &cs->vhbt, &cs->PFu, &cs->PFv, &cs->Cor_u, &cs->Cor_v,
-                        &cs->ubt_av, &cs->vbt_av})
+    for (double **pp : {&cs->eta, &cs->ubt, &cs->vbt, &cs->eta_b, &cs->ubt_b, &cs->vbt_b, &cs->ubt_av, &cs->vbt_av})
         rc = rc ? rc : dev_alloc(pp, n2, true);
+    // uniform coefficients: every value of a field equal, and the four Coriolis
+    // components equal to each other and between u and v points
+    if (!rc) {
+        auto same = [&](const double *a, double v) {
+            for (size_t q = 0; q < n2; ++q)
+                if (a[q] != v) return false;
+            return true;
+        };
+        bool uni = same(Datu, Datu[0]) && same(gtot_E, gtot_E[0]) && same(gtot_W, gtot_E[0]) &&
+                   same(gtot_N, gtot_E[0]) && same(gtot_S, gtot_E[0]) && same(bt_rem_u, bt_rem_u[0]) &&
+                   same(bt_rem_v, bt_rem_v[0]) && same(IareaT, IareaT[0]) && same(IdxCu, IdxCu[0]) &&
+                   same(IdyCv, IdyCv[0]);
+        std::vector<double> f(n2);
+        for (size_t q = 0; uni && q < n2; ++q) {
+            f[q] = f_4_u[4 * q];
+            for (int c = 0; c < 4; ++c)
+                if (f_4_u[4 * q + c] != f[q] || f_4_v[4 * q + c] != f[q]) uni = false;
+        }
+        if (uni) {
+            cs->uniform = 1;
+            cs->u_Datu = Datu[0]; cs->u_gtot = gtot_E[0]; cs->u_rem_u = bt_rem_u[0]; cs->u_rem_v = bt_rem_v[0];
+            cs->u_IareaT = IareaT[0]; cs->u_IdxCu = IdxCu[0]; cs->u_IdyCv = IdyCv[0];
+            rc = dev_upload(&cs->f4, f.data(), n2);
+        }
+    }
     if (rc) {
         mom6h_barotropic_end(cs);
         return rc;
@@ -287,16 +343,7 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
     p.px = cs->halo.px; p.py = cs->halo.py;
     p.extended = (p.px || p.py) ? 1 : 0;
     p.dtbt = cs->dtbt; p.bebt = cs->bebt; p.dgeo_de = cs->dgeo_de;
-    p.Datu = cs->Datu; p.Datv = cs->Datv; p.gtot_E = cs->gtot_E; p.gtot_W = cs->gtot_W;
-    p.gtot_N = cs->gtot_N; p.gtot_S = cs->gtot_S; p.f_4_u = cs->f_4_u; p.f_4_v = cs->f_4_v;
-    p.bt_rem_u = cs->bt_rem_u; p.bt_rem_v = cs->bt_rem_v; p.IareaT = cs->IareaT;
-    p.IdxCu = cs->IdxCu; p.IdyCv = cs->IdyCv;
-    p.eta = cs->eta; p.eta_pred = cs->eta_pred; p.ubt = cs->ubt; p.vbt = cs->vbt;
-    p.ubt_prev = cs->ubt_prev; p.vbt_prev = cs->vbt_prev; p.uhbt = cs->uhbt; p.vhbt = cs->vhbt;
-    p.PFu = cs->PFu; p.PFv = cs->PFv; p.Cor_u = cs->Cor_u; p.Cor_v = cs->Cor_v;
     p.ubt_av = cs->ubt_av; p.vbt_av = cs->vbt_av;
-    p.eta_in = eta_in; p.ubt_in = ubt_in; p.vbt_in = vbt_in;
-    p.u_av = u_av; p.v_av = v_av; p.eta_av = eta_av;
     // copy-in, then one launch per substep ping-ponging between the two state sets
     hipStream_t strm = (hipStream_t)stream;
     const Dims &d = cs->d;
@@ -307,14 +354,22 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
         MOM6H_LAUNCH_CHECK();
     }
     dim3 grid((d.n1 + BT_TX - 1) / BT_TX, (d.n2 + BT_TY - 1) / BT_TY), block(256);
-    double *A[3] = {cs->eta, cs->ubt, cs->vbt}, *B[3] = {cs->eta_pred, cs->ubt_prev, cs->vbt_prev};
+    double *A[3] = {cs->eta, cs->ubt, cs->vbt}, *B[3] = {cs->eta_b, cs->ubt_b, cs->vbt_b};
     for (int n = 1; n <= cs->nstep; ++n) {
         BTStep st;
         st.eta_o = A[0]; st.ubt_o = A[1]; st.vbt_o = A[2];
         st.eta_n = B[0]; st.ubt_n = B[1]; st.vbt_n = B[2];
         st.out_u = u_av; st.out_v = v_av; st.out_eta = eta_av;
         st.n = n;
-        k_bt_substep<<<grid, block, 0, strm>>>(p, st);
+        if (cs->uniform) {
+            BTUniform m{cs->u_Datu, cs->u_gtot, cs->u_rem_u, cs->u_rem_v, cs->u_IareaT, cs->u_IdxCu, cs->u_IdyCv,
+                        cs->Datv, cs->f4};
+            k_bt_substep<BTUniform><<<grid, block, 0, strm>>>(p, m, st);
+        } else {
+            BTArrays m{cs->Datu, cs->Datv, cs->gtot_E, cs->gtot_W, cs->gtot_N, cs->gtot_S, cs->f_4_u, cs->f_4_v,
+                       cs->bt_rem_u, cs->bt_rem_v, cs->IareaT, cs->IdxCu, cs->IdyCv};
+            k_bt_substep<BTArrays><<<grid, block, 0, strm>>>(p, m, st);
+        }
         MOM6H_LAUNCH_CHECK();
         for (int q = 0; q < 3; ++q) std::swap(A[q], B[q]);
     }
@@ -325,9 +380,8 @@ int mom6h_barotropic_end(mom6h_barotropic_cs *cs)
 {
     if (!cs) return MOM6H_OK;
     for (double *p : {cs->Datu, cs->Datv, cs->gtot_E, cs->gtot_W, cs->gtot_N, cs->gtot_S, cs->f_4_u,
-                      cs->f_4_v, cs->bt_rem_u, cs->bt_rem_v, cs->IareaT, cs->IdxCu, cs->IdyCv,
-                      cs->eta, cs->eta_pred, cs->ubt, cs->vbt, cs->ubt_prev, cs->vbt_prev, cs->uhbt,
-                      cs->vhbt, cs->PFu, cs->PFv, cs->Cor_u, cs->Cor_v, cs->ubt_av, cs->vbt_av})
+                      cs->f_4_v, cs->bt_rem_u, cs->bt_rem_v, cs->IareaT, cs->IdxCu, cs->IdyCv, cs->f4,
+                      cs->eta, cs->ubt, cs->vbt, cs->eta_b, cs->ubt_b, cs->vbt_b, cs->ubt_av, cs->vbt_av})
         dev_free(p);
     delete cs;
     return MOM6H_OK;

@@ -33,19 +33,16 @@ __device__ inline double ppm_slope(const Dims &d, const double *__restrict__ h,
     return copysign(1.0, s) * dmin(fabs(s), 2.0 * dmin(dMx, dMn));
 }
 
-__global__ void k_ppm_edges(Dims d, const double *__restrict__ h, const double *__restrict__ mT,
-                            double *__restrict__ hW, double *__restrict__ hE, int upwind_1st,
-                            int simple_2nd, int monotonic, double h_min)
+// Edge values of cell (i, j, k) (zonal_edge_thickness :185-207 with
+// PPM_reconstruction_x :1124-1247 and its limiter).
+__device__ inline void ppm_edge(const Dims &d, const double *__restrict__ h, const double *__restrict__ mT,
+                                int i, int j, int k, int upwind_1st, int simple_2nd, int monotonic,
+                                double h_min, double &hW, double &hE)
 {
-    const int i = d.isc - 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = d.jsc + blockIdx.y;
-    const int k = 1 + blockIdx.z;
-    if (i > d.iec + 1) return;
-    const long long c = IX3(d, i, j, k);
-    const double hi = h[c];
-    if (upwind_1st) {  // zonal_edge_thickness (:199-207): no reconstruction, no limiter
-        hW[c] = hi;
-        hE[c] = hi;
+    const double hi = h[IX3(d, i, j, k)];
+    if (upwind_1st) {  // (:199-207): no reconstruction, no limiter
+        hW = hi;
+        hE = hi;
         return;
     }
     const double mm = mT[IX2(d, i - 1, j)], mp = mT[IX2(d, i + 1, j)];
@@ -96,6 +93,21 @@ __global__ void k_ppm_edges(Dims d, const double *__restrict__ h, const double *
             }
         }
     }
+    hW = hl;
+    hE = hr;
+}
+
+__global__ void k_ppm_edges(Dims d, const double *__restrict__ h, const double *__restrict__ mT,
+                            double *__restrict__ hW, double *__restrict__ hE, int upwind_1st,
+                            int simple_2nd, int monotonic, double h_min)
+{
+    const int i = d.isc - 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.jsc + blockIdx.y;
+    const int k = 1 + blockIdx.z;
+    if (i > d.iec + 1) return;
+    const long long c = IX3(d, i, j, k);
+    double hl, hr;
+    ppm_edge(d, h, mT, i, j, k, upwind_1st, simple_2nd, monotonic, h_min, hl, hr);
     hW[c] = hl;
     hE[c] = hr;
 }
@@ -317,6 +329,465 @@ __global__ void k_mass_flux(Dims d, const double *__restrict__ u, const double *
                      : (1.5 * (duR - du0)) * ((FAmt_R - FA_avg) / (FAmt_R - FA_0));
 }
 
+// ----------------------------------------------------- block column path ---
+// A block of CW wavefronts owns CF = 64 consecutive faces I0-1 .. I0+62
+// (one face per lane, the same face in every wavefront) and all their levels:
+// wavefront w holds levels w*K+1 .. w*K+K of its lane's face in registers (u,
+// the PPM edges and thickness of both neighbouring cells, the layer transport
+// and marginal thickness of the last evaluation).  Every pass over the columns
+// (first fluxes, each Newton evaluation, the BT_cont evaluations) evaluates all
+// levels in parallel from registers, leaves the per-level terms in LDS, and
+// then one wavefront per column sum adds them up in the reference's k order,
+// one lane per face -- a sequential sum costs 64 faces one add per level.
+// The Newton logic of a face runs redundantly in every wavefront on the same
+// values, so all copies agree; the block keeps iterating while any face
+// needs another evaluation.  The block updates the 63 cells I0 .. I0+62, whose
+// convergence needs the transport at the face to the west (the neighbouring
+// lane); neighbouring blocks overlap by one face, which both evaluate
+// identically and only one writes.
+constexpr int CF = 64;
+
+struct ContArgs {
+    Dims d;
+    const double *u, *hin;
+    double *h, *uh;
+    double dt;
+    const double *uhbt;
+    const double *IareaT, *IdxT, *dy_Cu, *dxT, *dxCu, *mask2dCu, *mask2dT;
+    double tol_eta0, tol_vel, CFL_limit_adjust, h_min;
+    double *u_cor, *du_cor;
+    double *FA_u_EE, *FA_u_E0, *FA_u_W0, *FA_u_WW, *uBT_WW, *uBT_EE, *bt_h_u;
+};
+
+// zonal_flux_layer (:516-598) with the upwind cell selected instead of
+// branched; the same operations in the same order as zflux().
+__device__ __forceinline__ void zflux_sel(double u, double dt, double dy, double IdxT0, double IdxT1, double hW0,
+                                 double hE0, double h0, double hW1, double hE1, double h1, double &uh,
+                                 double &hmarg)
+{
+    const bool pos = u > 0.0;
+    const double A = pos ? hE0 : hW1;       // upwind edge facing the face
+    const double B = pos ? hW0 : hE1;       // upwind far edge
+    const double hc = pos ? h0 : h1;
+    const double c = u * dt * (pos ? IdxT0 : IdxT1);
+    const double CFL = pos ? c : -c;
+    const double curv_3 = (B + A) - 2.0 * hc;
+    const double D = B - A;
+    const double f = (dy * u) * (A + CFL * (0.5 * D + curv_3 * (CFL - 1.5)));
+    const double hm = A + CFL * (D + 3.0 * curv_3 * (CFL - 1.0));
+    const bool zero = !(u > 0.0) && !(u < 0.0);
+    uh = zero ? 0.0 : f;
+    hmarg = zero ? 0.5 * (hW1 + hE0) : hm;
+}
+
+// PPM_reconstruction_x (:1124-1247) slope from the three cells, branch-free
+__device__ __forceinline__ double ppm_slope_v(double mL, double m0, double mR, double hm, double h0, double hp)
+{
+    const double s = 0.5 * (hp - hm);
+    const double dMx = dmax(dmax(hp, hm), h0) - h0;
+    const double dMn = h0 - dmin(dmin(hp, hm), h0);
+    const double slp = copysign(1.0, s) * dmin(fabs(s), 2.0 * dmin(dMx, dMn));
+    return ((mL * m0 * mR) == 0.0) ? 0.0 : slp;
+}
+
+// Unlimited PPM edges of the cell at offset q of h6/m6 (cells i-2 .. i+3)
+__device__ __forceinline__ void ppm_edges_v(const double *m6, const double *h6, const double *sl, int q, double &hl,
+                                   double &hr)
+{
+    const double hi = h6[q];
+    const double mm = m6[q - 1], mp = m6[q + 1];
+    const double h_im1 = mm * h6[q - 1] + (1.0 - mm) * hi;
+    const double h_ip1 = mp * h6[q + 1] + (1.0 - mp) * hi;
+    const double oneSixth = 1.0 / 6.0;
+    hl = 0.5 * (h_im1 + hi) + oneSixth * (sl[q - 2] - sl[q - 1]);
+    hr = 0.5 * (h_ip1 + hi) + oneSixth * (sl[q - 1] - sl[q]);
+}
+
+// PPM_limit_pos (:1253-1294) / PPM_limit_CW84 (:1298-1338) on one cell
+__device__ __forceinline__ void ppm_limit(int monotonic, double hi, double h_min, double &hl, double &hr)
+{
+    if (monotonic) {
+        const float h_i = (float)hi;
+        if ((hr - (double)h_i) * ((double)h_i - hl) <= 0.0) {
+            hl = (double)h_i;
+            hr = (double)h_i;
+        } else {
+            const float RLdiff = (float)(hr - hl);
+            const float RLmean = (float)(0.5 * (hr + hl));
+            const float FunFac = 6.0f * RLdiff * (h_i - RLmean);
+            const float RLdiff2 = RLdiff * RLdiff;
+            if (FunFac > RLdiff2) hl = (double)(3.0f * h_i) - 2.0 * hr;
+            if (FunFac < -RLdiff2) hr = (double)(3.0f * h_i) - 2.0 * hl;
+        }
+        return;
+    }
+    const float curv = (float)(3.0 * ((hl + hr) - 2.0 * hi));
+    if ((double)curv > 0.0) {
+        const float dh = (float)(hr - hl);
+        if (fabsf(dh) < curv) {
+            if (hi <= h_min) {
+                hl = hi;
+                hr = hi;
+            } else {
+                const double lhs = (double)(12.0f * curv) * (hi - h_min);
+                const float rhs = curv * curv + 3.0f * (dh * dh);
+                if (lhs < (double)rhs) {
+                    const float scale = (float)(lhs / (double)rhs);
+                    hl = hi + (double)scale * (hl - hi);
+                    hr = hi + (double)scale * (hr - hi);
+                }
+            }
+        }
+    }
+}
+
+template <int K>
+struct ColRegs {
+    double wW[K], wE[K], w0[K], eW[K], eE[K], e0[K];
+    double dt, dy, IdxT0, IdxT1;
+    __device__ __forceinline__ void eval(int kk, double un, double &f, double &hm) const
+    {
+        zflux_sel(un, dt, dy, IdxT0, IdxT1, wW[kk], wE[kk], w0[kk], eW[kk], eE[kk], e0[kk], f, hm);
+    }
+    // Marks the stored column as rewritten, so that the compiler re-derives the
+    // per-level terms of the flux in every pass instead of keeping them live
+    // across passes (which would double the registers per level).
+    __device__ inline void fence()
+    {
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk)
+            asm volatile("" : "+v"(wW[kk]), "+v"(wE[kk]), "+v"(w0[kk]), "+v"(eW[kk]), "+v"(eE[kk]), "+v"(e0[kk]));
+    }
+};
+
+// LDS of one block: U = u of every level (levels past nk repeat level nk; u
+// lives here rather than in registers), S0/S1 = per-level terms of a pass,
+// R = results of the column sums (one row per quantity)
+template <int CW, int K>
+struct ContShared {
+    static constexpr int L = CW * K;
+    double U[L][CF], S0[L][CF], S1[L][CF], R[4][CF];
+};
+
+template <int CW, int K>
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(1, (CW >= 16) ? 4 : 2)))
+k_continuity_blk(ContArgs a, int monotonic)
+{
+    extern __shared__ double smem_raw[];
+    ContShared<CW, K> &S = *reinterpret_cast<ContShared<CW, K> *>(smem_raw);
+    const Dims &d = a.d;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int j = d.jsc + blockIdx.y;
+    const int I_raw = d.isc - 1 + blockIdx.x * (CF - 1) + lane;
+    const int I = I_raw > d.iec ? d.iec : I_raw;            // clamped: evaluated, never written
+    const bool live = I_raw <= d.iec;
+    const int i = I;
+    const int nk = d.nk;
+    const long long s = d.s2;
+    const int k0 = w * K;                                    // my levels: k0+1 .. k0+K
+    auto valid = [&](int kk) __attribute__((always_inline)) { return k0 + kk < nk; };
+
+    ColRegs<K> col;
+    col.dt = a.dt;
+    const long long c2 = IX2(d, I, j);
+    col.dy = a.dy_Cu[c2];
+    col.IdxT0 = a.IdxT[IX2(d, i, j)];
+    col.IdxT1 = a.IdxT[IX2(d, i + 1, j)];
+    const double mCu = a.mask2dCu[c2];
+    const long long cu = IX3(d, I, j, 1);
+    // PPM edges of cells i and i+1 from h and mask2dT at i-2 .. i+3
+    // (zonal_edge_thickness :185-207); levels past nk repeat level nk
+    {
+        double m6[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) m6[q] = a.mask2dT[IX2(d, i - 2 + q, j)];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const int kr = k0 + kk + 1;
+            const int k = kr < nk ? kr : nk;
+            const long long o = (long long)(k - 1) * s;
+            S.U[k0 + kk][lane] = a.u[cu + o];
+            double h6[6], sl[4];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) h6[q] = a.hin[IX3(d, i - 2 + q, j, 1) + o];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sl[q] = ppm_slope_v(m6[q], m6[q + 1], m6[q + 2], h6[q], h6[q + 1], h6[q + 2]);
+            ppm_edges_v(m6, h6, sl, 2, col.wW[kk], col.wE[kk]);
+            ppm_edges_v(m6, h6, sl, 3, col.eW[kk], col.eE[kk]);
+            col.w0[kk] = h6[2];
+            col.e0[kk] = h6[3];
+        }
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            ppm_limit(monotonic, col.w0[kk], a.h_min, col.wW[kk], col.wE[kk]);
+            ppm_limit(monotonic, col.e0[kk], a.h_min, col.eW[kk], col.eE[kk]);
+        }
+    }
+
+    auto uk = [&](int kk) __attribute__((always_inline)) -> double { return S.U[k0 + kk][lane]; };
+    double fk[K];   // layer transports of the last evaluation
+    // one pass: evaluate every level at u + du; terms (fk, dy*hm) to LDS
+    auto eval_pass = [&](double du, bool on) __attribute__((always_inline)) {
+        if (!on) return;
+        col.fence();
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            double hm;
+            col.eval(kk, uk(kk) + du, fk[kk], hm);
+            if (valid(kk)) {
+                S.S0[k0 + kk][lane] = fk[kk];
+                S.S1[k0 + kk][lane] = col.dy * hm;
+            }
+        }
+    };
+    // column sums in k order by wavefronts 0 and 1 (R[0] = x0 + sum S0, R[1] = x1 + sum S1)
+    auto sum_pass = [&](double x0, double x1, bool on) __attribute__((always_inline)) {
+        __syncthreads();
+        if (w < 2 && on) {
+            double acc = (w == 0) ? x0 : x1;
+            if (w == 0)
+                for (int k = 0; k < nk; ++k) acc = acc + S.S0[k][lane];
+            else
+                for (int k = 0; k < nk; ++k) acc = acc + S.S1[k][lane];
+            S.R[w][lane] = acc;
+        }
+        __syncthreads();
+    };
+
+    // first layer fluxes, their sums and the CFL bracket (zonal_mass_flux :300-393)
+    const double CFL_dt = a.CFL_limit_adjust / a.dt;
+    const double dx_W = a.dxT[IX2(d, i, j)], dx_E = a.dxT[IX2(d, i + 1, j)];
+    eval_pass(0.0, true);
+    __syncthreads();
+    if (w < 2) {
+        double acc = 0.0;
+        if (w == 0)
+            for (int k = 0; k < nk; ++k) acc = acc + S.S0[k][lane];
+        else
+            for (int k = 0; k < nk; ++k) acc = acc + S.S1[k][lane];
+        S.R[w][lane] = acc;
+    } else if (w == 2) {
+        double x = 2.0 * (CFL_dt * dx_W) * 1.0;
+        for (int k = 0; k < nk; ++k) {
+            const double uu = S.U[k][lane];
+            if (x > dx_W * CFL_dt - uu * mCu) x = (dx_W * CFL_dt - uu);
+        }
+        S.R[2][lane] = x;
+    } else if (w == 3) {
+        double x = -(2.0 * (CFL_dt * dx_E) * 1.0);
+        for (int k = 0; k < nk; ++k) {
+            const double uu = S.U[k][lane];
+            if (x < -dx_E * CFL_dt - uu * mCu) x = -(dx_E * CFL_dt + uu);
+        }
+        S.R[3][lane] = x;
+    }
+    __syncthreads();
+    const double uh_tot_0 = S.R[0][lane], duhdu_tot_0 = S.R[1][lane];
+    const double du_max_CFL = dmax(S.R[2][lane], 0.0), du_min_CFL = dmin(S.R[3][lane], 0.0);
+    const double dtIarea = a.dt * dmin(a.IareaT[IX2(d, i, j)], a.IareaT[IX2(d, i + 1, j)]);
+    const double bt = a.uhbt ? a.uhbt[c2] : 0.0;
+
+    // zonal_flux_adjust (:739-916).  keep: uh_3d present (fluxes of the last
+    // evaluation are the result; evaluated also at itt = max_itts).  Returns
+    // du; *du_eval = du of the last evaluation that was summed (NaN if none).
+    auto flux_adjust = [&](double uhbt, bool keep, double *du_eval) __attribute__((always_inline)) -> double {
+        const int max_itts = 20;
+        double du = 0.0, du_max = du_max_CFL, du_min = du_min_CFL;
+        double uh_err = uh_tot_0 - uhbt, duhdu_tot = duhdu_tot_0;
+        double uh_err_best = fabs(uh_err);
+        bool do_I = true;
+        *du_eval = 0.0;   // the first pass evaluated at du = 0
+        for (int itt = 1; itt <= max_itts; ++itt) {
+            if (do_I) {
+                double tol_eta;
+                if (itt <= 1) tol_eta = (double)1e-6f * a.tol_eta0;
+                else if (itt == 2) tol_eta = (double)1e-4f * a.tol_eta0;
+                else if (itt == 3) tol_eta = (double)1e-2f * a.tol_eta0;
+                else tol_eta = a.tol_eta0;
+                if (uh_err > 0.0) du_max = du;
+                else if (uh_err < 0.0) du_min = du;
+                else do_I = false;
+                if (do_I) {
+                    const double aerr = fabs(uh_err);
+                    if ((dtIarea * aerr > tol_eta) || ((aerr > a.tol_vel * duhdu_tot) || (aerr > uh_err_best))) {
+                        const double ddu = -uh_err / duhdu_tot;
+                        const double du_prev = du;
+                        du = du + ddu;
+                        if (fabs(ddu) < (double)1.0e-15f * fabs(du)) {
+                            do_I = false;
+                        } else if (ddu > 0.0) {
+                            if (du >= du_max) {
+                                du = 0.5 * (du_prev + du_max);
+                                if (du_max - du_prev < (double)1.0e-15f * fabs(du)) do_I = false;
+                            }
+                        } else {
+                            if (du <= du_min) {
+                                du = 0.5 * (du_prev + du_min);
+                                if (du_prev - du_min < (double)1.0e-15f * fabs(du)) do_I = false;
+                            }
+                        }
+                    } else {
+                        do_I = false;
+                    }
+                }
+            }
+            // a finished column keeps its du for the rest of the reference's row loop
+            const bool ev = do_I && ((itt < max_itts) || keep);
+            if (!__syncthreads_or(ev)) break;
+            eval_pass(du, ev);
+            const bool summed = ev && itt < max_itts;
+            sum_pass(-uhbt, 0.0, summed);
+            if (summed) {
+                uh_err = S.R[0][lane];
+                duhdu_tot = S.R[1][lane];
+                uh_err_best = dmin(uh_err_best, fabs(uh_err));
+                *du_eval = du;
+            } else if (ev) {
+                *du_eval = NAN;
+            }
+        }
+        return du;
+    };
+    double du_eval;
+    const double du = flux_adjust(bt, true, &du_eval);
+    // here fk holds the layer fluxes of the last evaluation and, when
+    // du_eval == du, R[1] the sum of dy*hm at u + du
+    const double sum_dyhm = S.R[1][lane];
+    const bool last_at_du = (du_eval == du);
+
+    // outputs at the face: uh, u_cor, du_cor; a face shared with the western
+    // neighbour block is written by that block
+    const bool writes_face = live && (lane > 0 || blockIdx.x == 0);
+    if (writes_face) {
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            if (valid(kk)) {
+                const long long o = cu + (long long)(k0 + kk) * s;
+                a.uh[o] = fk[kk];
+                if (a.u_cor) a.u_cor[o] = uk(kk) + du;
+            }
+        }
+        if (a.du_cor && w == 0) a.du_cor[c2] = du;
+    }
+    // continuity_zonal_convergence (:148-176) for cell i = I (faces I-1 and I)
+    {
+        const double dtIareaT = a.dt * a.IareaT[IX2(d, i, j)];
+        const bool writes_cell = live && lane > 0 && i >= d.isc;
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const double fw = __shfl_up(fk[kk], 1, 64);   // face I-1, same level
+            if (writes_cell && valid(kk)) {
+                const double v = col.w0[kk] - dtIareaT * (fk[kk] - fw);
+                a.h[IX3(d, i, j, k0 + kk + 1)] = dmax(v, 0.0);
+            }
+        }
+    }
+    if (!a.bt_h_u) return;   // block-uniform
+
+    // set_zonal_BT_cont (:920-1121).  The zero-transport correction equals du
+    // whenever the target is zero (identical Newton iterations).
+    double du0 = du, du0_eval = du_eval;
+    if (__syncthreads_or(bt != 0.0)) {
+        double de;
+        const double d2 = flux_adjust(0.0, false, &de);
+        if (bt != 0.0) { du0 = d2; du0_eval = de; }
+    }
+    // FAmt_0 = sum dy*hm(u + du0): the sum of the last evaluation when that
+    // was at du0 == du, else a pass of its own
+    const bool have0 = (bt == 0.0) && last_at_du && (du0_eval == du0);
+    double FAmt_0 = sum_dyhm;
+    if (__syncthreads_or(!have0)) {
+        eval_pass(du0, !have0);
+        sum_pass(0.0, 0.0, !have0);
+        if (!have0) FAmt_0 = S.R[1][lane];
+    }
+    // h_u = marginal thickness at u_cor = u + du (zonal_flux_thickness :602-735)
+    if (writes_face) {
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            double fc, hm;
+            col.eval(kk, uk(kk) + du, fc, hm);
+            if (valid(kk)) a.bt_h_u[cu + (long long)(k0 + kk) * s] = hm;
+        }
+    }
+    // test velocities duR, duL (visc_rem_lim = 1)
+    const double Idt = 1.0 / a.dt;
+    const double CFL_min = (double)1e-6f;
+    const double du_CFL = (CFL_min * Idt) * a.dxCu[c2];
+    __syncthreads();
+    if (w == 0) {
+        double x = dmin(0.0, du0 - du_CFL);
+        for (int k = 0; k < nk; ++k) {
+            const double uu = S.U[k][lane];
+            if (uu + x > -du_CFL) x = -(uu + du_CFL);
+        }
+        S.R[2][lane] = x;
+    } else if (w == 1) {
+        double x = dmax(0.0, du0 + du_CFL);
+        for (int k = 0; k < nk; ++k) {
+            const double uu = S.U[k][lane];
+            if (uu + x < du_CFL) x = -(uu - du_CFL);
+        }
+        S.R[3][lane] = x;
+    }
+    __syncthreads();
+    const double duR = S.R[2][lane], duL = S.R[3][lane];
+    // FAmt_L, uhtot_L at u + duL; FAmt_R, uhtot_R at u + duR
+    eval_pass(duL, true);
+    sum_pass(0.0, 0.0, true);
+    const double uhtot_L = S.R[0][lane], FAmt_L = S.R[1][lane];
+    eval_pass(duR, true);
+    sum_pass(0.0, 0.0, true);
+    const double uhtot_R = S.R[0][lane], FAmt_R = S.R[1][lane];
+    if (!writes_face || w != 0) return;
+    double FA_0 = FAmt_0, FA_avg = FAmt_0;
+    if ((duL - du0) != 0.0) FA_avg = uhtot_L / (duL - du0);
+    if (FA_avg > dmax(FA_0, FAmt_L)) FA_avg = dmax(FA_0, FAmt_L);
+    else if (FA_avg < dmin(FA_0, FAmt_L)) FA_0 = FA_avg;
+    a.FA_u_W0[c2] = FA_0;
+    a.FA_u_WW[c2] = FAmt_L;
+    a.uBT_WW[c2] = (fabs(FA_0 - FAmt_L) <= (double)1e-12f * FA_0)
+                       ? 0.0
+                       : (1.5 * (duL - du0)) * ((FAmt_L - FA_avg) / (FAmt_L - FA_0));
+    FA_0 = FAmt_0;
+    FA_avg = FAmt_0;
+    if ((duR - du0) != 0.0) FA_avg = uhtot_R / (duR - du0);
+    if (FA_avg > dmax(FA_0, FAmt_R)) FA_avg = dmax(FA_0, FAmt_R);
+    else if (FA_avg < dmin(FA_0, FAmt_R)) FA_0 = FA_avg;
+    a.FA_u_E0[c2] = FA_0;
+    a.FA_u_EE[c2] = FAmt_R;
+    a.uBT_EE[c2] = (fabs(FAmt_R - FA_0) <= (double)1e-12f * FA_0)
+                       ? 0.0
+                       : (1.5 * (duR - du0)) * ((FAmt_R - FA_avg) / (FAmt_R - FA_0));
+}
+
+// the driver's `h = h0` reset for the cells the convergence does not write:
+// rows outside jsc..jec in full, and the isd..isc-1 / iec+1..ied strips of the
+// compute rows (one thread per halo cell of a layer)
+__global__ void k_halo_copy(Dims d, const double *__restrict__ hin, double *__restrict__ h)
+{
+    const int k = 1 + blockIdx.y;
+    const int rows_out = (d.jsc - d.jsd) + (d.jed - d.jec);
+    const int side = (d.isc - d.isd) + (d.ied - d.iec);
+    const long long n_rows = (long long)rows_out * d.n1;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j;
+    if (t < n_rows) {
+        const int r = (int)(t / d.n1);
+        i = d.isd + (int)(t % d.n1);
+        j = r < d.jsc - d.jsd ? d.jsd + r : d.jec + 1 + (r - (d.jsc - d.jsd));
+    } else {
+        const long long t2 = t - n_rows;
+        if (t2 >= (long long)side * (d.jec - d.jsc + 1)) return;
+        const int c = (int)(t2 % side);
+        j = d.jsc + (int)(t2 / side);
+        i = c < d.isc - d.isd ? d.isd + c : d.iec + 1 + (c - (d.isc - d.isd));
+    }
+    const long long o = IX3(d, i, j, k);
+    h[o] = hin[o];
+}
+
 // continuity_zonal_convergence (:148-176) with hin; optionally copies hin into
 // the halo of h (the driver's `h = h0` reset folded into the predictor call).
 __global__ void k_convergence(Dims d, const double *hin, const double *__restrict__ uh, double dt,
@@ -363,6 +834,20 @@ int mom6h_continuity_init(mom6h_continuity_cs **out, int isd, int ied, int jsd, 
     cs->tol_eta = 1.0e-12;
     cs->tol_vel = 3.0e8;
     cs->CFL_limit_adjust = 0.5;
+    {
+        // the block kernels keep a whole column block in LDS (up to 125 KB)
+        int dev = 0, lds = 0;
+        MOM6H_CHECK(hipGetDevice(&dev));
+        MOM6H_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        cs->max_lds = lds;
+        const void *kern[] = {(const void *)k_continuity_blk<8, 1>, (const void *)k_continuity_blk<8, 2>,
+                              (const void *)k_continuity_blk<8, 4>, (const void *)k_continuity_blk<8, 10>};
+        const int bytes[] = {(int)sizeof(ContShared<8, 1>), (int)sizeof(ContShared<8, 2>),
+                             (int)sizeof(ContShared<8, 4>), (int)sizeof(ContShared<8, 10>)};
+        for (int q = 0; q < 4; ++q)
+            if (bytes[q] <= lds)
+                MOM6H_CHECK(hipFuncSetAttribute(kern[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q]));
+    }
     const size_t n2 = (size_t)cs->d.s2, n3 = n2 * nk;
     int rc = 0;
     rc = rc ? rc : dev_upload(&cs->IareaT, IareaT, n2);
@@ -394,6 +879,29 @@ int mom6h_continuity_ppm(mom6h_continuity_cs *cs, const double *u, const double 
                   "mom6h_continuity_ppm: incomplete BT_cont");
     const Dims &d = cs->d;
     hipStream_t s = (hipStream_t)stream;
+    if (d.nk <= 80 && !cs->upwind_1st && !cs->simple_2nd && (long long)sizeof(ContShared<8, 10>) <= cs->max_lds) {
+        ContArgs a{d, u, hin, h, uh, dt, uhbt, cs->IareaT, cs->IdxT, cs->dy_Cu, cs->dxT, cs->dxCu,
+                   cs->mask2dCu, cs->mask2dT, cs->tol_eta, cs->tol_vel, cs->CFL_limit_adjust, 2.0 * 1.0e-10,
+                   u_cor, du_cor,
+                   bt ? bt->FA_u_EE : nullptr, bt ? bt->FA_u_E0 : nullptr, bt ? bt->FA_u_W0 : nullptr,
+                   bt ? bt->FA_u_WW : nullptr, bt ? bt->uBT_WW : nullptr, bt ? bt->uBT_EE : nullptr,
+                   bt ? bt->h_u : nullptr};
+        const dim3 g((d.iec - d.isc + 1 + CF - 2) / (CF - 1), d.jec - d.jsc + 1);
+        const int mono = cs->monotonic;
+        if (d.nk <= 8) k_continuity_blk<8, 1><<<g, dim3(64 * 8), sizeof(ContShared<8, 1>), s>>>(a, mono);
+        else if (d.nk <= 16) k_continuity_blk<8, 2><<<g, dim3(64 * 8), sizeof(ContShared<8, 2>), s>>>(a, mono);
+        else if (d.nk <= 32) k_continuity_blk<8, 4><<<g, dim3(64 * 8), sizeof(ContShared<8, 4>), s>>>(a, mono);
+        else k_continuity_blk<8, 10><<<g, dim3(64 * 8), sizeof(ContShared<8, 10>), s>>>(a, mono);
+        MOM6H_LAUNCH_CHECK();
+        if (hin_halo_to_h) {
+            const long long cells = (long long)((d.jsc - d.jsd) + (d.jed - d.jec)) * d.n1 +
+                                    (long long)((d.isc - d.isd) + (d.ied - d.iec)) * (d.jec - d.jsc + 1);
+            const dim3 bh(256), gh((unsigned)((cells + 255) / 256), d.nk);
+            k_halo_copy<<<gh, bh, 0, s>>>(d, hin, h);
+            MOM6H_LAUNCH_CHECK();
+        }
+        return MOM6H_OK;
+    }
     {
         const int ni = (d.iec + 1) - (d.isc - 1) + 1;
         dim3 b(64), g((ni + 63) / 64, d.jec - d.jsc + 1, d.nk);

@@ -84,7 +84,8 @@ struct mom6h_continuity_cs {
     int monotonic, upwind_1st, simple_2nd;
     double tol_eta, tol_vel, CFL_limit_adjust;
     double *IareaT, *IdxT, *dy_Cu, *mask2dT, *dxT, *dxCu, *mask2dCu;
-    double *hW, *hE;         // PPM edge values (3D scratch)
+    double *hW, *hE;         // PPM edge values (3D scratch of the module kernels)
+    long long max_lds;       // device limit on LDS per block
 };
 
 struct mom6h_coriolis_cs {
@@ -102,18 +103,20 @@ struct mom6h_barotropic_cs {
     double dtbt, bebt, dgeo_de;
     double *Datu, *Datv, *gtot_E, *gtot_W, *gtot_N, *gtot_S, *f_4_u, *f_4_v;
     double *bt_rem_u, *bt_rem_v, *IareaT, *IdxCu, *IdyCv;
-    // persistent state/work arrays (zero at init, as the reference's never-written cells)
-    double *eta, *eta_pred, *ubt, *vbt, *ubt_prev, *vbt_prev, *uhbt, *vhbt;
-    double *PFu, *PFv, *Cor_u, *Cor_v, *ubt_av, *vbt_av;
-    unsigned int *barrier;   // grid-barrier counter of the persistent kernel
-    int grid_blocks;
+    // uniform grids: the coefficients as values, f4 = f_4 per point (2D)
+    int uniform;
+    double u_Datu, u_gtot, u_rem_u, u_rem_v, u_IareaT, u_IdxCu, u_IdyCv;
+    double *f4;
+    // state (two sets, ping-pong across substeps) and time averages; zero at
+    // init, as the reference's never-written cells
+    double *eta, *ubt, *vbt, *eta_b, *ubt_b, *vbt_b, *ubt_av, *vbt_av;
 };
 
 struct mom6h_vert_visc_cs {
     mom6h::Dims d;
     double Kv, Kv_ml, Kv_extra_bbl, Hmix, Hbbl;
     int keep_coefs;          // store a_u/a_v/h_u/h_v (the reference's CS arrays)
-    long long max_lds;       // device limit on shared memory per block
+    long long max_lds;       // device limit on LDS per block
     double *mask2dCu, *mask2dCv;
     double *a_u, *a_v;       // coupling coefficients, nk+1 levels
     double *h_u, *h_v;       // effective thicknesses at velocity points

@@ -165,147 +165,263 @@ __global__ void k_vert_visc(Dims d, double *__restrict__ u, double *__restrict__
 }
 
 
-// LDS variant: the per-column scratch lives in shared memory, one slot per
-// (level, lane) so a wavefront's accesses are conflict-free:
-//   SA[k]  z_i(k) during sweep A, then c1(k) during sweep B
-//   SH[k]  hvel(k) during sweep A, then the forward-eliminated vel(k)
-// HBM traffic per column is h(i), h(i+1), vel in and vel out; the coupling
-// coefficients and effective thicknesses are written only when requested.
-constexpr int VV_BLOCK = 64;
+// Chunked variant.  One thread per column; the only per-column scratch is one
+// LDS slot per level, holding z_i(k) after sweep A and c1(k) after sweep B
+// (slot nk+1 holds z_i(nk+1) = 0).  Sweep B re-derives hvel from h and the input
+// velocity (reloaded) and leaves the forward-eliminated velocity in the
+// velocity array, which sweep C reads back.
+//
+// Levels go in chunks (VV_A levels in sweeps A and C, VV_C in sweep B) with
+// straight-line (branch-free) bodies: the
+// loads of the next chunk are issued before the current one is computed, and
+// inside a chunk every division that is off the tridiagonal recurrence (h_harm,
+// botfn, the mixed-layer taper, the coupling coefficient) is independent across
+// levels, so the compiler interleaves them.  Only the Thomas recurrence itself
+// (one division per level) is serial.  Levels that do not fill a chunk take a
+// one-level loop with the same arithmetic.
+struct VVArgs {
+    Dims d;
+    double *u, *v;
+    const double *h;
+    double dt;
+    const double *taux, *tauy, *ray_u, *ray_v, *mask2dCu, *mask2dCv;
+    double *a_u, *a_v, *h_u, *h_v, *taux_bot, *tauy_bot;
+    double Kv, Kv_ml, Kv_extra_bbl, Hmix, Hbbl;
+};
 
-__device__ void vv_column_lds(double *SA, double *SH, int lane, const double *hA, const double *hB,
-                              double *vel, double *a_out, double *hu_out, long long s, int nk,
-                              double mask, double dt, double sfc_stress, double Ray_bot, double Kv,
-                              double Kv_ml, double Kv_extra_bbl, double Hmix, double Hbbl, double *tbot)
-{
-    const double h_neglect = 1.0e-10;
-    auto A = [&](int k) -> double & { return SA[(k - 1) * VV_BLOCK + lane]; };   // k = 1..nk+1
-    auto H = [&](int k) -> double & { return SH[(k - 1) * VV_BLOCK + lane]; };
-    if (!(mask > 0.0)) {  // masked column (:213-221)
-        if (hu_out)
-            for (int k = 1; k <= nk; ++k) hu_out[(long long)(k - 1) * s] = h_neglect;
-        if (a_out)
-            for (int K = 1; K <= nk + 1; ++K) a_out[(long long)(K - 1) * s] = 0.0;
-        return;
-    }
-    const double I_Hbbl = 1.0 / (Hbbl + h_neglect);
-    double zi = 0.0;
-    for (int k = nk; k >= 1; --k) {   // sweep A (:160-179)
-        const long long o = (long long)(k - 1) * s;
-        const double ha = hA[o], hb = hB[o];
-        const double h_harm = 2.0 * ha * hb / (ha + hb + h_neglect);
-        const double h_arith = 0.5 * (hb + ha);
-        const double h_delta = hb - ha;
-        double hv = h_harm;
-        if (vel[o] * h_delta < 0.0) {
-            const double botfn = botfn_of(zi);
-            hv = (1.0 - botfn) * h_harm + botfn * h_arith;
-        }
-        H(k) = hv;
-        zi = zi + h_harm * I_Hbbl;
-        A(k) = zi;
-    }
-    auto coefA = [&](int K2, double hv_m1, double hv_0, double &z_top) -> double {   // (:184-202)
-        z_top = z_top + hv_m1;
-        double Kv_tot = Kv;
-        if (z_top < Hmix) {
-            const double topfn = 1.0 - z_top / Hmix;
-            Kv_tot = Kv_tot + (Kv_ml - Kv) * topfn;
-        }
-        const double botfn = botfn_of(A(K2));  // z_i(K2)
-        Kv_tot = Kv_tot + Kv_extra_bbl * botfn;
-        const double h_shear = 0.5 * (hv_0 + hv_m1 + h_neglect);
-        return Kv_tot / h_shear;
-    };
-    // sweep B (:430-444) with the coefficients formed on the fly
-    double z_top = 0.0;
-    double hv_k = H(1);
-    double a_k = 0.0;
-    double hv_n = (nk >= 2) ? H(2) : 0.0;
-    double a_n = (nk >= 2) ? coefA(2, hv_k, hv_n, z_top) : (Kv + Kv_extra_bbl) / (0.5 * hv_k + h_neglect);
-    if (a_out) a_out[0] = 0.0;
-    double hu_k = hv_k + h_neglect;
-    if (hu_out) hu_out[0] = hu_k;
-    double Ray = (nk == 1) ? Ray_bot : 0.0;
-    double b_denom_1 = hu_k + dt * (Ray + a_k);
-    double b1 = 1.0 / (b_denom_1 + dt * a_n);
-    double d1 = b_denom_1 * b1;
-    double vprev = b1 * (hu_k * vel[0] + sfc_stress);
-    H(1) = vprev;
-    for (int k = 2; k <= nk; ++k) {
-        const long long o = (long long)(k - 1) * s;
-        a_k = a_n;
-        if (a_out) a_out[o] = a_k;
-        hv_k = hv_n;
-        if (k < nk) {
-            hv_n = H(k + 1);
-            a_n = coefA(k + 1, hv_k, hv_n, z_top);
-        } else {
-            a_n = (Kv + Kv_extra_bbl) / (0.5 * hv_k + h_neglect);
-        }
-        hu_k = hv_k + h_neglect;
-        if (hu_out) hu_out[o] = hu_k;
-        Ray = (k == nk) ? Ray_bot : 0.0;
-        A(k) = dt * a_k * b1;            // c1(k); z_i(k) was consumed one level up
-        b_denom_1 = hu_k + dt * (Ray + a_k * d1);
-        b1 = 1.0 / (b_denom_1 + dt * a_n);
-        d1 = b_denom_1 * b1;
-        vprev = (hu_k * vel[o] + dt * a_k * vprev) * b1;
-        H(k) = vprev;                    // hvel(k) was consumed one level up
-    }
-    if (a_out) a_out[(long long)nk * s] = a_n;
-    // sweep C (:447-458)
-    double vnext = vprev;
-    vel[(long long)(nk - 1) * s] = vnext;
-    for (int k = nk - 1; k >= 1; --k) {
-        vnext = H(k) + A(k + 1) * vnext;
-        vel[(long long)(k - 1) * s] = vnext;
-    }
-    const double RHO_0 = 1035.0;
-    double t = RHO_0 * vprev * a_n;
-    t = t + RHO_0 * Ray_bot * vprev;
-    *tbot = t;
-}
+constexpr int VV_C = 8;    // levels per chunk of sweep B (heavy bodies)
+constexpr int VV_A = 16;   // levels per chunk of sweeps A and C (light bodies, latency-bound loads)
 
-__global__ void __launch_bounds__(VV_BLOCK) k_vert_visc_lds(
-    Dims d, double *__restrict__ u, double *__restrict__ v, const double *__restrict__ h, double dt,
-    const double *__restrict__ taux, const double *__restrict__ tauy, const double *__restrict__ ray_u,
-    const double *__restrict__ ray_v, const double *__restrict__ mask2dCu, const double *__restrict__ mask2dCv,
-    double *__restrict__ a_u, double *__restrict__ a_v, double *__restrict__ h_u, double *__restrict__ h_v,
-    double *__restrict__ taux_bot, double *__restrict__ tauy_bot, double Kv, double Kv_ml,
-    double Kv_extra_bbl, double Hmix, double Hbbl)
+template <bool KEEP>
+__global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
 {
-    extern __shared__ double smem[];
-    double *SA = smem, *SH = smem + (size_t)(d.nk + 1) * VV_BLOCK;
+    extern __shared__ double Zs[];
+    const Dims &d = p.d;
     const int lane = threadIdx.x;
     const int rows_u = d.jec - d.jsc + 1;
     const int row = blockIdx.y;
-    const double dt_Rho0 = dt / 1035.0;
-    if (row < rows_u) {
-        const int I = d.isc - 1 + blockIdx.x * VV_BLOCK + lane;
-        const int j = d.jsc + row;
-        if (I > d.iec) return;
-        const long long c2 = IX2(d, I, j), o = IX3(d, I, j, 1);
-        const double m = mask2dCu[c2];
-        const double sfc = taux ? dt_Rho0 * taux[c2] * m : 0.0;
-        double tb = 0.0;
-        vv_column_lds(SA, SH, lane, h + o, h + IX3(d, I + 1, j, 1), u + o, a_u ? a_u + o : nullptr,
-                      h_u ? h_u + o : nullptr, d.s2, d.nk, m, dt, sfc, ray_u ? ray_u[c2] : 0.0, Kv, Kv_ml,
-                      Kv_extra_bbl, Hmix, Hbbl, &tb);
-        if (m > 0.0) taux_bot[c2] = tb;
-    } else {
-        const int i = d.isc + blockIdx.x * VV_BLOCK + lane;
-        const int J = d.jsc - 1 + (row - rows_u);
-        if (i > d.iec) return;
-        const long long c2 = IX2(d, i, J), o = IX3(d, i, J, 1);
-        const double m = mask2dCv[c2];
-        const double sfc = tauy ? dt_Rho0 * tauy[c2] * m : 0.0;
-        double tb = 0.0;
-        vv_column_lds(SA, SH, lane, h + o, h + IX3(d, i, J + 1, 1), v + o, a_v ? a_v + o : nullptr,
-                      h_v ? h_v + o : nullptr, d.s2, d.nk, m, dt, sfc, ray_v ? ray_v[c2] : 0.0, Kv, Kv_ml,
-                      Kv_extra_bbl, Hmix, Hbbl, &tb);
-        if (m > 0.0) tauy_bot[c2] = tb;
+    const long long s = d.s2;
+    const int nk = d.nk;
+    const double *hA, *hB;
+    double *vel, *a_out, *hu_out, *tb_out;
+    double m, sfc, Ray_bot;
+    {
+        const double dt_Rho0 = p.dt / 1035.0;
+        if (row < rows_u) {   // u column (I, j), I = is-1..ie
+            const int I = d.isc - 1 + blockIdx.x * 64 + lane, j = d.jsc + row;
+            if (I > d.iec) return;
+            const long long c2 = IX2(d, I, j), o = IX3(d, I, j, 1);
+            hA = p.h + o; hB = p.h + IX3(d, I + 1, j, 1); vel = p.u + o;
+            a_out = p.a_u + o; hu_out = p.h_u + o;
+            tb_out = p.taux_bot + c2;
+            m = p.mask2dCu[c2];
+            sfc = p.taux ? dt_Rho0 * p.taux[c2] * m : 0.0;
+            Ray_bot = p.ray_u ? p.ray_u[c2] : 0.0;
+        } else {              // v column (i, J), J = js-1..je
+            const int i = d.isc + blockIdx.x * 64 + lane, J = d.jsc - 1 + (row - rows_u);
+            if (i > d.iec) return;
+            const long long c2 = IX2(d, i, J), o = IX3(d, i, J, 1);
+            hA = p.h + o; hB = p.h + IX3(d, i, J + 1, 1); vel = p.v + o;
+            a_out = p.a_v + o; hu_out = p.h_v + o;
+            tb_out = p.tauy_bot + c2;
+            m = p.mask2dCv[c2];
+            sfc = p.tauy ? dt_Rho0 * p.tauy[c2] * m : 0.0;
+            Ray_bot = p.ray_v ? p.ray_v[c2] : 0.0;
+        }
     }
+    const double h_neglect = 1.0e-10;
+    if (!(m > 0.0)) {  // masked column: coefficients only (:213-221)
+        if (KEEP) {
+            for (int k = 0; k < nk; ++k) hu_out[k * s] = h_neglect;
+            for (int k = 0; k <= nk; ++k) a_out[k * s] = 0.0;
+        }
+        return;
+    }
+    auto Z = [&](int k) -> double & { return Zs[(k - 1) * 64 + lane]; };   // level k = 1..nk+1
+    const double Kv = p.Kv, Kv_ml = p.Kv_ml, Kv_extra_bbl = p.Kv_extra_bbl, Hmix = p.Hmix, dt = p.dt;
+    const double I_Hbbl = 1.0 / (p.Hbbl + h_neglect);
+    Z(nk + 1) = 0.0;
+
+    // ---- sweep A, bottom-up (:160-179): z_i(k) ----
+    double zi = 0.0;
+    auto A_level = [&](int k, double ha, double hb) {
+        const double h_harm = 2.0 * ha * hb / (ha + hb + h_neglect);
+        zi = zi + h_harm * I_Hbbl;
+        Z(k) = zi;
+    };
+    int k0 = nk;
+    if (k0 >= VV_A) {
+        double na[VV_A], nb[VV_A];
+#pragma unroll
+        for (int c = 0; c < VV_A; ++c) { na[c] = hA[(k0 - 1 - c) * s]; nb[c] = hB[(k0 - 1 - c) * s]; }
+        for (; k0 >= VV_A; k0 -= VV_A) {   // levels k0 .. k0-VV_A+1, all >= 1
+            double ha[VV_A], hb[VV_A];
+#pragma unroll
+            for (int c = 0; c < VV_A; ++c) { ha[c] = na[c]; hb[c] = nb[c]; }
+            if (k0 - VV_A >= VV_A) {
+#pragma unroll
+                for (int c = 0; c < VV_A; ++c) {
+                    na[c] = hA[(k0 - VV_A - 1 - c) * s];
+                    nb[c] = hB[(k0 - VV_A - 1 - c) * s];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < VV_A; ++c) A_level(k0 - c, ha[c], hb[c]);
+        }
+    }
+    for (; k0 >= 1; --k0) A_level(k0, hA[(k0 - 1) * s], hB[(k0 - 1) * s]);
+
+    // hvel (:165-176) given botfn(z_i(k+1)); the upwind blend is selected, not branched
+    auto hvel = [&](double ha, double hb, double vk, double bf) -> double {
+        const double h_harm = 2.0 * ha * hb / (ha + hb + h_neglect);
+        const double h_arith = 0.5 * (hb + ha);
+        const double h_delta = hb - ha;
+        const double blend = (1.0 - bf) * h_harm + bf * h_arith;
+        return (vk * h_delta < 0.0) ? blend : h_harm;
+    };
+    // coupling coefficient a(K2) (:184-202) from z_top (already advanced),
+    // botfn(z_i(K2)) and hvel at K2-1 and K2
+    auto coefA = [&](double z_top, double bfK, double hv_m1, double hv_0) -> double {
+        const double topfn = 1.0 - z_top / Hmix;
+        double Kv_tot = (z_top < Hmix) ? Kv + (Kv_ml - Kv) * topfn : Kv;
+        Kv_tot = Kv_tot + Kv_extra_bbl * bfK;
+        return Kv_tot / (0.5 * (hv_0 + hv_m1 + h_neglect));
+    };
+
+    // ---- sweep B, top-down (:430-444) ----
+    double z_top, hv_n, a_n, vk_cur, b1, d1, vprev;
+    {   // level 1
+        const double v1 = vel[0];
+        const double hv1 = hvel(hA[0], hB[0], v1, botfn_of(Z(2)));
+        hv_n = 0.0;
+        z_top = 0.0;
+        if (nk >= 2) {
+            const double v2 = vel[s];
+            hv_n = hvel(hA[s], hB[s], v2, botfn_of(Z(3)));
+            vk_cur = v2;
+            z_top = z_top + hv1;
+            a_n = coefA(z_top, botfn_of(Z(2)), hv1, hv_n);
+        } else {
+            vk_cur = 0.0;
+            a_n = (Kv + Kv_extra_bbl) / (0.5 * hv1 + h_neglect);
+        }
+        if (KEEP) a_out[0] = 0.0;
+        const double hu_k = hv1 + h_neglect;
+        if (KEEP) hu_out[0] = hu_k;
+        const double Ray = (nk == 1) ? Ray_bot : 0.0;
+        const double b_denom_1 = hu_k + dt * (Ray + 0.0);
+        b1 = 1.0 / (b_denom_1 + dt * a_n);
+        d1 = b_denom_1 * b1;
+        vprev = b1 * (hu_k * v1 + sfc);
+        vel[0] = vprev;
+    }
+    // one interior level k (2..nk-1): the recurrence, given a(k+1) and hvel(k+1)
+    auto B_serial = [&](int k, double a_next, double hv_next, double vk_next) {
+        const double a_k = a_n;
+        if (KEEP) a_out[(k - 1) * s] = a_k;
+        const double hv_k = hv_n;
+        const double hu_k = hv_k + h_neglect;
+        if (KEEP) hu_out[(k - 1) * s] = hu_k;
+        Z(k) = dt * a_k * b1;                              // c1(k); z_i(k) is no longer read
+        const double b_denom_1 = hu_k + dt * (0.0 + a_k * d1);
+        b1 = 1.0 / (b_denom_1 + dt * a_next);
+        d1 = b_denom_1 * b1;
+        vprev = (hu_k * vk_cur + dt * a_k * vprev) * b1;
+        vel[(k - 1) * s] = vprev;
+        a_n = a_next;
+        hv_n = hv_next;
+        vk_cur = vk_next;
+    };
+    k0 = 2;
+    if (k0 + VV_C - 1 <= nk - 1) {
+        double na[VV_C], nb[VV_C], nv[VV_C];
+#pragma unroll
+        for (int c = 0; c < VV_C; ++c) {   // h and u at k+1 for the chunk's levels
+            na[c] = hA[(k0 + c) * s]; nb[c] = hB[(k0 + c) * s]; nv[c] = vel[(k0 + c) * s];
+        }
+        for (; k0 + VV_C - 1 <= nk - 1; k0 += VV_C) {   // levels k0 .. k0+VV_C-1
+            double ha[VV_C], hb[VV_C], vv[VV_C];
+#pragma unroll
+            for (int c = 0; c < VV_C; ++c) { ha[c] = na[c]; hb[c] = nb[c]; vv[c] = nv[c]; }
+            if (k0 + 2 * VV_C - 1 <= nk - 1) {
+#pragma unroll
+                for (int c = 0; c < VV_C; ++c) {
+                    na[c] = hA[(k0 + VV_C + c) * s];
+                    nb[c] = hB[(k0 + VV_C + c) * s];
+                    nv[c] = vel[(k0 + VV_C + c) * s];
+                }
+            }
+            // off-recurrence terms: bf[c] = botfn(z_i(k0+1+c)), hvel(k+1), z_top, a(k+1)
+            double bf[VV_C + 1], hvx[VV_C], ax[VV_C];
+#pragma unroll
+            for (int c = 0; c <= VV_C; ++c) bf[c] = botfn_of(Z(k0 + 1 + c));
+#pragma unroll
+            for (int c = 0; c < VV_C; ++c) hvx[c] = hvel(ha[c], hb[c], vv[c], bf[c + 1]);
+#pragma unroll
+            for (int c = 0; c < VV_C; ++c) {
+                const double hv_k = (c == 0) ? hv_n : hvx[c - 1];
+                z_top = z_top + hv_k;
+                ax[c] = coefA(z_top, bf[c], hv_k, hvx[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < VV_C; ++c) B_serial(k0 + c, ax[c], hvx[c], vv[c]);
+        }
+    }
+    for (; k0 <= nk - 1; ++k0) {
+        const double vnx = vel[k0 * s];
+        const double hvx = hvel(hA[k0 * s], hB[k0 * s], vnx, botfn_of(Z(k0 + 2)));
+        z_top = z_top + hv_n;
+        const double ax = coefA(z_top, botfn_of(Z(k0 + 1)), hv_n, hvx);
+        B_serial(k0, ax, hvx, vnx);
+    }
+    if (nk >= 2) {   // bottom level: a(nk+1) from the bottom formula (:205-206), Rayleigh drag
+        const int k = nk;
+        const double a_k = a_n;
+        if (KEEP) a_out[(k - 1) * s] = a_k;
+        const double hv_k = hv_n;
+        a_n = (Kv + Kv_extra_bbl) / (0.5 * hv_k + h_neglect);
+        const double hu_k = hv_k + h_neglect;
+        if (KEEP) hu_out[(k - 1) * s] = hu_k;
+        Z(k) = dt * a_k * b1;
+        const double b_denom_1 = hu_k + dt * (Ray_bot + a_k * d1);
+        b1 = 1.0 / (b_denom_1 + dt * a_n);
+        d1 = b_denom_1 * b1;
+        vprev = (hu_k * vk_cur + dt * a_k * vprev) * b1;
+        vel[(k - 1) * s] = vprev;
+    }
+    if (KEEP) a_out[nk * s] = a_n;
+
+    // ---- sweep C, bottom-up back substitution (:447-449) ----
+    double vnext = vprev;
+    k0 = nk - 1;
+    if (k0 >= VV_A) {
+        double nf[VV_A];
+#pragma unroll
+        for (int c = 0; c < VV_A; ++c) nf[c] = vel[(k0 - 1 - c) * s];
+        for (; k0 >= VV_A; k0 -= VV_A) {   // levels k0 .. k0-VV_A+1
+            double fv[VV_A];
+#pragma unroll
+            for (int c = 0; c < VV_A; ++c) fv[c] = nf[c];
+            if (k0 - VV_A >= VV_A) {
+#pragma unroll
+                for (int c = 0; c < VV_A; ++c) nf[c] = vel[(k0 - VV_A - 1 - c) * s];
+            }
+#pragma unroll
+            for (int c = 0; c < VV_A; ++c) {
+                vnext = fv[c] + Z(k0 - c + 1) * vnext;
+                vel[(k0 - c - 1) * s] = vnext;
+            }
+        }
+    }
+    for (; k0 >= 1; --k0) {
+        vnext = vel[(k0 - 1) * s] + Z(k0 + 1) * vnext;
+        vel[(k0 - 1) * s] = vnext;
+    }
+    // bottom stress (:452-458)
+    const double RHO_0 = 1035.0;
+    double t = RHO_0 * vprev * a_n;
+    t = t + RHO_0 * Ray_bot * vprev;
+    *tb_out = t;
 }
 
 }  // namespace mom6h
@@ -361,14 +477,16 @@ int mom6h_vert_visc(mom6h_vert_visc_cs *cs, double *u, double *v, const double *
     const Dims &d = cs->d;
     const int ni = d.iec - d.isc + 2;
     const int rows = (d.jec - d.jsc + 1) + (d.jec - d.jsc + 2);
-    const size_t lds = 2 * (size_t)(d.nk + 1) * VV_BLOCK * sizeof(double);
+    const size_t lds = (size_t)(d.nk + 1) * 64 * sizeof(double);
     if ((long long)lds <= cs->max_lds) {
-        dim3 b(VV_BLOCK), g((ni + VV_BLOCK - 1) / VV_BLOCK, rows);
-        const bool keep = cs->keep_coefs != 0;
-        k_vert_visc_lds<<<g, b, lds, (hipStream_t)stream>>>(
-            d, u, v, h, dt, taux, tauy, ray_u_bot, ray_v_bot, cs->mask2dCu, cs->mask2dCv, keep ? cs->a_u : nullptr,
-            keep ? cs->a_v : nullptr, keep ? cs->h_u : nullptr, keep ? cs->h_v : nullptr, cs->taux_bot,
-            cs->tauy_bot, cs->Kv, cs->Kv_ml, cs->Kv_extra_bbl, cs->Hmix, cs->Hbbl);
+        VVArgs p{d, u, v, h, dt, taux, tauy, ray_u_bot, ray_v_bot, cs->mask2dCu, cs->mask2dCv,
+                 cs->a_u, cs->a_v, cs->h_u, cs->h_v, cs->taux_bot, cs->tauy_bot, cs->Kv, cs->Kv_ml,
+                 cs->Kv_extra_bbl, cs->Hmix, cs->Hbbl};
+        const dim3 g((ni + 63) / 64, rows);
+        if (cs->keep_coefs)
+            k_vert_visc_chunked<true><<<g, dim3(64), lds, (hipStream_t)stream>>>(p);
+        else
+            k_vert_visc_chunked<false><<<g, dim3(64), lds, (hipStream_t)stream>>>(p);
         MOM6H_LAUNCH_CHECK();
         return MOM6H_OK;
     }

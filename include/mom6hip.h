@@ -32,7 +32,6 @@ extern "C" {
 #define MOM6H_EINVAL 1001      /* bad argument (bounds, NULL, unknown name) */
 #define MOM6H_ENOMEM 1002      /* device allocation failed */
 #define MOM6H_ESTATE 1003      /* call out of order (e.g. step before prepare) */
-#define MOM6H_EBARRIER 1004    /* persistent kernel grid barrier timed out */
 
 const char *mom6h_last_error(void);
 const char *mom6h_version(void);
@@ -100,9 +99,11 @@ int mom6h_coriolis_end(mom6h_coriolis_cs *cs);
 /* ------------------------------------------------------------ barotropic --
  * barotropic_init_cuda_c (src/cuda_c/mom6_barotropic_cuda_c.F90:226-300),
  * btstep_cuda_c (:305-433), barotropic_end_cuda_c (:629-669); arithmetic of
- * src/serial/mom6_barotropic_serial.F90:178-356.  The whole substep loop runs
- * in ONE persistent kernel (grid-wide barriers between dependent phases)
- * instead of 9 launches per substep.  periodic_x/periodic_y select the
+ * src/serial/mom6_barotropic_serial.F90:178-356.  Each substep is ONE launch
+ * of an LDS-tiled kernel (eta predictor, pressure forces, both velocity
+ * updates, transports, free surface, time averages on a tile plus its halo)
+ * instead of ~9 dependent launches; grids whose coefficients are uniform take
+ * a variant that holds them as values.  periodic_x/periodic_y select the
  * split-API ranges (btstep_do_step, :379-474) with a halo refill of
  * ubt, vbt, eta after every substep but the last. */
 typedef struct mom6h_barotropic_cs mom6h_barotropic_cs;
