@@ -45,7 +45,9 @@ def step_bytes(cfg) -> float:
 
 
 # Algorithmic bytes of ONE call of each stage (3D arrays x 8 B x n1*n2*nk +
-# 2D arrays x 8 B x n1*n2), following the per-stage counts of the step total.
+# 2D arrays x 8 B x n1*n2), following the per-stage counts of the step total
+# (each input read once, each output written once; metrics, u_cor and du_cor
+# excluded).
 STAGE_ARRAYS = {            # (3D, 2D) per call
     "transports": (5, 0),    # u, v, h -> uh, vh
     "hor_visc": (5, 0),      # u, v, h -> diffu, diffv
@@ -61,6 +63,27 @@ STAGE_ARRAYS = {            # (3D, 2D) per call
 }
 STAGE_CALLS = {"transports": 2, "hor_visc": 2, "coriolis": 2, "vel_update": 2, "vert_visc": 2,
                "continuity": 2, "btstep": 2, "halo": 1, "momentum": 2}
+# Stages that are ONE kernel launch per call on the 3D stream (the fused path),
+# and that kernel.  The barotropic stage brackets bt_nsteps+1 launches on the
+# second stream, so its events do not time a single kernel; it is reported in
+# "stages" only.
+ROOFLINE_KERNELS = {"momentum": "k_momentum", "vert_visc": "k_vert_visc_chunked",
+                    "continuity": "k_continuity_blk"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_720x720x75.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a kernel at 720x720x75 from the committed PMC
+    summary (lines "traffic <kernel> <bytes>"), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(cfg, steps_cap=1):
@@ -143,16 +166,19 @@ def main():
         calls = STAGE_CALLS.get(name, 1) * args.steps
         per_stage[name] = {"ms_per_step": ms / args.steps, "ms_per_call": ms / calls}
     roofline = None
-    dominant = None
     if per_stage:
-        cand = {k: v for k, v in per_stage.items() if k in STAGE_ARRAYS and v["ms_per_step"] > 0}
-        dominant = max(cand, key=lambda k: cand[k]["ms_per_step"])
-        a3, a2 = STAGE_ARRAYS[dominant]
-        traffic_alg = 8.0 * (a3 * n3 + a2 * n2)
-        ach = traffic_alg / (per_stage[dominant]["ms_per_call"] * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
-                    "algorithmic_bytes_per_launch": traffic_alg}
+        cand = {k: v for k, v in per_stage.items() if k in ROOFLINE_KERNELS and v["ms_per_step"] > 0}
+        if cand:
+            dominant = max(cand, key=lambda k: cand[k]["ms_per_step"])
+            a3, a2 = STAGE_ARRAYS[dominant]
+            alg = 8.0 * (a3 * n3 + a2 * n2)
+            ach = alg / (per_stage[dominant]["ms_per_call"] * 1e-3) / 1e9
+            kern = ROOFLINE_KERNELS[dominant]
+            traffic = measured_traffic(kern) if (cfg.ni, cfg.nj, cfg.nk) == (720, 720, 75) else None
+            roofline = {"bound": "hbm", "kernel": kern, "stage": dominant, "achieved": round(ach, 1),
+                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                        "traffic": traffic, "algorithmic_bytes_per_launch": alg,
+                        "mean_launch_ms": per_stage[dominant]["ms_per_call"]}
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,

@@ -103,7 +103,8 @@ __global__ void __launch_bounds__(256) k_bt_substep(BT p, M m, BTStep st)
     __shared__ double s_PFu[BT_H][BT_W], s_PFv[BT_H][BT_W];
     const Dims &d = p.d;
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
-    const int i0 = d.isd + blockIdx.x * BT_TX, j0 = d.jsd + blockIdx.y * BT_TY;
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int i0 = d.isd + (tile % gridDim.x) * BT_TX, j0 = d.jsd + (tile / gridDim.x) * BT_TY;
     const int i1 = i0 + BT_TX - 1, j1 = j0 + BT_TY - 1;
     const int bi = i0 - 2, bj = j0 - 2;  // LDS origin
     const int tid = threadIdx.x;

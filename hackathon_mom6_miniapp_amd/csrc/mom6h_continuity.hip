@@ -469,6 +469,25 @@ struct ContShared {
     double U[L][CF], S0[L][CF], S1[L][CF], R[4][CF];
 };
 
+// Sequential fold of rows 0..nk-1 of one LDS column (k order), the reads
+// issued a batch ahead of the dependent chain.
+template <int L, typename Op>
+__device__ __forceinline__ double lds_fold(const double (*col)[CF], int nk, int lane, double acc, Op op)
+{
+    constexpr int B = 16;
+    for (int k0 = 0; k0 < nk; k0 += B) {
+        double x[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int k = k0 + b;
+            x[b] = col[k < L ? k : L - 1][lane];
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) acc = (k0 + b < nk) ? op(acc, x[b]) : acc;
+    }
+    return acc;
+}
+
 template <int CW, int K>
 __global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(1, (CW >= 16) ? 4 : 2)))
 k_continuity_blk(ContArgs a, int monotonic)
@@ -545,12 +564,9 @@ k_continuity_blk(ContArgs a, int monotonic)
     auto sum_pass = [&](double x0, double x1, bool on) __attribute__((always_inline)) {
         __syncthreads();
         if (w < 2 && on) {
-            double acc = (w == 0) ? x0 : x1;
-            if (w == 0)
-                for (int k = 0; k < nk; ++k) acc = acc + S.S0[k][lane];
-            else
-                for (int k = 0; k < nk; ++k) acc = acc + S.S1[k][lane];
-            S.R[w][lane] = acc;
+            const auto plus = [](double a_, double x_) { return a_ + x_; };
+            S.R[w][lane] = (w == 0) ? lds_fold<ContShared<CW, K>::L>(S.S0, nk, lane, x0, plus)
+                                    : lds_fold<ContShared<CW, K>::L>(S.S1, nk, lane, x1, plus);
         }
         __syncthreads();
     };
@@ -561,26 +577,16 @@ k_continuity_blk(ContArgs a, int monotonic)
     eval_pass(0.0, true);
     __syncthreads();
     if (w < 2) {
-        double acc = 0.0;
-        if (w == 0)
-            for (int k = 0; k < nk; ++k) acc = acc + S.S0[k][lane];
-        else
-            for (int k = 0; k < nk; ++k) acc = acc + S.S1[k][lane];
-        S.R[w][lane] = acc;
+        const auto plus = [](double a_, double x_) { return a_ + x_; };
+        S.R[w][lane] = (w == 0) ? lds_fold<ContShared<CW, K>::L>(S.S0, nk, lane, 0.0, plus) : lds_fold<ContShared<CW, K>::L>(S.S1, nk, lane, 0.0, plus);
     } else if (w == 2) {
-        double x = 2.0 * (CFL_dt * dx_W) * 1.0;
-        for (int k = 0; k < nk; ++k) {
-            const double uu = S.U[k][lane];
-            if (x > dx_W * CFL_dt - uu * mCu) x = (dx_W * CFL_dt - uu);
-        }
-        S.R[2][lane] = x;
+        S.R[2][lane] = lds_fold<ContShared<CW, K>::L>(S.U, nk, lane, 2.0 * (CFL_dt * dx_W) * 1.0, [&](double x, double uu) {
+            return (x > dx_W * CFL_dt - uu * mCu) ? (dx_W * CFL_dt - uu) : x;
+        });
     } else if (w == 3) {
-        double x = -(2.0 * (CFL_dt * dx_E) * 1.0);
-        for (int k = 0; k < nk; ++k) {
-            const double uu = S.U[k][lane];
-            if (x < -dx_E * CFL_dt - uu * mCu) x = -(dx_E * CFL_dt + uu);
-        }
-        S.R[3][lane] = x;
+        S.R[3][lane] = lds_fold<ContShared<CW, K>::L>(S.U, nk, lane, -(2.0 * (CFL_dt * dx_E) * 1.0), [&](double x, double uu) {
+            return (x < -dx_E * CFL_dt - uu * mCu) ? -(dx_E * CFL_dt + uu) : x;
+        });
     }
     __syncthreads();
     const double uh_tot_0 = S.R[0][lane], duhdu_tot_0 = S.R[1][lane];
@@ -717,19 +723,13 @@ k_continuity_blk(ContArgs a, int monotonic)
     const double du_CFL = (CFL_min * Idt) * a.dxCu[c2];
     __syncthreads();
     if (w == 0) {
-        double x = dmin(0.0, du0 - du_CFL);
-        for (int k = 0; k < nk; ++k) {
-            const double uu = S.U[k][lane];
-            if (uu + x > -du_CFL) x = -(uu + du_CFL);
-        }
-        S.R[2][lane] = x;
+        S.R[2][lane] = lds_fold<ContShared<CW, K>::L>(S.U, nk, lane, dmin(0.0, du0 - du_CFL), [&](double x, double uu) {
+            return (uu + x > -du_CFL) ? -(uu + du_CFL) : x;
+        });
     } else if (w == 1) {
-        double x = dmax(0.0, du0 + du_CFL);
-        for (int k = 0; k < nk; ++k) {
-            const double uu = S.U[k][lane];
-            if (uu + x < du_CFL) x = -(uu - du_CFL);
-        }
-        S.R[3][lane] = x;
+        S.R[3][lane] = lds_fold<ContShared<CW, K>::L>(S.U, nk, lane, dmax(0.0, du0 + du_CFL), [&](double x, double uu) {
+            return (uu + x < du_CFL) ? -(uu - du_CFL) : x;
+        });
     }
     __syncthreads();
     const double duR = S.R[2][lane], duL = S.R[3][lane];

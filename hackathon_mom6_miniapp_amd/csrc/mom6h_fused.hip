@@ -32,23 +32,52 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgs a)
     const Dims &d = a.d;
     const UMetrics &g = a.g;
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
-    const int i0 = is + blockIdx.x * MT_X, j0 = js + blockIdx.y * MT_Y;
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int i0 = is + (tile % gridDim.x) * MT_X, j0 = js + (tile / gridDim.x) * MT_Y;
     const int bi = i0 - 1, bj = j0 - 1;
     const int tid = threadIdx.x;
     const int tx = tid % MT_X, ty = tid / MT_X;          // 64 x 4 threads, two rows each
 
-    for (int k = 1; k <= d.nk; ++k) {
-        // stage u, v, h on [i0-1, i1+1] x [j0-1, j1+1]
-        for (int t = tid; t < MW * MH; t += blockDim.x) {
+    // u, v, h of the staged region [i0-1, i1+1] x [j0-1, j1+1] of one layer go
+    // through registers: the next layer's loads are issued before the current
+    // layer is computed (NS points per thread), and so are the base velocities
+    // of the thread's two output rows
+    constexpr int NS = (MW * MH + 255) / 256;
+    double pu[NS], pv[NS], ph[NS], pbu[2], pbv[2];
+    auto load_layer = [&](int k) {
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            const int t = tid + 256 * r;
             const int li = t % MW, lj = t / MW, i = bi + li, j = bj + lj;
-            double uu = 0.0, vv = 0.0, hh = 0.0;
-            if (i <= d.ied && j <= d.jed) {
+            pu[r] = 0.0; pv[r] = 0.0; ph[r] = 0.0;
+            if (t < MW * MH && i <= d.ied && j <= d.jed) {
                 const long long c = IX3(d, i, j, k);
-                uu = a.u[c]; vv = a.v[c]; hh = a.h[c];
+                pu[r] = a.u[c]; pv[r] = a.v[c]; ph[r] = a.h[c];
             }
-            s_u[lj][li] = uu; s_v[lj][li] = vv; s_h[lj][li] = hh;
         }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int i = i0 + tx, j = j0 + ty + 4 * r;
+            pbu[r] = 0.0; pbv[r] = 0.0;
+            if (i <= ie && j <= je) {
+                const long long c = IX3(d, i, j, k);
+                pbu[r] = a.bu[c]; pbv[r] = a.bv[c];
+            }
+        }
+    };
+    load_layer(1);
+    for (int k = 1; k <= d.nk; ++k) {
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            const int t = tid + 256 * r;
+            if (t < MW * MH) {
+                const int li = t % MW, lj = t / MW;
+                s_u[lj][li] = pu[r]; s_v[lj][li] = pv[r]; s_h[lj][li] = ph[r];
+            }
+        }
+        const double bu_r[2] = {pbu[0], pbu[1]}, bv_r[2] = {pbv[0], pbv[1]};
         __syncthreads();
+        if (k < d.nk) load_layer(k + 1);
         // q on q points [i0-1, i1] x [j0-1, j1] (mom6_coriolis_serial.F90:134-168)
         // and KE on h points [i0, i1+1] x [j0, j1+1] (:196-203)
         for (int t = tid; t < MW * MH; t += blockDim.x) {
@@ -73,6 +102,7 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgs a)
             }
         }
         __syncthreads();
+#pragma unroll 1
         for (int r = 0; r < 2; ++r) {
             const int i = i0 + tx, j = j0 + ty + 4 * r;
             if (i > ie || j > je) continue;
@@ -104,7 +134,7 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgs a)
                 const double KEx = (s_KE[lj][li + 1] - s_KE[lj][li]) * g.IdxCu;
                 const double CAu = 0.25 * ((s_q[lj][li] * (vh(li + 1, lj) + vh(li, lj))) +
                                            (s_q[lj - 1][li] * (vh(li, lj - 1) + vh(li + 1, lj - 1)))) * g.IdxCu - KEx;
-                a.ou[c] = a.bu[c] + a.fdt * (CAu + diffu);
+                a.ou[c] = bu_r[r] + a.fdt * (CAu + diffu);
             }
             if (j <= je - 1) {
                 // diffv (:1070-1078)
@@ -117,7 +147,7 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgs a)
                 const double KEy = (s_KE[lj + 1][li] - s_KE[lj][li]) * g.IdyCv;
                 const double CAv = -(0.25 * ((s_q[lj][li - 1] * (uh(li - 1, lj) + uh(li - 1, lj + 1))) +
                                              (s_q[lj][li] * (uh(li, lj) + uh(li, lj + 1)))) * g.IdyCv) - KEy;
-                a.ov[c] = a.bv[c] + a.fdt * (CAv + diffv);
+                a.ov[c] = bv_r[r] + a.fdt * (CAv + diffv);
             }
         }
         __syncthreads();

@@ -43,6 +43,18 @@ __host__ __device__ inline long long IX3(const Dims &d, int i, int j, int k)
     return ((long long)(k - 1) * d.n2 + (j - d.jsd)) * d.n1 + (i - d.isd);
 }
 
+// Block-to-tile mapping for the 8 XCDs of MI355X.  The hardware hands
+// consecutive workgroups to the XCDs in turn, each XCD with its own L2; this
+// renumbers them so that XCD x works on one contiguous range of tiles, and
+// tiles that share halos (and 2D fields read by every layer) meet in the same
+// L2.  A bijection on [0, nb) for any nb.
+__device__ inline int xcd_tile(int b, int nb)
+{
+    const int m = nb / 8, r = nb % 8;
+    const int x = b % 8, q = b / 8;
+    return x < r ? x * (m + 1) + q : r * (m + 1) + (x - r) * m + q;
+}
+
 __device__ inline double dmax(double a, double b) { return a > b ? a : b; }
 __device__ inline double dmin(double a, double b) { return a < b ? a : b; }
 

@@ -446,6 +446,9 @@ int mom6h_model_step(mom6h_model *m, int niter)
     MOM6H_REQUIRE(m != nullptr, "mom6h_model_step: NULL model");
     MOM6H_REQUIRE(m->prepared, "mom6h_model_step: call mom6h_model_prepare first");
     MOM6H_REQUIRE(niter >= 0, "mom6h_model_step: niter >= 0 required");
+    // The fused path consumes vert_visc's coupling coefficients on chip; the
+    // module path keeps them in the control structure, as the reference does.
+    RC(mom6h_vert_visc_keep_coefficients(m->vv, fused_ok(m) ? 0 : 1));
     // The barotropic chain and the 3D chain share no array (the 2D fields the
     // barotropic solver reads and writes are never touched by the 3D stages),
     // so s1 runs all its steps without waiting for s0; the streams join once.

@@ -198,8 +198,11 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
     extern __shared__ double Zs[];
     const Dims &d = p.d;
     const int lane = threadIdx.x;
-    const int rows_u = d.jec - d.jsc + 1;
-    const int row = blockIdx.y;
+    // rows interleaved v, u, v, u, ... (v row J = js-1+r/2, u row j = js+(r-1)/2)
+    // so that consecutive rows share their h rows, tiles renumbered per XCD
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int bx = tile % gridDim.x, row = tile / gridDim.x;
+    const bool is_u = (row & 1) != 0;
     const long long s = d.s2;
     const int nk = d.nk;
     const double *hA, *hB;
@@ -207,8 +210,8 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
     double m, sfc, Ray_bot;
     {
         const double dt_Rho0 = p.dt / 1035.0;
-        if (row < rows_u) {   // u column (I, j), I = is-1..ie
-            const int I = d.isc - 1 + blockIdx.x * 64 + lane, j = d.jsc + row;
+        if (is_u) {   // u column (I, j), I = is-1..ie
+            const int I = d.isc - 1 + bx * 64 + lane, j = d.jsc + (row - 1) / 2;
             if (I > d.iec) return;
             const long long c2 = IX2(d, I, j), o = IX3(d, I, j, 1);
             hA = p.h + o; hB = p.h + IX3(d, I + 1, j, 1); vel = p.u + o;
@@ -218,7 +221,7 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
             sfc = p.taux ? dt_Rho0 * p.taux[c2] * m : 0.0;
             Ray_bot = p.ray_u ? p.ray_u[c2] : 0.0;
         } else {              // v column (i, J), J = js-1..je
-            const int i = d.isc + blockIdx.x * 64 + lane, J = d.jsc - 1 + (row - rows_u);
+            const int i = d.isc + bx * 64 + lane, J = d.jsc - 1 + row / 2;
             if (i > d.iec) return;
             const long long c2 = IX2(d, i, J), o = IX3(d, i, J, 1);
             hA = p.h + o; hB = p.h + IX3(d, i, J + 1, 1); vel = p.v + o;
