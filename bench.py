@@ -9,9 +9,19 @@ state h = 4000/75 + N(0, 0.5) m, u, v ~ N(0, 0.05) m/s, eta = 0.
 
 A step is one RK2 step (app/rk2_serial_driver.F90:218-399) on device-resident
 state.  value = ni*nj*nk / t_step (cell-updates/s), t_step from the wall clock
-between device synchronisations around exactly --steps steps.
+between device synchronisations around exactly --steps steps, run as in
+production (barotropic chain overlapping the 3D chain on a second stream, no
+events).  The roofline figures come from an untimed pass of up to 3 further
+steps with HIP events around every stage and the barotropic chain serialised
+onto the 3D stream, so that each event pair times its kernel alone, as a
+kernel trace does.
 
-Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+The line also carries "mixed": the mixed-precision form (FP32 layer fields
+and stencils, FP64 vertical solve and barotropic solver) timed the same way,
+and its per-field error against the FP64 path after --mixed-steps steps from
+the same seeded state.
+
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--mixed-steps 100]
 """
 from __future__ import annotations
 
@@ -30,10 +40,11 @@ HBM_PEAK_GBPS = 8000.0          # MI355X HBM3E
 METRIC = "RK2 cell-updates/s @720x720x75, FP64 vs mixed/batched; achieved HBM GB/s"
 
 
-def workload_config(ni=720, nj=720, nk=75, bt=50):
+def workload_config(ni=720, nj=720, nk=75, bt=50, ah=0.0):
     from hackathon_mom6_miniapp_amd.state import BC_PERIODIC, BC_WALL, Config
     return Config(ni=ni, nj=nj, nk=nk, niter=10, bt_nsteps=bt, bc_x=BC_PERIODIC, bc_y=BC_WALL,
-                  coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, Kv=1e-4, ray_bottom=1e-4)
+                  coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, Kv=1e-4, ray_bottom=1e-4,
+                  biharmonic=ah > 0.0, Ah=ah if ah > 0.0 else 1.0e10)
 
 
 def step_bytes(cfg) -> float:
@@ -116,6 +127,54 @@ def cpu_baseline(cfg, steps_cap=1):
             "s_per_step": t_step}
 
 
+def mixed_form(cfg, args, t_step_fp64):
+    """The mixed-precision form on the same workload: its step time (same
+    warm-up and step counts as the FP64 line), and the per-field error against
+    the FP64 path after --mixed-steps steps from the same seeded state."""
+    from dataclasses import replace
+
+    import numpy as np
+
+    from hackathon_mom6_miniapp_amd.model import Model
+    from hackathon_mom6_miniapp_amd.state import make_case
+
+    mcfg = replace(cfg, precision="mixed")
+    m = Model(mcfg, make_case(mcfg, init="seeded", seed=2024, lazy3d=True))
+    m.step(args.warmup)
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.step(args.steps)
+    m.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    m.close()
+    print(f"[bench] mixed: {args.steps} steps, {t_step * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+    res = {"value": cfg.cells / t_step, "unit": "cell-updates/s", "ms_per_step": t_step * 1e3,
+           "speedup_vs_fp64": t_step_fp64 / t_step,
+           "dtype": "f32 layer fields and stencils; f64 vertical solve, barotropic solver, 2D fields"}
+    n = args.mixed_steps
+    if n > 0:
+        errs = {}
+        ref = Model(cfg, make_case(cfg, init="seeded", seed=2024, lazy3d=True))
+        mix = Model(mcfg, make_case(mcfg, init="seeded", seed=2024, lazy3d=True))
+        ref.step(n)
+        mix.step(n)
+        ref.synchronize()
+        mix.synchronize()
+        for f in ("h", "u", "v", "eta"):
+            a, b = mix.get(f), ref.get(f)
+            d = np.abs(a - b)
+            errs[f] = {"max_rel": float(d.max() / np.abs(b).max()),
+                       "rms_rel": float(np.sqrt((d * d).mean()) / np.sqrt((b * b).mean()))}
+            del a, b, d
+        ref.close()
+        mix.close()
+        res["errors_vs_fp64"] = {"steps": n, "normalised_by": "max|fp64| and rms(fp64) of each field",
+                                 "fields": errs}
+        print(f"[bench] mixed vs fp64 after {n} steps: " +
+              ", ".join(f"{f} {e['max_rel']:.2e}" for f, e in errs.items()), file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,8 +184,12 @@ def main():
     ap.add_argument("--nj", type=int, default=720)
     ap.add_argument("--nk", type=int, default=75)
     ap.add_argument("--bt", type=int, default=50)
+    ap.add_argument("--biharmonic", type=float, default=0.0, metavar="AH",
+                    help="add biharmonic viscosity with this Kh4 (m^4/s), e.g. 1e11; 0 = Laplacian only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
+    ap.add_argument("--mixed-steps", type=int, default=100,
+                    help="steps of the FP64-vs-mixed error check (0: timing only, -1: skip the mixed form)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -137,7 +200,7 @@ def main():
     from hackathon_mom6_miniapp_amd.state import make_case
     from hackathon_mom6_miniapp_amd._lib import lib, check
 
-    cfg = workload_config(args.ni, args.nj, args.nk, args.bt)
+    cfg = workload_config(args.ni, args.nj, args.nk, args.bt, args.biharmonic)
     t0 = time.perf_counter()
     case = make_case(cfg, init="seeded", seed=2024, lazy3d=True)
     print(f"[bench] state generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -147,15 +210,30 @@ def main():
 
     model.step(args.warmup)
     model.synchronize()
-    model.set_option("profile", 0.0 if args.no_profile else 1.0)
     check(L.mom6h_device_synchronize())
     t0 = time.perf_counter()
     model.step(args.steps)
     model.synchronize()
     check(L.mom6h_device_synchronize())
     elapsed = time.perf_counter() - t0
-    stages = model.stage_times() if not args.no_profile else {}
     print(f"[bench] {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # Per-kernel durations come from a second, untimed pass with HIP events
+    # around every stage and the barotropic chain moved onto the 3D stream:
+    # with the two chains overlapping (the timed pass) an event pair on the
+    # 3D stream would also count the barotropic substeps sharing the CUs.
+    stages = {}
+    prof_steps = 0
+    if not args.no_profile:
+        prof_steps = max(1, min(args.steps, 3))
+        model.set_option("serialize", 1.0)
+        model.set_option("profile", 1.0)
+        model.step(prof_steps)
+        model.synchronize()
+        stages = model.stage_times()
+        model.set_option("profile", 0.0)
+        model.set_option("serialize", 0.0)
+        print(f"[bench] {prof_steps} serialized steps timed per stage", file=sys.stderr, flush=True)
 
     t_step = elapsed / args.steps
     value = cfg.cells / t_step
@@ -163,8 +241,8 @@ def main():
     n2 = cfg.n1 * cfg.n2
     per_stage = {}
     for name, ms in stages.items():
-        calls = STAGE_CALLS.get(name, 1) * args.steps
-        per_stage[name] = {"ms_per_step": ms / args.steps, "ms_per_call": ms / calls}
+        calls = STAGE_CALLS.get(name, 1) * prof_steps
+        per_stage[name] = {"ms_per_step": ms / prof_steps, "ms_per_call": ms / calls}
     roofline = None
     if per_stage:
         cand = {k: v for k, v in per_stage.items() if k in ROOFLINE_KERNELS and v["ms_per_step"] > 0}
@@ -185,7 +263,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded SplitMix64/Irwin-Hall state)",
         "config": {"workload": f"split-RK2 step FP64 {cfg.ni}x{cfg.nj}x{cfg.nk} bt_nsteps={cfg.bt_nsteps}, "
-                               "beta-plane, zonally periodic, walls in y, Kh=1e3",
+                               "beta-plane, zonally periodic, walls in y, Kh=1e3" +
+                               (f", biharmonic Kh4={cfg.Ah:g}" if cfg.biharmonic else ""),
                    "ni": cfg.ni, "nj": cfg.nj, "nk": cfg.nk, "bt_nsteps": cfg.bt_nsteps, "dt": cfg.dt,
                    "parallelism": "single GPU"},
         "step_algorithmic_GBps": step_bytes(cfg) / t_step / 1e9,
@@ -193,6 +272,8 @@ def main():
         "stages": per_stage,
     }
     model.close()
+    if args.mixed_steps >= 0:
+        out["mixed"] = mixed_form(cfg, args, t_step)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out))

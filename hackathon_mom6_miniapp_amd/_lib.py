@@ -22,7 +22,8 @@ class MConfig(C.Structure):
                 ("laplacian", C.c_int), ("biharmonic", C.c_int), ("no_slip", C.c_int),
                 ("Kv", C.c_double), ("Kv_ml", C.c_double), ("Kv_extra_bbl", C.c_double),
                 ("Hmix", C.c_double), ("Hbbl", C.c_double),
-                ("coriolis_scheme", C.c_int), ("monotonic", C.c_int)]
+                ("coriolis_scheme", C.c_int), ("monotonic", C.c_int), ("meridional", C.c_int),
+                ("precision", C.c_int)]
 
 
 _lib = None
@@ -47,7 +48,9 @@ _SIGS = {
                              P, C.c_long, P], C.c_int),
     "mom6h_model_create": ([C.POINTER(P), C.POINTER(MConfig)], C.c_int),
     "mom6h_model_field": ([P, C.c_char_p, C.POINTER(P), C.POINTER(C.c_long)], C.c_int),
+    "mom6h_model_field_bytes": ([P, C.c_char_p, C.POINTER(C.c_int)], C.c_int),
     "mom6h_model_upload": ([P, C.c_char_p, P, C.c_long], C.c_int),
+    "mom6h_model_fill": ([P, C.c_char_p, P, C.c_long], C.c_int),
     "mom6h_model_download": ([P, C.c_char_p, P, C.c_long], C.c_int),
     "mom6h_model_prepare": ([P], C.c_int),
     "mom6h_model_step": ([P, C.c_int], C.c_int),

@@ -347,63 +347,84 @@ __global__ void k_mass_flux(Dims d, const double *__restrict__ u, const double *
 // identically and only one writes.
 constexpr int CF = 64;
 
-struct ContArgs {
+// Arguments of one pass.  Names are those of the zonal pass; the meridional
+// pass passes v, vh, ... and its own metrics in the same slots:
+//   IdxT -> IdyT, dy_Cu -> dx_Cv, dxT -> dyT, dxCu -> dyCv, mask2dCu -> mask2dCv.
+// T is the type of the 3D layer fields and of the flux arithmetic: double
+// (the reference's FP64) or float (the mixed-precision form); metrics, the
+// 2D BT_cont outputs and du_cor stay FP64 in memory.
+template <typename T>
+struct ContArgsT {
     Dims d;
-    const double *u, *hin;
-    double *h, *uh;
+    const T *u, *hin;
+    T *h, *uh;
     double dt;
     const double *uhbt;
     const double *IareaT, *IdxT, *dy_Cu, *dxT, *dxCu, *mask2dCu, *mask2dT;
     double tol_eta0, tol_vel, CFL_limit_adjust, h_min;
-    double *u_cor, *du_cor;
-    double *FA_u_EE, *FA_u_E0, *FA_u_W0, *FA_u_WW, *uBT_WW, *uBT_EE, *bt_h_u;
+    T *u_cor;
+    double *du_cor;
+    double *FA_u_EE, *FA_u_E0, *FA_u_W0, *FA_u_WW, *uBT_WW, *uBT_EE;
+    T *bt_h_u;
 };
+using ContArgs = ContArgsT<double>;
+
+__device__ __forceinline__ double vabs(double x) { return fabs(x); }
+__device__ __forceinline__ float vabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double vsign1(double s) { return copysign(1.0, s); }
+__device__ __forceinline__ float vsign1(float s) { return copysignf(1.0f, s); }
+template <typename T>
+__device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
 
 // zonal_flux_layer (:516-598) with the upwind cell selected instead of
 // branched; the same operations in the same order as zflux().
-__device__ __forceinline__ void zflux_sel(double u, double dt, double dy, double IdxT0, double IdxT1, double hW0,
-                                 double hE0, double h0, double hW1, double hE1, double h1, double &uh,
-                                 double &hmarg)
+template <typename T>
+__device__ __forceinline__ void zflux_sel(T u, T dt, T dy, T IdxT0, T IdxT1, T hW0, T hE0, T h0, T hW1, T hE1,
+                                          T h1, T &uh, T &hmarg)
 {
-    const bool pos = u > 0.0;
-    const double A = pos ? hE0 : hW1;       // upwind edge facing the face
-    const double B = pos ? hW0 : hE1;       // upwind far edge
-    const double hc = pos ? h0 : h1;
-    const double c = u * dt * (pos ? IdxT0 : IdxT1);
-    const double CFL = pos ? c : -c;
-    const double curv_3 = (B + A) - 2.0 * hc;
-    const double D = B - A;
-    const double f = (dy * u) * (A + CFL * (0.5 * D + curv_3 * (CFL - 1.5)));
-    const double hm = A + CFL * (D + 3.0 * curv_3 * (CFL - 1.0));
-    const bool zero = !(u > 0.0) && !(u < 0.0);
-    uh = zero ? 0.0 : f;
-    hmarg = zero ? 0.5 * (hW1 + hE0) : hm;
+    const bool pos = u > T(0.0);
+    const T A = pos ? hE0 : hW1;       // upwind edge facing the face
+    const T B = pos ? hW0 : hE1;       // upwind far edge
+    const T hc = pos ? h0 : h1;
+    const T c = u * dt * (pos ? IdxT0 : IdxT1);
+    const T CFL = pos ? c : -c;
+    const T curv_3 = (B + A) - T(2.0) * hc;
+    const T D = B - A;
+    const T f = (dy * u) * (A + CFL * (T(0.5) * D + curv_3 * (CFL - T(1.5))));
+    const T hm = A + CFL * (D + T(3.0) * curv_3 * (CFL - T(1.0)));
+    const bool zero = !(u > T(0.0)) && !(u < T(0.0));
+    uh = zero ? T(0.0) : f;
+    hmarg = zero ? T(0.5) * (hW1 + hE0) : hm;
 }
 
 // PPM_reconstruction_x (:1124-1247) slope from the three cells, branch-free
-__device__ __forceinline__ double ppm_slope_v(double mL, double m0, double mR, double hm, double h0, double hp)
+template <typename T>
+__device__ __forceinline__ T ppm_slope_v(T mL, T m0, T mR, T hm, T h0, T hp)
 {
-    const double s = 0.5 * (hp - hm);
-    const double dMx = dmax(dmax(hp, hm), h0) - h0;
-    const double dMn = h0 - dmin(dmin(hp, hm), h0);
-    const double slp = copysign(1.0, s) * dmin(fabs(s), 2.0 * dmin(dMx, dMn));
-    return ((mL * m0 * mR) == 0.0) ? 0.0 : slp;
+    const T s = T(0.5) * (hp - hm);
+    const T dMx = vmax(vmax(hp, hm), h0) - h0;
+    const T dMn = h0 - vmin(vmin(hp, hm), h0);
+    const T slp = vsign1(s) * vmin(vabs(s), T(2.0) * vmin(dMx, dMn));
+    return ((mL * m0 * mR) == T(0.0)) ? T(0.0) : slp;
 }
 
 // Unlimited PPM edges of the cell at offset q of h6/m6 (cells i-2 .. i+3)
-__device__ __forceinline__ void ppm_edges_v(const double *m6, const double *h6, const double *sl, int q, double &hl,
-                                   double &hr)
+template <typename T>
+__device__ __forceinline__ void ppm_edges_v(const T *m6, const T *h6, const T *sl, int q, T &hl, T &hr)
 {
-    const double hi = h6[q];
-    const double mm = m6[q - 1], mp = m6[q + 1];
-    const double h_im1 = mm * h6[q - 1] + (1.0 - mm) * hi;
-    const double h_ip1 = mp * h6[q + 1] + (1.0 - mp) * hi;
-    const double oneSixth = 1.0 / 6.0;
-    hl = 0.5 * (h_im1 + hi) + oneSixth * (sl[q - 2] - sl[q - 1]);
-    hr = 0.5 * (h_ip1 + hi) + oneSixth * (sl[q - 1] - sl[q]);
+    const T hi = h6[q];
+    const T mm = m6[q - 1], mp = m6[q + 1];
+    const T h_im1 = mm * h6[q - 1] + (T(1.0) - mm) * hi;
+    const T h_ip1 = mp * h6[q + 1] + (T(1.0) - mp) * hi;
+    const T oneSixth = T(1.0) / T(6.0);
+    hl = T(0.5) * (h_im1 + hi) + oneSixth * (sl[q - 2] - sl[q - 1]);
+    hr = T(0.5) * (h_ip1 + hi) + oneSixth * (sl[q - 1] - sl[q]);
 }
 
-// PPM_limit_pos (:1253-1294) / PPM_limit_CW84 (:1298-1338) on one cell
+// PPM_limit_pos (:1253-1294) / PPM_limit_CW84 (:1298-1338) on one cell, with
+// the reference's REAL(4) locals
 __device__ __forceinline__ void ppm_limit(int monotonic, double hi, double h_min, double &hl, double &hr)
 {
     if (monotonic) {
@@ -441,11 +462,48 @@ __device__ __forceinline__ void ppm_limit(int monotonic, double hi, double h_min
     }
 }
 
-template <int K>
+// The same limiters entirely in FP32 (the mixed-precision form)
+__device__ __forceinline__ void ppm_limit(int monotonic, float hi, float h_min, float &hl, float &hr)
+{
+    if (monotonic) {
+        if ((hr - hi) * (hi - hl) <= 0.0f) {
+            hl = hi;
+            hr = hi;
+        } else {
+            const float RLdiff = hr - hl;
+            const float RLmean = 0.5f * (hr + hl);
+            const float FunFac = 6.0f * RLdiff * (hi - RLmean);
+            const float RLdiff2 = RLdiff * RLdiff;
+            if (FunFac > RLdiff2) hl = 3.0f * hi - 2.0f * hr;
+            if (FunFac < -RLdiff2) hr = 3.0f * hi - 2.0f * hl;
+        }
+        return;
+    }
+    const float curv = 3.0f * ((hl + hr) - 2.0f * hi);
+    if (curv > 0.0f) {
+        const float dh = hr - hl;
+        if (fabsf(dh) < curv) {
+            if (hi <= h_min) {
+                hl = hi;
+                hr = hi;
+            } else {
+                const float lhs = (12.0f * curv) * (hi - h_min);
+                const float rhs = curv * curv + 3.0f * (dh * dh);
+                if (lhs < rhs) {
+                    const float scale = lhs / rhs;
+                    hl = hi + scale * (hl - hi);
+                    hr = hi + scale * (hr - hi);
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int K>
 struct ColRegs {
-    double wW[K], wE[K], w0[K], eW[K], eE[K], e0[K];
-    double dt, dy, IdxT0, IdxT1;
-    __device__ __forceinline__ void eval(int kk, double un, double &f, double &hm) const
+    T wW[K], wE[K], w0[K], eW[K], eE[K], e0[K];
+    T dt, dy, IdxT0, IdxT1;
+    __device__ __forceinline__ void eval(int kk, T un, T &f, T &hm) const
     {
         zflux_sel(un, dt, dy, IdxT0, IdxT1, wW[kk], wE[kk], w0[kk], eW[kk], eE[kk], e0[kk], f, hm);
     }
@@ -463,20 +521,20 @@ struct ColRegs {
 // LDS of one block: U = u of every level (levels past nk repeat level nk; u
 // lives here rather than in registers), S0/S1 = per-level terms of a pass,
 // R = results of the column sums (one row per quantity)
-template <int CW, int K>
+template <typename T, int CW, int K>
 struct ContShared {
     static constexpr int L = CW * K;
-    double U[L][CF], S0[L][CF], S1[L][CF], R[4][CF];
+    T U[L][CF], S0[L][CF], S1[L][CF], R[5][CF];
 };
 
 // Sequential fold of rows 0..nk-1 of one LDS column (k order), the reads
 // issued a batch ahead of the dependent chain.
-template <int L, typename Op>
-__device__ __forceinline__ double lds_fold(const double (*col)[CF], int nk, int lane, double acc, Op op)
+template <int L, typename T, typename Op>
+__device__ __forceinline__ T lds_fold(const T (*col)[CF], int nk, int lane, T acc, Op op)
 {
     constexpr int B = 16;
     for (int k0 = 0; k0 < nk; k0 += B) {
-        double x[B];
+        T x[B];
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int k = k0 + b;
@@ -488,48 +546,99 @@ __device__ __forceinline__ double lds_fold(const double (*col)[CF], int nk, int 
     return acc;
 }
 
-template <int CW, int K>
-__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(1, (CW >= 16) ? 4 : 2)))
-k_continuity_blk(ContArgs a, int monotonic)
+// Column sum x0 + sum_k col[k] in k order: plain FP64 adds (the reference's
+// summation), or for FP32 terms a Kahan-compensated sum.
+template <int L>
+__device__ __forceinline__ double col_sum(const double (*col)[CF], int nk, int lane, double x0)
 {
+    return lds_fold<L>(col, nk, lane, x0, [](double a_, double x_) { return a_ + x_; });
+}
+
+template <int L>
+__device__ __forceinline__ float col_sum(const float (*col)[CF], int nk, int lane, float x0)
+{
+    constexpr int B = 16;
+    float acc = x0, c = 0.0f;
+    for (int k0 = 0; k0 < nk; k0 += B) {
+        float x[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int k = k0 + b;
+            x[b] = col[k < L ? k : L - 1][lane];
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const float y = x[b] - c;
+            const float t = acc + y;
+            const float cn = (t - acc) - y;
+            const bool in = k0 + b < nk;
+            acc = in ? t : acc;
+            c = in ? cn : c;
+        }
+    }
+    return acc;
+}
+
+// Y = false: the zonal pass.  A block covers the faces I0-1 .. I0+62 of row j
+// and also updates the thickness of cells I0 .. I0+62 (the convergence needs
+// the neighbouring lane's transport).  Y = true: the meridional pass, the same
+// algorithm across the rows: a block covers the faces (i, J), i = i0 .. i0+63,
+// of one face row J (lanes still run along i, so every load stays coalesced);
+// its convergence is a separate kernel (the faces J-1 belong to another block).
+//
+// FP32 form: the Newton stopping test of zonal_flux_adjust also accepts a
+// residual at the rounding level of the layer fluxes (4 ulp of sum |uh_k| of
+// the first pass); the FP64 tolerance of 1e-12 m lies below what FP32 sums
+// resolve, and the iteration would otherwise run to its 20-pass limit.
+template <typename T, bool Y, int CW, int K>
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(1, (CW >= 16) ? 4 : 2)))
+k_continuity_blk(ContArgsT<T> a, int monotonic)
+{
+    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int L = ContShared<T, CW, K>::L;
     extern __shared__ double smem_raw[];
-    ContShared<CW, K> &S = *reinterpret_cast<ContShared<CW, K> *>(smem_raw);
+    ContShared<T, CW, K> &S = *reinterpret_cast<ContShared<T, CW, K> *>(smem_raw);
     const Dims &d = a.d;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int j = d.jsc + blockIdx.y;
-    const int I_raw = d.isc - 1 + blockIdx.x * (CF - 1) + lane;
-    const int I = I_raw > d.iec ? d.iec : I_raw;            // clamped: evaluated, never written
-    const bool live = I_raw <= d.iec;
-    const int i = I;
+    // face (fi, fj) between cell 0 = (fi, fj) and cell 1 = (fi+1, fj) [x] or (fi, fj+1) [y]
+    const int f_raw = Y ? d.isc + blockIdx.x * CF + lane : d.isc - 1 + blockIdx.x * (CF - 1) + lane;
+    const int f_max = d.iec;
+    const int fi = f_raw > f_max ? f_max : f_raw;           // clamped: evaluated, never written
+    const int fj = Y ? d.jsc - 1 + (int)blockIdx.y : d.jsc + (int)blockIdx.y;
+    const bool live = f_raw <= f_max;
+    auto cell = [&](int q) __attribute__((always_inline)) -> long long {   // q-th cell of the stencil, q = 2: cell 0
+        return Y ? IX2(d, fi, fj - 2 + q) : IX2(d, fi - 2 + q, fj);
+    };
+    const int i = fi, j = fj;
     const int nk = d.nk;
     const long long s = d.s2;
     const int k0 = w * K;                                    // my levels: k0+1 .. k0+K
     auto valid = [&](int kk) __attribute__((always_inline)) { return k0 + kk < nk; };
 
-    ColRegs<K> col;
-    col.dt = a.dt;
-    const long long c2 = IX2(d, I, j);
-    col.dy = a.dy_Cu[c2];
-    col.IdxT0 = a.IdxT[IX2(d, i, j)];
-    col.IdxT1 = a.IdxT[IX2(d, i + 1, j)];
-    const double mCu = a.mask2dCu[c2];
-    const long long cu = IX3(d, I, j, 1);
-    // PPM edges of cells i and i+1 from h and mask2dT at i-2 .. i+3
+    ColRegs<T, K> col;
+    col.dt = (T)a.dt;
+    const long long c2 = IX2(d, fi, fj);
+    col.dy = (T)a.dy_Cu[c2];
+    col.IdxT0 = (T)a.IdxT[cell(2)];
+    col.IdxT1 = (T)a.IdxT[cell(3)];
+    const T mCu = (T)a.mask2dCu[c2];
+    const long long cu = IX3(d, fi, fj, 1);
+    // PPM edges of cells 0 and 1 from h and mask2dT on the 6-cell stencil
     // (zonal_edge_thickness :185-207); levels past nk repeat level nk
     {
-        double m6[6];
+        T m6[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) m6[q] = a.mask2dT[IX2(d, i - 2 + q, j)];
+        for (int q = 0; q < 6; ++q) m6[q] = (T)a.mask2dT[cell(q)];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
             const int kr = k0 + kk + 1;
             const int k = kr < nk ? kr : nk;
             const long long o = (long long)(k - 1) * s;
             S.U[k0 + kk][lane] = a.u[cu + o];
-            double h6[6], sl[4];
+            T h6[6], sl[4];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) h6[q] = a.hin[IX3(d, i - 2 + q, j, 1) + o];
+            for (int q = 0; q < 6; ++q) h6[q] = a.hin[cell(q) + o];
 #pragma unroll
             for (int q = 0; q < 4; ++q) sl[q] = ppm_slope_v(m6[q], m6[q + 1], m6[q + 2], h6[q], h6[q + 1], h6[q + 2]);
             ppm_edges_v(m6, h6, sl, 2, col.wW[kk], col.wE[kk]);
@@ -539,20 +648,20 @@ k_continuity_blk(ContArgs a, int monotonic)
         }
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            ppm_limit(monotonic, col.w0[kk], a.h_min, col.wW[kk], col.wE[kk]);
-            ppm_limit(monotonic, col.e0[kk], a.h_min, col.eW[kk], col.eE[kk]);
+            ppm_limit(monotonic, col.w0[kk], (T)a.h_min, col.wW[kk], col.wE[kk]);
+            ppm_limit(monotonic, col.e0[kk], (T)a.h_min, col.eW[kk], col.eE[kk]);
         }
     }
 
-    auto uk = [&](int kk) __attribute__((always_inline)) -> double { return S.U[k0 + kk][lane]; };
-    double fk[K];   // layer transports of the last evaluation
+    auto uk = [&](int kk) __attribute__((always_inline)) -> T { return S.U[k0 + kk][lane]; };
+    T fk[K];   // layer transports of the last evaluation
     // one pass: evaluate every level at u + du; terms (fk, dy*hm) to LDS
-    auto eval_pass = [&](double du, bool on) __attribute__((always_inline)) {
+    auto eval_pass = [&](T du, bool on) __attribute__((always_inline)) {
         if (!on) return;
         col.fence();
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            double hm;
+            T hm;
             col.eval(kk, uk(kk) + du, fk[kk], hm);
             if (valid(kk)) {
                 S.S0[k0 + kk][lane] = fk[kk];
@@ -561,76 +670,77 @@ k_continuity_blk(ContArgs a, int monotonic)
         }
     };
     // column sums in k order by wavefronts 0 and 1 (R[0] = x0 + sum S0, R[1] = x1 + sum S1)
-    auto sum_pass = [&](double x0, double x1, bool on) __attribute__((always_inline)) {
+    auto sum_pass = [&](T x0, T x1, bool on) __attribute__((always_inline)) {
         __syncthreads();
-        if (w < 2 && on) {
-            const auto plus = [](double a_, double x_) { return a_ + x_; };
-            S.R[w][lane] = (w == 0) ? lds_fold<ContShared<CW, K>::L>(S.S0, nk, lane, x0, plus)
-                                    : lds_fold<ContShared<CW, K>::L>(S.S1, nk, lane, x1, plus);
-        }
+        if (w < 2 && on)
+            S.R[w][lane] = (w == 0) ? col_sum<L>(S.S0, nk, lane, x0) : col_sum<L>(S.S1, nk, lane, x1);
         __syncthreads();
     };
 
     // first layer fluxes, their sums and the CFL bracket (zonal_mass_flux :300-393)
-    const double CFL_dt = a.CFL_limit_adjust / a.dt;
-    const double dx_W = a.dxT[IX2(d, i, j)], dx_E = a.dxT[IX2(d, i + 1, j)];
-    eval_pass(0.0, true);
+    const T CFL_dt = (T)a.CFL_limit_adjust / (T)a.dt;
+    const T dx_W = (T)a.dxT[cell(2)], dx_E = (T)a.dxT[cell(3)];
+    eval_pass(T(0.0), true);
     __syncthreads();
     if (w < 2) {
-        const auto plus = [](double a_, double x_) { return a_ + x_; };
-        S.R[w][lane] = (w == 0) ? lds_fold<ContShared<CW, K>::L>(S.S0, nk, lane, 0.0, plus) : lds_fold<ContShared<CW, K>::L>(S.S1, nk, lane, 0.0, plus);
+        S.R[w][lane] = (w == 0) ? col_sum<L>(S.S0, nk, lane, T(0.0)) : col_sum<L>(S.S1, nk, lane, T(0.0));
     } else if (w == 2) {
-        S.R[2][lane] = lds_fold<ContShared<CW, K>::L>(S.U, nk, lane, 2.0 * (CFL_dt * dx_W) * 1.0, [&](double x, double uu) {
+        S.R[2][lane] = lds_fold<L>(S.U, nk, lane, T(2.0) * (CFL_dt * dx_W) * T(1.0), [&](T x, T uu) {
             return (x > dx_W * CFL_dt - uu * mCu) ? (dx_W * CFL_dt - uu) : x;
         });
     } else if (w == 3) {
-        S.R[3][lane] = lds_fold<ContShared<CW, K>::L>(S.U, nk, lane, -(2.0 * (CFL_dt * dx_E) * 1.0), [&](double x, double uu) {
+        S.R[3][lane] = lds_fold<L>(S.U, nk, lane, -(T(2.0) * (CFL_dt * dx_E) * T(1.0)), [&](T x, T uu) {
             return (x < -dx_E * CFL_dt - uu * mCu) ? -(dx_E * CFL_dt + uu) : x;
         });
+    } else if (F32 && w == 4) {   // scale of the rounding of the flux sum
+        S.R[4][lane] = lds_fold<L>(S.S0, nk, lane, T(0.0), [](T x, T f) { return x + vabs(f); });
     }
     __syncthreads();
-    const double uh_tot_0 = S.R[0][lane], duhdu_tot_0 = S.R[1][lane];
-    const double du_max_CFL = dmax(S.R[2][lane], 0.0), du_min_CFL = dmin(S.R[3][lane], 0.0);
-    const double dtIarea = a.dt * dmin(a.IareaT[IX2(d, i, j)], a.IareaT[IX2(d, i + 1, j)]);
-    const double bt = a.uhbt ? a.uhbt[c2] : 0.0;
+    const T uh_tot_0 = S.R[0][lane], duhdu_tot_0 = S.R[1][lane];
+    const T du_max_CFL = vmax(S.R[2][lane], T(0.0)), du_min_CFL = vmin(S.R[3][lane], T(0.0));
+    const T dtIarea = (T)(a.dt * dmin(a.IareaT[cell(2)], a.IareaT[cell(3)]));
+    const T bt = a.uhbt ? (T)a.uhbt[c2] : T(0.0);
+    const T tol_floor = F32 ? dtIarea * (T(4.0f * 1.1920929e-7f) * S.R[4][lane]) : T(0.0);
+    const T tol_eta0 = (T)a.tol_eta0, tol_vel = (T)a.tol_vel;
 
     // zonal_flux_adjust (:739-916).  keep: uh_3d present (fluxes of the last
     // evaluation are the result; evaluated also at itt = max_itts).  Returns
     // du; *du_eval = du of the last evaluation that was summed (NaN if none).
-    auto flux_adjust = [&](double uhbt, bool keep, double *du_eval) __attribute__((always_inline)) -> double {
+    auto flux_adjust = [&](T uhbt, bool keep, T *du_eval) __attribute__((always_inline)) -> T {
         const int max_itts = 20;
-        double du = 0.0, du_max = du_max_CFL, du_min = du_min_CFL;
-        double uh_err = uh_tot_0 - uhbt, duhdu_tot = duhdu_tot_0;
-        double uh_err_best = fabs(uh_err);
+        T du = T(0.0), du_max = du_max_CFL, du_min = du_min_CFL;
+        T uh_err = uh_tot_0 - uhbt, duhdu_tot = duhdu_tot_0;
+        T uh_err_best = vabs(uh_err);
         bool do_I = true;
-        *du_eval = 0.0;   // the first pass evaluated at du = 0
+        *du_eval = T(0.0);   // the first pass evaluated at du = 0
         for (int itt = 1; itt <= max_itts; ++itt) {
             if (do_I) {
-                double tol_eta;
-                if (itt <= 1) tol_eta = (double)1e-6f * a.tol_eta0;
-                else if (itt == 2) tol_eta = (double)1e-4f * a.tol_eta0;
-                else if (itt == 3) tol_eta = (double)1e-2f * a.tol_eta0;
-                else tol_eta = a.tol_eta0;
-                if (uh_err > 0.0) du_max = du;
-                else if (uh_err < 0.0) du_min = du;
+                T tol_eta;
+                if (itt <= 1) tol_eta = (T)1e-6f * tol_eta0;
+                else if (itt == 2) tol_eta = (T)1e-4f * tol_eta0;
+                else if (itt == 3) tol_eta = (T)1e-2f * tol_eta0;
+                else tol_eta = tol_eta0;
+                if (F32) tol_eta = vmax(tol_eta, tol_floor);
+                if (uh_err > T(0.0)) du_max = du;
+                else if (uh_err < T(0.0)) du_min = du;
                 else do_I = false;
                 if (do_I) {
-                    const double aerr = fabs(uh_err);
-                    if ((dtIarea * aerr > tol_eta) || ((aerr > a.tol_vel * duhdu_tot) || (aerr > uh_err_best))) {
-                        const double ddu = -uh_err / duhdu_tot;
-                        const double du_prev = du;
+                    const T aerr = vabs(uh_err);
+                    if ((dtIarea * aerr > tol_eta) || ((aerr > tol_vel * duhdu_tot) || (aerr > uh_err_best))) {
+                        const T ddu = -uh_err / duhdu_tot;
+                        const T du_prev = du;
                         du = du + ddu;
-                        if (fabs(ddu) < (double)1.0e-15f * fabs(du)) {
+                        if (vabs(ddu) < (T)1.0e-15f * vabs(du)) {
                             do_I = false;
-                        } else if (ddu > 0.0) {
+                        } else if (ddu > T(0.0)) {
                             if (du >= du_max) {
-                                du = 0.5 * (du_prev + du_max);
-                                if (du_max - du_prev < (double)1.0e-15f * fabs(du)) do_I = false;
+                                du = T(0.5) * (du_prev + du_max);
+                                if (du_max - du_prev < (T)1.0e-15f * vabs(du)) do_I = false;
                             }
                         } else {
                             if (du <= du_min) {
-                                du = 0.5 * (du_prev + du_min);
-                                if (du_prev - du_min < (double)1.0e-15f * fabs(du)) do_I = false;
+                                du = T(0.5) * (du_prev + du_min);
+                                if (du_prev - du_min < (T)1.0e-15f * vabs(du)) do_I = false;
                             }
                         }
                     } else {
@@ -643,28 +753,28 @@ k_continuity_blk(ContArgs a, int monotonic)
             if (!__syncthreads_or(ev)) break;
             eval_pass(du, ev);
             const bool summed = ev && itt < max_itts;
-            sum_pass(-uhbt, 0.0, summed);
+            sum_pass(-uhbt, T(0.0), summed);
             if (summed) {
                 uh_err = S.R[0][lane];
                 duhdu_tot = S.R[1][lane];
-                uh_err_best = dmin(uh_err_best, fabs(uh_err));
+                uh_err_best = vmin(uh_err_best, vabs(uh_err));
                 *du_eval = du;
             } else if (ev) {
-                *du_eval = NAN;
+                *du_eval = (T)NAN;
             }
         }
         return du;
     };
-    double du_eval;
-    const double du = flux_adjust(bt, true, &du_eval);
+    T du_eval;
+    const T du = flux_adjust(bt, true, &du_eval);
     // here fk holds the layer fluxes of the last evaluation and, when
     // du_eval == du, R[1] the sum of dy*hm at u + du
-    const double sum_dyhm = S.R[1][lane];
+    const T sum_dyhm = S.R[1][lane];
     const bool last_at_du = (du_eval == du);
 
     // outputs at the face: uh, u_cor, du_cor; a face shared with the western
     // neighbour block is written by that block
-    const bool writes_face = live && (lane > 0 || blockIdx.x == 0);
+    const bool writes_face = live && (Y || lane > 0 || blockIdx.x == 0);
     if (writes_face) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
@@ -674,18 +784,18 @@ k_continuity_blk(ContArgs a, int monotonic)
                 if (a.u_cor) a.u_cor[o] = uk(kk) + du;
             }
         }
-        if (a.du_cor && w == 0) a.du_cor[c2] = du;
+        if (a.du_cor && w == 0) a.du_cor[c2] = (double)du;
     }
     // continuity_zonal_convergence (:148-176) for cell i = I (faces I-1 and I)
-    {
-        const double dtIareaT = a.dt * a.IareaT[IX2(d, i, j)];
+    if (!Y) {
+        const T dtIareaT = (T)(a.dt * a.IareaT[IX2(d, i, j)]);
         const bool writes_cell = live && lane > 0 && i >= d.isc;
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            const double fw = __shfl_up(fk[kk], 1, 64);   // face I-1, same level
+            const T fw = __shfl_up(fk[kk], 1, 64);   // face I-1, same level
             if (writes_cell && valid(kk)) {
-                const double v = col.w0[kk] - dtIareaT * (fk[kk] - fw);
-                a.h[IX3(d, i, j, k0 + kk + 1)] = dmax(v, 0.0);
+                const T v = col.w0[kk] - dtIareaT * (fk[kk] - fw);
+                a.h[IX3(d, i, j, k0 + kk + 1)] = vmax(v, T(0.0));
             }
         }
     }
@@ -693,79 +803,80 @@ k_continuity_blk(ContArgs a, int monotonic)
 
     // set_zonal_BT_cont (:920-1121).  The zero-transport correction equals du
     // whenever the target is zero (identical Newton iterations).
-    double du0 = du, du0_eval = du_eval;
-    if (__syncthreads_or(bt != 0.0)) {
-        double de;
-        const double d2 = flux_adjust(0.0, false, &de);
-        if (bt != 0.0) { du0 = d2; du0_eval = de; }
+    T du0 = du, du0_eval = du_eval;
+    if (__syncthreads_or(bt != T(0.0))) {
+        T de;
+        const T d2 = flux_adjust(T(0.0), false, &de);
+        if (bt != T(0.0)) { du0 = d2; du0_eval = de; }
     }
     // FAmt_0 = sum dy*hm(u + du0): the sum of the last evaluation when that
     // was at du0 == du, else a pass of its own
-    const bool have0 = (bt == 0.0) && last_at_du && (du0_eval == du0);
-    double FAmt_0 = sum_dyhm;
+    const bool have0 = (bt == T(0.0)) && last_at_du && (du0_eval == du0);
+    T FAmt_0 = sum_dyhm;
     if (__syncthreads_or(!have0)) {
         eval_pass(du0, !have0);
-        sum_pass(0.0, 0.0, !have0);
+        sum_pass(T(0.0), T(0.0), !have0);
         if (!have0) FAmt_0 = S.R[1][lane];
     }
     // h_u = marginal thickness at u_cor = u + du (zonal_flux_thickness :602-735)
     if (writes_face) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            double fc, hm;
+            T fc, hm;
             col.eval(kk, uk(kk) + du, fc, hm);
             if (valid(kk)) a.bt_h_u[cu + (long long)(k0 + kk) * s] = hm;
         }
     }
     // test velocities duR, duL (visc_rem_lim = 1)
-    const double Idt = 1.0 / a.dt;
-    const double CFL_min = (double)1e-6f;
-    const double du_CFL = (CFL_min * Idt) * a.dxCu[c2];
+    const T Idt = T(1.0) / (T)a.dt;
+    const T CFL_min = (T)1e-6f;
+    const T du_CFL = (CFL_min * Idt) * (T)a.dxCu[c2];
     __syncthreads();
     if (w == 0) {
-        S.R[2][lane] = lds_fold<ContShared<CW, K>::L>(S.U, nk, lane, dmin(0.0, du0 - du_CFL), [&](double x, double uu) {
+        S.R[2][lane] = lds_fold<L>(S.U, nk, lane, vmin(T(0.0), du0 - du_CFL), [&](T x, T uu) {
             return (uu + x > -du_CFL) ? -(uu + du_CFL) : x;
         });
     } else if (w == 1) {
-        S.R[3][lane] = lds_fold<ContShared<CW, K>::L>(S.U, nk, lane, dmax(0.0, du0 + du_CFL), [&](double x, double uu) {
+        S.R[3][lane] = lds_fold<L>(S.U, nk, lane, vmax(T(0.0), du0 + du_CFL), [&](T x, T uu) {
             return (uu + x < du_CFL) ? -(uu - du_CFL) : x;
         });
     }
     __syncthreads();
-    const double duR = S.R[2][lane], duL = S.R[3][lane];
+    const T duR = S.R[2][lane], duL = S.R[3][lane];
     // FAmt_L, uhtot_L at u + duL; FAmt_R, uhtot_R at u + duR
     eval_pass(duL, true);
-    sum_pass(0.0, 0.0, true);
-    const double uhtot_L = S.R[0][lane], FAmt_L = S.R[1][lane];
+    sum_pass(T(0.0), T(0.0), true);
+    const T uhtot_L = S.R[0][lane], FAmt_L = S.R[1][lane];
     eval_pass(duR, true);
-    sum_pass(0.0, 0.0, true);
-    const double uhtot_R = S.R[0][lane], FAmt_R = S.R[1][lane];
+    sum_pass(T(0.0), T(0.0), true);
+    const T uhtot_R = S.R[0][lane], FAmt_R = S.R[1][lane];
     if (!writes_face || w != 0) return;
-    double FA_0 = FAmt_0, FA_avg = FAmt_0;
-    if ((duL - du0) != 0.0) FA_avg = uhtot_L / (duL - du0);
-    if (FA_avg > dmax(FA_0, FAmt_L)) FA_avg = dmax(FA_0, FAmt_L);
-    else if (FA_avg < dmin(FA_0, FAmt_L)) FA_0 = FA_avg;
-    a.FA_u_W0[c2] = FA_0;
-    a.FA_u_WW[c2] = FAmt_L;
-    a.uBT_WW[c2] = (fabs(FA_0 - FAmt_L) <= (double)1e-12f * FA_0)
-                       ? 0.0
-                       : (1.5 * (duL - du0)) * ((FAmt_L - FA_avg) / (FAmt_L - FA_0));
+    T FA_0 = FAmt_0, FA_avg = FAmt_0;
+    if ((duL - du0) != T(0.0)) FA_avg = uhtot_L / (duL - du0);
+    if (FA_avg > vmax(FA_0, FAmt_L)) FA_avg = vmax(FA_0, FAmt_L);
+    else if (FA_avg < vmin(FA_0, FAmt_L)) FA_0 = FA_avg;
+    a.FA_u_W0[c2] = (double)FA_0;
+    a.FA_u_WW[c2] = (double)FAmt_L;
+    a.uBT_WW[c2] = (double)((vabs(FA_0 - FAmt_L) <= (T)1e-12f * FA_0)
+                                ? T(0.0)
+                                : (T(1.5) * (duL - du0)) * ((FAmt_L - FA_avg) / (FAmt_L - FA_0)));
     FA_0 = FAmt_0;
     FA_avg = FAmt_0;
-    if ((duR - du0) != 0.0) FA_avg = uhtot_R / (duR - du0);
-    if (FA_avg > dmax(FA_0, FAmt_R)) FA_avg = dmax(FA_0, FAmt_R);
-    else if (FA_avg < dmin(FA_0, FAmt_R)) FA_0 = FA_avg;
-    a.FA_u_E0[c2] = FA_0;
-    a.FA_u_EE[c2] = FAmt_R;
-    a.uBT_EE[c2] = (fabs(FAmt_R - FA_0) <= (double)1e-12f * FA_0)
-                       ? 0.0
-                       : (1.5 * (duR - du0)) * ((FAmt_R - FA_avg) / (FAmt_R - FA_0));
+    if ((duR - du0) != T(0.0)) FA_avg = uhtot_R / (duR - du0);
+    if (FA_avg > vmax(FA_0, FAmt_R)) FA_avg = vmax(FA_0, FAmt_R);
+    else if (FA_avg < vmin(FA_0, FAmt_R)) FA_0 = FA_avg;
+    a.FA_u_E0[c2] = (double)FA_0;
+    a.FA_u_EE[c2] = (double)FAmt_R;
+    a.uBT_EE[c2] = (double)((vabs(FAmt_R - FA_0) <= (T)1e-12f * FA_0)
+                                ? T(0.0)
+                                : (T(1.5) * (duR - du0)) * ((FAmt_R - FA_avg) / (FAmt_R - FA_0)));
 }
 
 // the driver's `h = h0` reset for the cells the convergence does not write:
 // rows outside jsc..jec in full, and the isd..isc-1 / iec+1..ied strips of the
 // compute rows (one thread per halo cell of a layer)
-__global__ void k_halo_copy(Dims d, const double *__restrict__ hin, double *__restrict__ h)
+template <typename T>
+__global__ void k_halo_copy(Dims d, const T *__restrict__ hin, T *__restrict__ h)
 {
     const int k = 1 + blockIdx.y;
     const int rows_out = (d.jsc - d.jsd) + (d.jed - d.jec);
@@ -788,6 +899,21 @@ __global__ void k_halo_copy(Dims d, const double *__restrict__ hin, double *__re
     h[o] = hin[o];
 }
 
+// The meridional counterpart of continuity_zonal_convergence:
+// h(i, j) = max(hin - dt*IareaT*(vh(i, J=j) - vh(i, J=j-1)), 0) on the compute domain.
+template <typename T>
+__global__ void k_convergence_y(Dims d, const T *hin, const T *__restrict__ vh, double dt,
+                                const double *__restrict__ IareaT, T *h)
+{
+    const int i = d.isc + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.jsc + blockIdx.y;
+    const int k = 1 + blockIdx.z;
+    if (i > d.iec) return;
+    const long long c = IX3(d, i, j, k);
+    const T dtIareaT = (T)(dt * IareaT[IX2(d, i, j)]);
+    h[c] = vmax(hin[c] - dtIareaT * (vh[c] - vh[c - d.n1]), T(0.0));
+}
+
 // continuity_zonal_convergence (:148-176) with hin; optionally copies hin into
 // the halo of h (the driver's `h = h0` reset folded into the predictor call).
 __global__ void k_convergence(Dims d, const double *hin, const double *__restrict__ uh, double dt,
@@ -806,6 +932,123 @@ __global__ void k_convergence(Dims d, const double *hin, const double *__restric
         h[c] = hin[c];
     }
 }
+
+template <typename T, bool Y>
+int launch_blk(const ContArgsT<T> &a, int mono, dim3 g, hipStream_t s)
+{
+    const int nk = a.d.nk;
+    const dim3 b(64 * 8);
+    if (nk <= 8) k_continuity_blk<T, Y, 8, 1><<<g, b, sizeof(ContShared<T, 8, 1>), s>>>(a, mono);
+    else if (nk <= 16) k_continuity_blk<T, Y, 8, 2><<<g, b, sizeof(ContShared<T, 8, 2>), s>>>(a, mono);
+    else if (nk <= 32) k_continuity_blk<T, Y, 8, 4><<<g, b, sizeof(ContShared<T, 8, 4>), s>>>(a, mono);
+    else k_continuity_blk<T, Y, 8, 10><<<g, b, sizeof(ContShared<T, 8, 10>), s>>>(a, mono);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
+}
+
+// Raises the dynamic LDS limit of every block-kernel instantiation of type T
+// that fits the device
+template <typename T>
+int set_blk_lds(long long max_lds)
+{
+    const void *kern[] = {(const void *)k_continuity_blk<T, false, 8, 1>, (const void *)k_continuity_blk<T, false, 8, 2>,
+                          (const void *)k_continuity_blk<T, false, 8, 4>, (const void *)k_continuity_blk<T, false, 8, 10>,
+                          (const void *)k_continuity_blk<T, true, 8, 1>, (const void *)k_continuity_blk<T, true, 8, 2>,
+                          (const void *)k_continuity_blk<T, true, 8, 4>, (const void *)k_continuity_blk<T, true, 8, 10>};
+    const int bytes[] = {(int)sizeof(ContShared<T, 8, 1>), (int)sizeof(ContShared<T, 8, 2>),
+                         (int)sizeof(ContShared<T, 8, 4>), (int)sizeof(ContShared<T, 8, 10>)};
+    for (int q = 0; q < 8; ++q)
+        if (bytes[q % 4] <= max_lds)
+            MOM6H_CHECK(hipFuncSetAttribute(kern[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q % 4]));
+    return MOM6H_OK;
+}
+
+template <typename T>
+void launch_halo_copy(const Dims &d, const T *hin, T *h, hipStream_t s)
+{
+    const long long cells = (long long)((d.jsc - d.jsd) + (d.jed - d.jec)) * d.n1 +
+                            (long long)((d.isc - d.isd) + (d.ied - d.iec)) * (d.jec - d.jsc + 1);
+    const dim3 bh(256), gh((unsigned)((cells + 255) / 256), d.nk);
+    k_halo_copy<T><<<gh, bh, 0, s>>>(d, hin, h);
+}
+
+bool blk_fits(const mom6h_continuity_cs *cs)
+{
+    return cs->d.nk <= 80 && !cs->upwind_1st && !cs->simple_2nd &&
+           (long long)sizeof(ContShared<double, 8, 10>) <= cs->max_lds;
+}
+
+// The zonal pass of the block kernel for either type
+template <typename T>
+int continuity_blk_x(mom6h_continuity_cs *cs, const T *u, const T *hin, T *h, T *uh, double dt, const double *uhbt,
+                     const mom6h_bt_cont *bt, T *bt_h_u, T *u_cor, double *du_cor, int hin_halo_to_h, hipStream_t s)
+{
+    const Dims &d = cs->d;
+    ContArgsT<T> a{d, u, hin, h, uh, dt, uhbt, cs->IareaT, cs->IdxT, cs->dy_Cu, cs->dxT, cs->dxCu,
+                   cs->mask2dCu, cs->mask2dT, cs->tol_eta, cs->tol_vel, cs->CFL_limit_adjust, 2.0 * 1.0e-10,
+                   u_cor, du_cor,
+                   bt ? bt->FA_u_EE : nullptr, bt ? bt->FA_u_E0 : nullptr, bt ? bt->FA_u_W0 : nullptr,
+                   bt ? bt->FA_u_WW : nullptr, bt ? bt->uBT_WW : nullptr, bt ? bt->uBT_EE : nullptr,
+                   bt ? bt_h_u : nullptr};
+    const dim3 g((d.iec - d.isc + 1 + CF - 2) / (CF - 1), d.jec - d.jsc + 1);
+    int rc = launch_blk<T, false>(a, cs->monotonic, g, s);
+    if (rc) return rc;
+    if (hin_halo_to_h) {
+        launch_halo_copy(d, hin, h, s);
+        MOM6H_LAUNCH_CHECK();
+    }
+    return MOM6H_OK;
+}
+
+// The meridional pass (faces and convergence) for either type
+template <typename T>
+int continuity_blk_y(mom6h_continuity_cs *cs, const T *v, const T *hin, T *h, T *vh, double dt, const double *vhbt,
+                     const mom6h_bt_cont *bt, T *bt_h_v, T *v_cor, double *dv_cor, int hin_halo_to_h, hipStream_t s)
+{
+    const Dims &d = cs->d;
+    ContArgsT<T> a{d, v, hin, h, vh, dt, vhbt, cs->IareaT, cs->IdyT, cs->dx_Cv, cs->dyT, cs->dyCv, cs->mask2dCv,
+                   cs->mask2dT, cs->tol_eta, cs->tol_vel, cs->CFL_limit_adjust, 2.0 * 1.0e-10, v_cor, dv_cor,
+                   bt ? bt->FA_u_EE : nullptr, bt ? bt->FA_u_E0 : nullptr, bt ? bt->FA_u_W0 : nullptr,
+                   bt ? bt->FA_u_WW : nullptr, bt ? bt->uBT_WW : nullptr, bt ? bt->uBT_EE : nullptr,
+                   bt ? bt_h_v : nullptr};
+    const dim3 g((d.iec - d.isc + 1 + CF - 1) / CF, d.jec - d.jsc + 2);
+    int rc = launch_blk<T, true>(a, cs->monotonic, g, s);
+    if (rc) return rc;
+    {
+        dim3 b(128), gc((d.iec - d.isc + 1 + 127) / 128, d.jec - d.jsc + 1, d.nk);
+        k_convergence_y<T><<<gc, b, 0, s>>>(d, hin, vh, dt, cs->IareaT, h);
+        MOM6H_LAUNCH_CHECK();
+    }
+    if (hin_halo_to_h) {
+        launch_halo_copy(d, hin, h, s);
+        MOM6H_LAUNCH_CHECK();
+    }
+    return MOM6H_OK;
+}
+
+int continuity_f32(mom6h_continuity_cs *cs, const float *u, const float *hin, float *h, float *uh, double dt,
+                   const mom6h_bt_cont *bt, float *bt_h_u, float *u_cor, double *du_cor, int hin_halo_to_h,
+                   hipStream_t s)
+{
+    MOM6H_REQUIRE(blk_fits(cs) && (long long)sizeof(ContShared<float, 8, 10>) <= cs->max_lds,
+                  "continuity (FP32 state): needs nk <= 80 and the PPM reconstruction");
+    return continuity_blk_x<float>(cs, u, hin, h, uh, dt, nullptr, bt, bt_h_u, u_cor, du_cor, hin_halo_to_h, s);
+}
+
+int continuity_y_f32(mom6h_continuity_cs *cs, const float *v, const float *hin, float *h, float *vh, double dt,
+                     const mom6h_bt_cont *bt, float *bt_h_v, float *v_cor, double *dv_cor, int hin_halo_to_h,
+                     hipStream_t s)
+{
+    MOM6H_REQUIRE(cs->IdyT != nullptr, "continuity (FP32 state): meridional metrics not set");
+    MOM6H_REQUIRE(blk_fits(cs), "continuity (FP32 state): needs nk <= 80 and the PPM reconstruction");
+    return continuity_blk_y<float>(cs, v, hin, h, vh, dt, nullptr, bt, bt_h_v, v_cor, dv_cor, hin_halo_to_h, s);
+}
+
+#define RC_(expr)              \
+    do {                       \
+        int rc_ = (expr);      \
+        if (rc_) return rc_;   \
+    } while (0)
 
 }  // namespace mom6h
 
@@ -840,13 +1083,12 @@ int mom6h_continuity_init(mom6h_continuity_cs **out, int isd, int ied, int jsd, 
         MOM6H_CHECK(hipGetDevice(&dev));
         MOM6H_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         cs->max_lds = lds;
-        const void *kern[] = {(const void *)k_continuity_blk<8, 1>, (const void *)k_continuity_blk<8, 2>,
-                              (const void *)k_continuity_blk<8, 4>, (const void *)k_continuity_blk<8, 10>};
-        const int bytes[] = {(int)sizeof(ContShared<8, 1>), (int)sizeof(ContShared<8, 2>),
-                             (int)sizeof(ContShared<8, 4>), (int)sizeof(ContShared<8, 10>)};
-        for (int q = 0; q < 4; ++q)
-            if (bytes[q] <= lds)
-                MOM6H_CHECK(hipFuncSetAttribute(kern[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q]));
+        int rc0 = set_blk_lds<double>(lds);
+        rc0 = rc0 ? rc0 : set_blk_lds<float>(lds);
+        if (rc0) {
+            delete cs;
+            return rc0;
+        }
     }
     const size_t n2 = (size_t)cs->d.s2, n3 = n2 * nk;
     int rc = 0;
@@ -879,29 +1121,9 @@ int mom6h_continuity_ppm(mom6h_continuity_cs *cs, const double *u, const double 
                   "mom6h_continuity_ppm: incomplete BT_cont");
     const Dims &d = cs->d;
     hipStream_t s = (hipStream_t)stream;
-    if (d.nk <= 80 && !cs->upwind_1st && !cs->simple_2nd && (long long)sizeof(ContShared<8, 10>) <= cs->max_lds) {
-        ContArgs a{d, u, hin, h, uh, dt, uhbt, cs->IareaT, cs->IdxT, cs->dy_Cu, cs->dxT, cs->dxCu,
-                   cs->mask2dCu, cs->mask2dT, cs->tol_eta, cs->tol_vel, cs->CFL_limit_adjust, 2.0 * 1.0e-10,
-                   u_cor, du_cor,
-                   bt ? bt->FA_u_EE : nullptr, bt ? bt->FA_u_E0 : nullptr, bt ? bt->FA_u_W0 : nullptr,
-                   bt ? bt->FA_u_WW : nullptr, bt ? bt->uBT_WW : nullptr, bt ? bt->uBT_EE : nullptr,
-                   bt ? bt->h_u : nullptr};
-        const dim3 g((d.iec - d.isc + 1 + CF - 2) / (CF - 1), d.jec - d.jsc + 1);
-        const int mono = cs->monotonic;
-        if (d.nk <= 8) k_continuity_blk<8, 1><<<g, dim3(64 * 8), sizeof(ContShared<8, 1>), s>>>(a, mono);
-        else if (d.nk <= 16) k_continuity_blk<8, 2><<<g, dim3(64 * 8), sizeof(ContShared<8, 2>), s>>>(a, mono);
-        else if (d.nk <= 32) k_continuity_blk<8, 4><<<g, dim3(64 * 8), sizeof(ContShared<8, 4>), s>>>(a, mono);
-        else k_continuity_blk<8, 10><<<g, dim3(64 * 8), sizeof(ContShared<8, 10>), s>>>(a, mono);
-        MOM6H_LAUNCH_CHECK();
-        if (hin_halo_to_h) {
-            const long long cells = (long long)((d.jsc - d.jsd) + (d.jed - d.jec)) * d.n1 +
-                                    (long long)((d.isc - d.isd) + (d.ied - d.iec)) * (d.jec - d.jsc + 1);
-            const dim3 bh(256), gh((unsigned)((cells + 255) / 256), d.nk);
-            k_halo_copy<<<gh, bh, 0, s>>>(d, hin, h);
-            MOM6H_LAUNCH_CHECK();
-        }
-        return MOM6H_OK;
-    }
+    if (blk_fits(cs))
+        return continuity_blk_x<double>(cs, u, hin, h, uh, dt, uhbt, bt, bt ? bt->h_u : nullptr, u_cor, du_cor,
+                                        hin_halo_to_h, s);
     {
         const int ni = (d.iec + 1) - (d.isc - 1) + 1;
         dim3 b(64), g((ni + 63) / 64, d.jec - d.jsc + 1, d.nk);
@@ -929,11 +1151,48 @@ int mom6h_continuity_ppm(mom6h_continuity_cs *cs, const double *u, const double 
     return MOM6H_OK;
 }
 
+int mom6h_continuity_set_meridional(mom6h_continuity_cs *cs, const double *IdyT, const double *dx_Cv,
+                                    const double *dyT, const double *dyCv, const double *mask2dCv)
+{
+    MOM6H_REQUIRE(cs != nullptr, "mom6h_continuity_set_meridional: NULL cs");
+    MOM6H_REQUIRE(IdyT && dx_Cv && dyT && dyCv && mask2dCv, "mom6h_continuity_set_meridional: NULL metric");
+    const Dims &d = cs->d;
+    MOM6H_REQUIRE(d.jsd < d.jsc - 2 && d.jec + 2 < d.jed,
+                  "mom6h_continuity_set_meridional: PPM needs 3 halo rows (jsd..jsc-3, jec+3..jed)");
+    const size_t n2 = (size_t)d.s2;
+    for (double *p : {cs->IdyT, cs->dx_Cv, cs->dyT, cs->dyCv, cs->mask2dCv}) dev_free(p);
+    cs->IdyT = cs->dx_Cv = cs->dyT = cs->dyCv = cs->mask2dCv = nullptr;
+    int rc = 0;
+    rc = rc ? rc : dev_upload(&cs->IdyT, IdyT, n2);
+    rc = rc ? rc : dev_upload(&cs->dx_Cv, dx_Cv, n2);
+    rc = rc ? rc : dev_upload(&cs->dyT, dyT, n2);
+    rc = rc ? rc : dev_upload(&cs->dyCv, dyCv, n2);
+    rc = rc ? rc : dev_upload(&cs->mask2dCv, mask2dCv, n2);
+    return rc;
+}
+
+int mom6h_continuity_ppm_y(mom6h_continuity_cs *cs, const double *v, const double *hin, double *h,
+                           double *vh, double dt, const double *vhbt, const mom6h_bt_cont *bt,
+                           double *v_cor, double *dv_cor, int hin_halo_to_h, void *stream)
+{
+    MOM6H_REQUIRE(cs != nullptr, "mom6h_continuity_ppm_y: NULL cs");
+    MOM6H_REQUIRE(cs->IdyT != nullptr, "mom6h_continuity_ppm_y: call mom6h_continuity_set_meridional first");
+    MOM6H_REQUIRE(v && hin && h && vh, "mom6h_continuity_ppm_y: NULL array");
+    MOM6H_REQUIRE(dt > 0.0, "mom6h_continuity_ppm_y: dt must be positive");
+    MOM6H_REQUIRE(!bt || (bt->FA_u_EE && bt->FA_u_E0 && bt->FA_u_W0 && bt->FA_u_WW && bt->uBT_WW &&
+                          bt->uBT_EE && bt->h_u),
+                  "mom6h_continuity_ppm_y: incomplete BT_cont");
+    MOM6H_REQUIRE(blk_fits(cs),
+                  "mom6h_continuity_ppm_y: the meridional pass supports nk <= 80 with the PPM reconstruction");
+    return continuity_blk_y<double>(cs, v, hin, h, vh, dt, vhbt, bt, bt ? bt->h_u : nullptr, v_cor, dv_cor,
+                                    hin_halo_to_h, (hipStream_t)stream);
+}
+
 int mom6h_continuity_end(mom6h_continuity_cs *cs)
 {
     if (!cs) return MOM6H_OK;
     for (double *p : {cs->IareaT, cs->IdxT, cs->dy_Cu, cs->mask2dT, cs->dxT, cs->dxCu,
-                      cs->mask2dCu, cs->hW, cs->hE})
+                      cs->mask2dCu, cs->hW, cs->hE, cs->IdyT, cs->dx_Cv, cs->dyT, cs->dyCv, cs->mask2dCv})
         dev_free(p);
     delete cs;
     return MOM6H_OK;

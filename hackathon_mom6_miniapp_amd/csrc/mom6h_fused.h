@@ -12,20 +12,49 @@ struct UMetrics {
     double areaT, IareaT, dxT, IdxT, dyCu, dy_Cu, dxCu, IdxCu, IdyCu, dxCv, dyCv, IdxCv, IdyCv;
     double IareaBu, IareaCu, IareaCv, Area_q;
     double DY_dxT, DX_dyT, DY_dxBu, DX_dyBu, dx2h, dy2h, dx2q, dy2q;
+    double Idx2dyCu, Idxdy2u, Idx2dyCv, Idxdy2v;   // biharmonic (hor_visc_init :262-283)
 };
 
-struct MomArgs {
+// The metric values in the stencil's arithmetic type (T = double: unchanged)
+template <typename T>
+struct UMetT {
+    T areaT, IareaT, dxT, IdxT, dyCu, dy_Cu, dxCu, IdxCu, IdyCu, dxCv, dyCv, IdxCv, IdyCv;
+    T IareaBu, IareaCu, IareaCv, Area_q;
+    T DY_dxT, DX_dyT, DY_dxBu, DX_dyBu, dx2h, dy2h, dx2q, dy2q;
+    T Idx2dyCu, Idxdy2u, Idx2dyCv, Idxdy2v;
+    __host__ __device__ explicit UMetT(const UMetrics &g)
+        : areaT((T)g.areaT), IareaT((T)g.IareaT), dxT((T)g.dxT), IdxT((T)g.IdxT), dyCu((T)g.dyCu),
+          dy_Cu((T)g.dy_Cu), dxCu((T)g.dxCu), IdxCu((T)g.IdxCu), IdyCu((T)g.IdyCu), dxCv((T)g.dxCv),
+          dyCv((T)g.dyCv), IdxCv((T)g.IdxCv), IdyCv((T)g.IdyCv), IareaBu((T)g.IareaBu), IareaCu((T)g.IareaCu),
+          IareaCv((T)g.IareaCv), Area_q((T)g.Area_q), DY_dxT((T)g.DY_dxT), DX_dyT((T)g.DX_dyT),
+          DY_dxBu((T)g.DY_dxBu), DX_dyBu((T)g.DX_dyBu), dx2h((T)g.dx2h), dy2h((T)g.dy2h), dx2q((T)g.dx2q),
+          dy2q((T)g.dy2q), Idx2dyCu((T)g.Idx2dyCu), Idxdy2u((T)g.Idxdy2u), Idx2dyCv((T)g.Idx2dyCv),
+          Idxdy2v((T)g.Idxdy2v)
+    {
+    }
+};
+
+// Arguments of the fused momentum stencil.  T is the type of the 3D layer
+// fields and of the stencil arithmetic (double: the reference's FP64; float:
+// the mixed-precision form); 2D masks and Coriolis stay FP64 in memory.
+template <typename T>
+struct MomArgsT {
     Dims d;
     UMetrics g;
-    double Kh, h_neglect, fdt;
+    T Kh, h_neglect, fdt;
     int no_slip;
-    const double *u, *v, *h;          // stencil inputs
-    const double *bu, *bv;            // bases of the update (may alias ou/ov)
-    double *ou, *ov;
+    const T *u, *v, *h;               // stencil inputs
+    const T *bu, *bv;                 // bases of the update (may alias ou/ov)
+    T *ou, *ov;
     const double *mask2dT, *mask2dBu, *CoriolisBu;
+    // biharmonic viscosity: Ah and the Del2 scratch (NULL: Laplacian only)
+    T Ah;
+    T *del2u, *del2v;
 };
+using MomArgs = MomArgsT<double>;
 
-int launch_momentum(const MomArgs &a, hipStream_t s);
+template <typename T>
+int launch_momentum(const MomArgsT<T> &a, hipStream_t s);
 
 // true and the common value when every element of a[0..n) is identical
 inline bool uniform_value(const double *a, size_t n, double *v)

@@ -25,12 +25,52 @@ namespace mom6h {
 constexpr int MT_X = 64, MT_Y = 8;             // output tile
 constexpr int MW = MT_X + 2, MH = MT_Y + 2;    // staged region [i0-1, i1+1] x [j0-1, j1+1]
 
-__global__ void __launch_bounds__(256) k_momentum(MomArgs a)
+// Biharmonic viscosity, first half: Del2u on u points I = is-1..ie,
+// j = js..je and Del2v on v points i = is..ie, J = js-1..je
+// (mom6_hor_visc_serial.F90:587-607, uniform metrics).  The arrays are zero
+// at allocation and never written outside those ranges, which is what the
+// reference's second half reads there.
+template <typename T>
+__global__ void __launch_bounds__(256) k_del2(MomArgsT<T> a)
 {
-    __shared__ double s_u[MH][MW], s_v[MH][MW], s_h[MH][MW];
-    __shared__ double s_q[MH][MW], s_KE[MH][MW];
     const Dims &d = a.d;
-    const UMetrics &g = a.g;
+    const UMetT<T> g(a.g);
+    const int i = d.isc - 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = d.jsc - 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = 1 + blockIdx.z;
+    if (i > d.iec || j > d.jec) return;
+    auto U = [&](int ii, int jj) -> T { return a.u[IX3(d, ii, jj, k)]; };
+    auto V = [&](int ii, int jj) -> T { return a.v[IX3(d, ii, jj, k)]; };
+    auto sh_xx = [&](int ii, int jj) -> T {   // h point (:503-510)
+        const T dudx = g.DY_dxT * ((g.IdyCu * U(ii, jj)) - (g.IdyCu * U(ii - 1, jj)));
+        const T dvdy = g.DX_dyT * ((g.IdxCv * V(ii, jj)) - (g.IdxCv * V(ii, jj - 1)));
+        return dudx - dvdy;
+    };
+    auto sh_xy = [&](int I, int J) -> T {    // q point (:513-547)
+        const T dvdx = g.DY_dxBu * ((V(I + 1, J) * g.IdyCv) - (V(I, J) * g.IdyCv));
+        const T dudy = g.DX_dyBu * ((U(I, J + 1) * g.IdxCu) - (U(I, J) * g.IdxCu));
+        const T m = (T)a.mask2dBu[IX2(d, I, J)];
+        return (a.no_slip ? (T(2.0) - m) : m) * (dvdx + dudy);
+    };
+    const long long c = IX3(d, i, j, k);
+    if (j >= d.jsc)
+        a.del2u[c] = g.Idx2dyCu * ((g.dx2q * sh_xy(i, j)) - (g.dx2q * sh_xy(i, j - 1))) +
+                     g.Idxdy2u * ((g.dy2h * sh_xx(i + 1, j)) - (g.dy2h * sh_xx(i, j)));
+    if (i >= d.isc)
+        a.del2v[c] = g.Idxdy2v * ((g.dy2q * sh_xy(i, j)) - (g.dy2q * sh_xy(i - 1, j))) -
+                     g.Idx2dyCv * ((g.dx2h * sh_xx(i, j + 1)) - (g.dx2h * sh_xx(i, j)));
+}
+
+// BIH: the biharmonic stress of the Del2 fields staged like u and v, added
+// to the Laplacian stress (:824-886, :979-1036).
+template <typename T, bool BIH>
+__global__ void __launch_bounds__(256) k_momentum(MomArgsT<T> a)
+{
+    __shared__ T s_u[MH][MW], s_v[MH][MW], s_h[MH][MW];
+    __shared__ T s_q[MH][MW], s_KE[MH][MW];
+    __shared__ T s_d2u[BIH ? MH : 1][BIH ? MW : 1], s_d2v[BIH ? MH : 1][BIH ? MW : 1];
+    const Dims &d = a.d;
+    const UMetT<T> g(a.g);
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
     const int i0 = is + (tile % gridDim.x) * MT_X, j0 = js + (tile / gridDim.x) * MT_Y;
@@ -43,22 +83,25 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgs a)
     // layer is computed (NS points per thread), and so are the base velocities
     // of the thread's two output rows
     constexpr int NS = (MW * MH + 255) / 256;
-    double pu[NS], pv[NS], ph[NS], pbu[2], pbv[2];
+    T pu[NS], pv[NS], ph[NS], pbu[2], pbv[2];
+    T p2u[BIH ? NS : 1], p2v[BIH ? NS : 1];
     auto load_layer = [&](int k) {
 #pragma unroll
         for (int r = 0; r < NS; ++r) {
             const int t = tid + 256 * r;
             const int li = t % MW, lj = t / MW, i = bi + li, j = bj + lj;
-            pu[r] = 0.0; pv[r] = 0.0; ph[r] = 0.0;
+            pu[r] = T(0.0); pv[r] = T(0.0); ph[r] = T(0.0);
+            if (BIH) { p2u[BIH ? r : 0] = T(0.0); p2v[BIH ? r : 0] = T(0.0); }
             if (t < MW * MH && i <= d.ied && j <= d.jed) {
                 const long long c = IX3(d, i, j, k);
                 pu[r] = a.u[c]; pv[r] = a.v[c]; ph[r] = a.h[c];
+                if (BIH) { p2u[BIH ? r : 0] = a.del2u[c]; p2v[BIH ? r : 0] = a.del2v[c]; }
             }
         }
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             const int i = i0 + tx, j = j0 + ty + 4 * r;
-            pbu[r] = 0.0; pbv[r] = 0.0;
+            pbu[r] = T(0.0); pbv[r] = T(0.0);
             if (i <= ie && j <= je) {
                 const long long c = IX3(d, i, j, k);
                 pbu[r] = a.bu[c]; pbv[r] = a.bv[c];
@@ -73,9 +116,10 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgs a)
             if (t < MW * MH) {
                 const int li = t % MW, lj = t / MW;
                 s_u[lj][li] = pu[r]; s_v[lj][li] = pv[r]; s_h[lj][li] = ph[r];
+                if (BIH) { s_d2u[BIH ? lj : 0][BIH ? li : 0] = p2u[BIH ? r : 0]; s_d2v[BIH ? lj : 0][BIH ? li : 0] = p2v[BIH ? r : 0]; }
             }
         }
-        const double bu_r[2] = {pbu[0], pbu[1]}, bv_r[2] = {pbv[0], pbv[1]};
+        const T bu_r[2] = {pbu[0], pbu[1]}, bv_r[2] = {pbv[0], pbv[1]};
         __syncthreads();
         if (k < d.nk) load_layer(k + 1);
         // q on q points [i0-1, i1] x [j0-1, j1] (mom6_coriolis_serial.F90:134-168)
@@ -84,20 +128,20 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgs a)
             const int li = t % MW, lj = t / MW, i = bi + li, j = bj + lj;
             if (li < MW - 1 && lj < MH - 1 && i <= ie && j <= je) {
                 const long long c2 = IX2(d, i, j);
-                const double dvdx = (s_v[lj][li + 1] * g.dyCv) - (s_v[lj][li] * g.dyCv);
-                const double dudy = (s_u[lj + 1][li] * g.dxCu) - (s_u[lj][li] * g.dxCu);
-                const double hAu0 = (j < js) ? 0.0 : 0.5 * ((g.areaT * s_h[lj][li]) + (g.areaT * s_h[lj][li + 1]));
-                const double hAu1 = 0.5 * ((g.areaT * s_h[lj + 1][li]) + (g.areaT * s_h[lj + 1][li + 1]));
-                const double hAv0 = (i < is) ? 0.0 : 0.5 * ((g.areaT * s_h[lj][li]) + (g.areaT * s_h[lj + 1][li]));
-                const double hAv1 = 0.5 * ((g.areaT * s_h[lj][li + 1]) + (g.areaT * s_h[lj + 1][li + 1]));
-                const double rel_vort = a.mask2dBu[c2] * (dvdx - dudy) * g.IareaBu;
-                const double abs_vort = a.CoriolisBu[c2] + rel_vort;
-                const double hArea_q = (hAu0 + hAu1) + (hAv0 + hAv1);
-                s_q[lj][li] = abs_vort * (g.Area_q / (hArea_q + 1.0e-20));
+                const T dvdx = (s_v[lj][li + 1] * g.dyCv) - (s_v[lj][li] * g.dyCv);
+                const T dudy = (s_u[lj + 1][li] * g.dxCu) - (s_u[lj][li] * g.dxCu);
+                const T hAu0 = (j < js) ? T(0.0) : T(0.5) * ((g.areaT * s_h[lj][li]) + (g.areaT * s_h[lj][li + 1]));
+                const T hAu1 = T(0.5) * ((g.areaT * s_h[lj + 1][li]) + (g.areaT * s_h[lj + 1][li + 1]));
+                const T hAv0 = (i < is) ? T(0.0) : T(0.5) * ((g.areaT * s_h[lj][li]) + (g.areaT * s_h[lj + 1][li]));
+                const T hAv1 = T(0.5) * ((g.areaT * s_h[lj][li + 1]) + (g.areaT * s_h[lj + 1][li + 1]));
+                const T rel_vort = (T)a.mask2dBu[c2] * (dvdx - dudy) * g.IareaBu;
+                const T abs_vort = (T)a.CoriolisBu[c2] + rel_vort;
+                const T hArea_q = (hAu0 + hAu1) + (hAv0 + hAv1);
+                s_q[lj][li] = abs_vort * (g.Area_q / (hArea_q + T(1.0e-20)));
             }
             if (li >= 1 && lj >= 1 && i <= ie && j <= je) {
-                const double uc = s_u[lj][li], uw = s_u[lj][li - 1], vc = s_v[lj][li], vs = s_v[lj - 1][li];
-                s_KE[lj][li] = 0.25 * ((g.dyCu * (uc * uc) + g.dyCu * (uw * uw)) +
+                const T uc = s_u[lj][li], uw = s_u[lj][li - 1], vc = s_v[lj][li], vs = s_v[lj - 1][li];
+                s_KE[lj][li] = T(0.25) * ((g.dyCu * (uc * uc) + g.dyCu * (uw * uw)) +
                                        (g.dxCv * (vc * vc) + g.dxCv * (vs * vs))) / g.areaT;
             }
         }
@@ -109,43 +153,55 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgs a)
             const int li = i - bi, lj = j - bj;
             const long long c = IX3(d, i, j, k);
             // Laplacian stress pieces (mom6_hor_visc_serial.F90:503-547, 807-1053)
-            auto str_xx = [&](int pi, int pj) -> double {   // LDS coords of an h point
-                const double dudx = g.DY_dxT * ((g.IdyCu * s_u[pj][pi]) - (g.IdyCu * s_u[pj][pi - 1]));
-                const double dvdy = g.DX_dyT * ((g.IdxCv * s_v[pj][pi]) - (g.IdxCv * s_v[pj - 1][pi]));
-                return -(a.Kh * (dudx - dvdy)) * (s_h[pj][pi] * 1.0);
+            auto str_xx = [&](int pi, int pj) -> T {   // LDS coords of an h point
+                const T dudx = g.DY_dxT * ((g.IdyCu * s_u[pj][pi]) - (g.IdyCu * s_u[pj][pi - 1]));
+                const T dvdy = g.DX_dyT * ((g.IdxCv * s_v[pj][pi]) - (g.IdxCv * s_v[pj - 1][pi]));
+                T str = -(a.Kh * (dudx - dvdy));
+                if (BIH) {
+                    const int qi = BIH ? pi : 0, qj = BIH ? pj : 0;
+                    const T d_del2u = (g.IdyCu * s_d2u[qj][qi]) - (g.IdyCu * s_d2u[qj][BIH ? qi - 1 : 0]);
+                    const T d_del2v = (g.IdxCv * s_d2v[qj][qi]) - (g.IdxCv * s_d2v[BIH ? qj - 1 : 0][qi]);
+                    str = str + a.Ah * ((g.DY_dxT * d_del2u) - (g.DX_dyT * d_del2v));
+                }
+                return str * (s_h[pj][pi] * T(1.0));
             };
-            auto str_xy = [&](int pi, int pj, int I, int J) -> double {   // q point
-                const double dvdx = g.DY_dxBu * ((s_v[pj][pi + 1] * g.IdyCv) - (s_v[pj][pi] * g.IdyCv));
-                const double dudy = g.DX_dyBu * ((s_u[pj + 1][pi] * g.IdxCu) - (s_u[pj][pi] * g.IdxCu));
-                const double m = a.mask2dBu[IX2(d, I, J)];
-                const double sh = (a.no_slip ? (2.0 - m) : m) * (dvdx + dudy);
-                const double hq = 0.25 * ((s_h[pj][pi] + s_h[pj + 1][pi + 1]) + (s_h[pj][pi + 1] + s_h[pj + 1][pi]));
-                const double str = -(a.Kh * sh);
-                return a.no_slip ? str * (hq * 1.0) : str * (hq * m * 1.0);
+            auto str_xy = [&](int pi, int pj, int I, int J) -> T {   // q point
+                const T dvdx = g.DY_dxBu * ((s_v[pj][pi + 1] * g.IdyCv) - (s_v[pj][pi] * g.IdyCv));
+                const T dudy = g.DX_dyBu * ((s_u[pj + 1][pi] * g.IdxCu) - (s_u[pj][pi] * g.IdxCu));
+                const T m = (T)a.mask2dBu[IX2(d, I, J)];
+                const T sh = (a.no_slip ? (T(2.0) - m) : m) * (dvdx + dudy);
+                const T hq = T(0.25) * ((s_h[pj][pi] + s_h[pj + 1][pi + 1]) + (s_h[pj][pi + 1] + s_h[pj + 1][pi]));
+                T str = -(a.Kh * sh);
+                if (BIH) {
+                    const int qi = BIH ? pi : 0, qj = BIH ? pj : 0;
+                    str = str + a.Ah * ((g.DY_dxBu * ((s_d2v[qj][BIH ? qi + 1 : 0] * g.IdyCv) - (s_d2v[qj][qi] * g.IdyCv))) +
+                                        (g.DX_dyBu * ((s_d2u[BIH ? qj + 1 : 0][qi] * g.IdxCu) - (s_d2u[qj][qi] * g.IdxCu))));
+                }
+                return a.no_slip ? str * (hq * T(1.0)) : str * (hq * m * T(1.0));
             };
             if (i <= ie - 1) {
                 // diffu (:1059-1067)
-                const double h_u = 0.5 * (a.mask2dT[IX2(d, i, j)] * s_h[lj][li] + a.mask2dT[IX2(d, i + 1, j)] * s_h[lj][li + 1]);
-                const double diffu = ((g.IdxCu * ((g.dx2q * str_xy(li, lj - 1, i, j - 1)) - (g.dx2q * str_xy(li, lj, i, j))) +
+                const T h_u = T(0.5) * ((T)a.mask2dT[IX2(d, i, j)] * s_h[lj][li] + (T)a.mask2dT[IX2(d, i + 1, j)] * s_h[lj][li + 1]);
+                const T diffu = ((g.IdxCu * ((g.dx2q * str_xy(li, lj - 1, i, j - 1)) - (g.dx2q * str_xy(li, lj, i, j))) +
                                        g.IdyCu * ((g.dy2h * str_xx(li, lj)) - (g.dy2h * str_xx(li + 1, lj)))) *
                                       g.IareaCu) / (h_u + a.h_neglect);
                 // CAu, Sadourny (:208-216); vh from compute_transports
-                auto vh = [&](int pi, int pj) { return s_v[pj][pi] * 0.5 * (s_h[pj][pi] + s_h[pj + 1][pi]) * g.dxCv; };
-                const double KEx = (s_KE[lj][li + 1] - s_KE[lj][li]) * g.IdxCu;
-                const double CAu = 0.25 * ((s_q[lj][li] * (vh(li + 1, lj) + vh(li, lj))) +
+                auto vh = [&](int pi, int pj) { return s_v[pj][pi] * T(0.5) * (s_h[pj][pi] + s_h[pj + 1][pi]) * g.dxCv; };
+                const T KEx = (s_KE[lj][li + 1] - s_KE[lj][li]) * g.IdxCu;
+                const T CAu = T(0.25) * ((s_q[lj][li] * (vh(li + 1, lj) + vh(li, lj))) +
                                            (s_q[lj - 1][li] * (vh(li, lj - 1) + vh(li + 1, lj - 1)))) * g.IdxCu - KEx;
                 a.ou[c] = bu_r[r] + a.fdt * (CAu + diffu);
             }
             if (j <= je - 1) {
                 // diffv (:1070-1078)
-                const double h_v = 0.5 * (a.mask2dT[IX2(d, i, j)] * s_h[lj][li] + a.mask2dT[IX2(d, i, j + 1)] * s_h[lj + 1][li]);
-                const double diffv = ((g.IdyCv * ((g.dy2q * str_xy(li - 1, lj, i - 1, j)) - (g.dy2q * str_xy(li, lj, i, j))) -
+                const T h_v = T(0.5) * ((T)a.mask2dT[IX2(d, i, j)] * s_h[lj][li] + (T)a.mask2dT[IX2(d, i, j + 1)] * s_h[lj + 1][li]);
+                const T diffv = ((g.IdyCv * ((g.dy2q * str_xy(li - 1, lj, i - 1, j)) - (g.dy2q * str_xy(li, lj, i, j))) -
                                        g.IdxCv * ((g.dx2h * str_xx(li, lj)) - (g.dx2h * str_xx(li, lj + 1)))) *
                                       g.IareaCv) / (h_v + a.h_neglect);
                 // CAv, Sadourny (:218-226); uh from compute_transports
-                auto uh = [&](int pi, int pj) { return s_u[pj][pi] * 0.5 * (s_h[pj][pi] + s_h[pj][pi + 1]) * g.dyCu; };
-                const double KEy = (s_KE[lj + 1][li] - s_KE[lj][li]) * g.IdyCv;
-                const double CAv = -(0.25 * ((s_q[lj][li - 1] * (uh(li - 1, lj) + uh(li - 1, lj + 1))) +
+                auto uh = [&](int pi, int pj) { return s_u[pj][pi] * T(0.5) * (s_h[pj][pi] + s_h[pj][pi + 1]) * g.dyCu; };
+                const T KEy = (s_KE[lj + 1][li] - s_KE[lj][li]) * g.IdyCv;
+                const T CAv = -(T(0.25) * ((s_q[lj][li - 1] * (uh(li - 1, lj) + uh(li - 1, lj + 1))) +
                                              (s_q[lj][li] * (uh(li, lj) + uh(li, lj + 1)))) * g.IdyCv) - KEy;
                 a.ov[c] = bv_r[r] + a.fdt * (CAv + diffv);
             }
@@ -154,13 +210,23 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgs a)
     }
 }
 
-int launch_momentum(const MomArgs &a, hipStream_t s)
+template <typename T>
+int launch_momentum(const MomArgsT<T> &a, hipStream_t s)
 {
     const Dims &d = a.d;
+    if (a.del2u) {
+        dim3 g2((d.iec - d.isc + 2 + 63) / 64, (d.jec - d.jsc + 2 + 3) / 4, d.nk);
+        k_del2<T><<<g2, dim3(256), 0, s>>>(a);
+        MOM6H_LAUNCH_CHECK();
+    }
     dim3 grid((d.iec - d.isc + 1 + MT_X - 1) / MT_X, (d.jec - d.jsc + 1 + MT_Y - 1) / MT_Y), block(256);
-    k_momentum<<<grid, block, 0, s>>>(a);
+    if (a.del2u) k_momentum<T, true><<<grid, block, 0, s>>>(a);
+    else k_momentum<T, false><<<grid, block, 0, s>>>(a);
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
 }
+
+template int launch_momentum<double>(const MomArgsT<double> &, hipStream_t);
+template int launch_momentum<float>(const MomArgsT<float> &, hipStream_t);
 
 }  // namespace mom6h

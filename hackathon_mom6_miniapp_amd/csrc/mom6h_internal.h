@@ -85,6 +85,32 @@ void dev_free(double *p);
 
 // Halo fill of nk layers (nk = 1 for 2D) on a stream; no-op when not periodic.
 int halo_fill(const Dims &d, const Halo &hm, double *a, int nk, hipStream_t s);
+int halo_fill(const Dims &d, const Halo &hm, float *a, int nk, hipStream_t s);
+
+// Element-wise FP64 <-> FP32 copies on a stream (round to nearest / exact)
+int narrow_to_f32(const double *src, float *dst, long long n, hipStream_t s);
+int widen_to_f64(const float *src, double *dst, long long n, hipStream_t s);
+int dev_alloc_bytes(void **p, size_t bytes, bool zero);
+
+}  // namespace mom6h
+
+struct mom6h_vert_visc_cs;
+struct mom6h_continuity_cs;
+
+namespace mom6h {
+
+// The mixed-precision forms (FP32 layer fields) of the column physics.
+int vert_visc_f32(mom6h_vert_visc_cs *cs, float *u, float *v, const float *h, double dt, const double *taux,
+                  const double *tauy, const double *ray_u_bot, const double *ray_v_bot, hipStream_t stream);
+// continuity_PPM on FP32 thickness and velocity: the zonal pass (with the
+// convergence) and the meridional pass; BT_cont 2D fields and du_cor FP64,
+// the marginal thickness bt_h_u FP32.
+int continuity_f32(mom6h_continuity_cs *cs, const float *u, const float *hin, float *h, float *uh, double dt,
+                   const mom6h_bt_cont *bt, float *bt_h_u, float *u_cor, double *du_cor, int hin_halo_to_h,
+                   hipStream_t s);
+int continuity_y_f32(mom6h_continuity_cs *cs, const float *v, const float *hin, float *h, float *vh, double dt,
+                     const mom6h_bt_cont *bt, float *bt_h_v, float *v_cor, double *dv_cor, int hin_halo_to_h,
+                     hipStream_t s);
 
 }  // namespace mom6h
 
@@ -97,6 +123,7 @@ struct mom6h_continuity_cs {
     double tol_eta, tol_vel, CFL_limit_adjust;
     double *IareaT, *IdxT, *dy_Cu, *mask2dT, *dxT, *dxCu, *mask2dCu;
     double *hW, *hE;         // PPM edge values (3D scratch of the module kernels)
+    double *IdyT, *dx_Cv, *dyT, *dyCv, *mask2dCv;   // meridional pass (NULL until set)
     long long max_lds;       // device limit on LDS per block
 };
 

@@ -35,8 +35,10 @@ const double kDepth = 4000.0;     // mom6_barotropic_serial.F90:76
 
 struct Field {
     std::string name;
-    double **p;
+    void **p;
     size_t n;
+    int bytes;    // 8: FP64, 4: FP32 (the layer fields of the mixed-precision form)
+    bool owned;   // freed with the field list (BT_cont 2D arrays are freed with their struct)
 };
 
 enum Stage {
@@ -54,6 +56,7 @@ struct mom6h_model {
     Halo halo;
     bool prepared = false;
     int use_graph = 0, profile = 0;
+    int serialize = 0;       // barotropic chain on the 3D stream (per-kernel timing without overlap)
     int fused = 1;           // fused momentum stencil (module-by-module when 0)
     bool uniform = false;    // every metric array constant (fused kernels allowed)
     UMetrics ug{};
@@ -62,8 +65,19 @@ struct mom6h_model {
     // state and driver arrays
     double *u, *v, *h, *h0, *htmp, *uh, *vh, *CAu, *CAv, *up, *vp, *diffu, *diffv;
     double *eta, *ubt, *vbt, *ubt_av, *vbt_av, *eta_av, *u_cor, *du_cor;
+    double *hx = nullptr, *v_cor = nullptr, *dv_cor = nullptr;   // meridional pass
+    mom6h_bt_cont bt_v{};                                          // its v-side BT_cont
     double *taux, *tauy, *ray_u, *ray_v, *CoriolisBu, *mask2dT, *mask2dCu, *mask2dCv, *mask2dBu;
     mom6h_bt_cont bt{};
+    // mixed-precision form: FP32 layer fields (the FP64 pointers above that
+    // name them stay NULL); 2D fields, BT_cont 2D and du_cor stay FP64
+    bool mixed = false;
+    struct {
+        float *u, *v, *h, *h0, *htmp, *uh, *vh, *up, *vp, *u_cor, *bt_h_u;
+        float *hx, *v_cor, *bt_h_v;   // meridional pass
+        float *del2u, *del2v;         // biharmonic scratch of the fused stencil
+    } f{};
+    double *del2u = nullptr, *del2v = nullptr;   // biharmonic scratch of the fused stencil (FP64)
     std::vector<Field> fields;
     // host copies of the uniform metrics (init_ocean_grid)
     std::vector<double> IareaT, areaT, dxT, dyT, IdxT, IdyT, dxCu, dyCu, dy_Cu, IdxCu, IdyCu;
@@ -74,6 +88,10 @@ struct mom6h_model {
     mom6h_barotropic_cs *bt_cs = nullptr;
     mom6h_vert_visc_cs *vv = nullptr;
     mom6h_hor_visc_cs *hv = nullptr;
+    // one step captured as a graph (both streams, fork and join inside)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_fused = -1;    // fused_ok() at capture time
     // profiling
     std::vector<hipEvent_t> ev;
     double stage_ms[ST_COUNT] = {0};
@@ -85,8 +103,23 @@ int add_field(mom6h_model *m, const char *name, double **p, size_t n, bool zero)
 {
     int rc = dev_alloc(p, n, zero);
     if (rc) return rc;
-    m->fields.push_back(Field{name, p, n});
+    m->fields.push_back(Field{name, (void **)p, n, 8, true});
     return MOM6H_OK;
+}
+
+int add_field(mom6h_model *m, const char *name, float **p, size_t n, bool zero)
+{
+    int rc = dev_alloc_bytes((void **)p, n * sizeof(float), zero);
+    if (rc) return rc;
+    m->fields.push_back(Field{name, (void **)p, n, 4, true});
+    return MOM6H_OK;
+}
+
+const Field *find_field(const mom6h_model *m, const char *name)
+{
+    for (auto &f : m->fields)
+        if (f.name == name) return &f;
+    return nullptr;
 }
 
 // init_ocean_grid (src/common/mom6_types.F90:130-258): uniform spacing dx.
@@ -113,7 +146,8 @@ std::vector<double> download(const double *d_p, size_t n)
     return h;
 }
 
-int fill_halos(mom6h_model *m, std::initializer_list<std::pair<double *, int>> arrays)
+template <typename T>
+int fill_halos(mom6h_model *m, std::initializer_list<std::pair<T *, int>> arrays)
 {
     if (!m->halo.px && !m->halo.py) return MOM6H_OK;
     for (auto &a : arrays) {
@@ -121,6 +155,16 @@ int fill_halos(mom6h_model *m, std::initializer_list<std::pair<double *, int>> a
         if (rc) return rc;
     }
     return MOM6H_OK;
+}
+
+int fill_halos(mom6h_model *m, std::initializer_list<std::pair<double *, int>> arrays)
+{
+    return fill_halos<double>(m, arrays);
+}
+
+int fill_halos(mom6h_model *m, std::initializer_list<std::pair<float *, int>> arrays)
+{
+    return fill_halos<float>(m, arrays);
 }
 
 #define RC(expr)               \
@@ -158,7 +202,7 @@ struct StageTimer {
 // fields it owns.  It never waits for the 3D stream.
 int enqueue_barotropic(mom6h_model *m)
 {
-    hipStream_t s1 = m->s1;
+    hipStream_t s1 = m->serialize ? m->s0 : m->s1;
     {
         StageTimer t(m, ST_BTSTEP, s1);
         RC(mom6h_btstep(m->bt_cs, m->eta, m->ubt, m->vbt, m->ubt_av, m->vbt_av, m->eta_av, s1));
@@ -171,7 +215,7 @@ int enqueue_barotropic(mom6h_model *m)
 
 bool fused_ok(const mom6h_model *m)
 {
-    return m->fused && m->uniform && m->c.laplacian && !m->c.biharmonic && m->c.coriolis_scheme == 1;
+    return m->fused && m->uniform && m->c.laplacian && m->c.coriolis_scheme == 1;
 }
 
 // transports + hor_visc + CorAdCalc + velocity update of one phase
@@ -187,6 +231,7 @@ int momentum_phase(mom6h_model *m, const double *u, const double *v, const doubl
         a.d = d; a.g = m->ug; a.Kh = m->c.Kh; a.h_neglect = 1.0e-3; a.fdt = fdt; a.no_slip = m->c.no_slip;
         a.u = u; a.v = v; a.h = h; a.bu = m->u; a.bv = m->v; a.ou = ou; a.ov = ov;
         a.mask2dT = m->mask2dT; a.mask2dBu = m->mask2dBu; a.CoriolisBu = m->CoriolisBu;
+        a.Ah = m->c.Ah; a.del2u = m->del2u; a.del2v = m->del2v;
         return launch_momentum(a, s0);
     }
     { StageTimer t(m, ST_TRANSPORTS, s0);
@@ -200,9 +245,75 @@ int momentum_phase(mom6h_model *m, const double *u, const double *v, const doubl
     return MOM6H_OK;
 }
 
+// continuity_PPM of one phase: u, hin -> hout, and with the meridional option
+// the meridional pass after the zonal one, on the zonal result with its
+// periodic halos refilled.
+int continuity_phase(mom6h_model *m, const double *u, const double *v, const double *hin, double *hout,
+                     double dt)
+{
+    hipStream_t s0 = m->s0;
+    StageTimer t(m, ST_CONTINUITY, s0);
+    if (!m->c.meridional)
+        return mom6h_continuity_ppm(m->cont, u, hin, hout, m->uh, dt, nullptr, &m->bt, m->u_cor, m->du_cor, 1, s0);
+    RC(mom6h_continuity_ppm(m->cont, u, hin, m->hx, m->uh, dt, nullptr, &m->bt, m->u_cor, m->du_cor, 1, s0));
+    RC(fill_halos(m, {{m->hx, m->c.nk}}));
+    return mom6h_continuity_ppm_y(m->cont, v, m->hx, hout, m->vh, dt, nullptr, &m->bt_v, m->v_cor, m->dv_cor, 1,
+                                  s0);
+}
+
+// ---- the mixed-precision form: FP32 layer fields and stencils, FP64
+// vertical solve, barotropic chain and 2D fields
+int momentum_phase_f32(mom6h_model *m, const float *u, const float *v, const float *h, float *ou, float *ov,
+                       double fdt)
+{
+    StageTimer t(m, ST_MOMENTUM, m->s0);
+    MomArgsT<float> a;
+    a.d = m->d; a.g = m->ug; a.Kh = (float)m->c.Kh; a.h_neglect = 1.0e-3f; a.fdt = (float)fdt;
+    a.no_slip = m->c.no_slip;
+    a.u = u; a.v = v; a.h = h; a.bu = m->f.u; a.bv = m->f.v; a.ou = ou; a.ov = ov;
+    a.mask2dT = m->mask2dT; a.mask2dBu = m->mask2dBu; a.CoriolisBu = m->CoriolisBu;
+    a.Ah = (float)m->c.Ah; a.del2u = m->f.del2u; a.del2v = m->f.del2v;
+    return launch_momentum(a, m->s0);
+}
+
+int continuity_phase_f32(mom6h_model *m, const float *u, const float *v, const float *hin, float *hout, double dt)
+{
+    hipStream_t s0 = m->s0;
+    StageTimer t(m, ST_CONTINUITY, s0);
+    if (!m->c.meridional)
+        return continuity_f32(m->cont, u, hin, hout, m->f.uh, dt, &m->bt, m->f.bt_h_u, m->f.u_cor, m->du_cor, 1, s0);
+    RC(continuity_f32(m->cont, u, hin, m->f.hx, m->f.uh, dt, &m->bt, m->f.bt_h_u, m->f.u_cor, m->du_cor, 1, s0));
+    RC(fill_halos(m, {{m->f.hx, m->c.nk}}));
+    return continuity_y_f32(m->cont, v, m->f.hx, hout, m->f.vh, dt, &m->bt_v, m->f.bt_h_v, m->f.v_cor, m->dv_cor,
+                            1, s0);
+}
+
+int enqueue_step_f32(mom6h_model *m)
+{
+    const double dt = m->c.dt;
+    const int nk = m->c.nk;
+    hipStream_t s0 = m->s0;
+    auto &f = m->f;
+    // ---- predictor (h = h0 throughout)
+    RC(momentum_phase_f32(m, f.u, f.v, f.h0, f.up, f.vp, dt));
+    { StageTimer t(m, ST_VERT_VISC, s0);
+      RC(vert_visc_f32(m->vv, f.up, f.vp, f.h0, dt, m->taux, m->tauy, m->ray_u, m->ray_v, s0)); }
+    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{f.up, nk}, {f.vp, nk}})); }
+    RC(continuity_phase_f32(m, f.up, f.vp, f.h0, f.htmp, dt));
+    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{f.htmp, nk}})); }
+    // ---- corrector (h = h_pred, held in htmp)
+    RC(momentum_phase_f32(m, f.up, f.vp, f.htmp, f.u, f.v, 0.5 * dt));
+    { StageTimer t(m, ST_VERT_VISC, s0);
+      RC(vert_visc_f32(m->vv, f.u, f.v, f.htmp, 0.5 * dt, m->taux, m->tauy, m->ray_u, m->ray_v, s0)); }
+    RC(continuity_phase_f32(m, f.u, f.v, f.htmp, f.h, 0.5 * dt));
+    { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{f.u, nk}, {f.v, nk}, {f.h, nk}})); }
+    return MOM6H_OK;
+}
+
 // The 3D part of one RK2 step (rk2_serial_driver.F90:218-399) on stream s0.
 int enqueue_step(mom6h_model *m)
 {
+    if (m->mixed) return enqueue_step_f32(m);
     const double dt = m->c.dt;
     const int nk = m->c.nk;
     hipStream_t s0 = m->s0;
@@ -213,16 +324,14 @@ int enqueue_step(mom6h_model *m)
     { StageTimer t(m, ST_VERT_VISC, s0);
       RC(mom6h_vert_visc(m->vv, m->up, m->vp, m->h0, dt, m->taux, m->tauy, rayu, rayv, s0)); }
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->up, nk}, {m->vp, nk}})); }
-    { StageTimer t(m, ST_CONTINUITY, s0);
-      RC(mom6h_continuity_ppm(m->cont, m->up, m->h0, m->htmp, m->uh, dt, nullptr, &m->bt, m->u_cor, m->du_cor, 1, s0)); }
+    RC(continuity_phase(m, m->up, m->vp, m->h0, m->htmp, dt));
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->htmp, nk}})); }
 
     // ---- corrector (h = h_pred, held in htmp)
     RC(momentum_phase(m, m->up, m->vp, m->htmp, m->u, m->v, 0.5 * dt));
     { StageTimer t(m, ST_VERT_VISC, s0);
       RC(mom6h_vert_visc(m->vv, m->u, m->v, m->htmp, 0.5 * dt, m->taux, m->tauy, rayu, rayv, s0)); }
-    { StageTimer t(m, ST_CONTINUITY, s0);
-      RC(mom6h_continuity_ppm(m->cont, m->u, m->htmp, m->h, m->uh, 0.5 * dt, nullptr, &m->bt, m->u_cor, m->du_cor, 1, s0)); }
+    RC(continuity_phase(m, m->u, m->v, m->htmp, m->h, 0.5 * dt));
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}})); }
     return MOM6H_OK;
 }
@@ -238,31 +347,70 @@ int mom6h_model_create(mom6h_model **out, const mom6h_config *c)
     MOM6H_REQUIRE(c->bt_nsteps >= 1 && c->dt > 0.0 && c->dx_km > 0.0, "mom6h_model_create: bad bt_nsteps/dt/dx");
     MOM6H_REQUIRE(!(c->periodic_y && c->walls_y), "mom6h_model_create: periodic_y and walls_y exclude each other");
     MOM6H_REQUIRE(c->coriolis_scheme >= 1 && c->coriolis_scheme <= 3, "mom6h_model_create: bad coriolis_scheme");
+    MOM6H_REQUIRE(c->precision == 0 || c->precision == 1, "mom6h_model_create: precision must be 0 (FP64) or 1 (mixed)");
+    MOM6H_REQUIRE(c->precision == 0 || (c->coriolis_scheme == 1 && c->laplacian && c->nk <= 80),
+                  "mom6h_model_create: the mixed-precision form runs the fused stencils: Sadourny Coriolis, "
+                  "Laplacian (+ biharmonic) viscosity, nk <= 80");
     auto *m = new mom6h_model();
     m->c = *c;
+    m->mixed = c->precision == 1;
     // mom6_types.F90:145-148
     m->d = make_dims(1, c->ni + 2, 1, c->nj + 2, 4, c->ni - 1, 4, c->nj - 1, c->nk);
     m->halo = Halo{c->periodic_x, c->periodic_y};
     const size_t n2 = (size_t)m->d.s2, n3 = n2 * c->nk;
     int rc = 0;
 #define F3(x) rc = rc ? rc : add_field(m, #x, &m->x, n3, true)
+#define G3(x) rc = rc ? rc : add_field(m, #x, &m->f.x, n3, true)
 #define F2(x) rc = rc ? rc : add_field(m, #x, &m->x, n2, true)
-    F3(u); F3(v); F3(h); F3(h0); F3(htmp); F3(uh); F3(vh); F3(CAu); F3(CAv); F3(up); F3(vp);
-    F3(diffu); F3(diffv); F3(u_cor);
+    if (!m->mixed) {
+        F3(u); F3(v); F3(h); F3(h0); F3(htmp); F3(uh); F3(vh); F3(CAu); F3(CAv); F3(up); F3(vp);
+        F3(diffu); F3(diffv); F3(u_cor);
+    } else {
+        G3(u); G3(v); G3(h); G3(h0); G3(htmp); G3(uh); G3(vh); G3(up); G3(vp); G3(u_cor);
+    }
     F2(eta); F2(ubt); F2(vbt); F2(ubt_av); F2(vbt_av); F2(eta_av); F2(du_cor);
     F2(taux); F2(tauy); F2(ray_u); F2(ray_v); F2(CoriolisBu); F2(mask2dT); F2(mask2dCu);
     F2(mask2dCv); F2(mask2dBu);
 #undef F3
+#undef G3
 #undef F2
-    if (!rc) rc = mom6h_bt_cont_alloc(&m->bt, 1, c->ni + 2, 1, c->nj + 2, c->nk);
-    if (!rc) {
-        m->fields.push_back(Field{"FA_u_EE", &m->bt.FA_u_EE, n2});
-        m->fields.push_back(Field{"FA_u_E0", &m->bt.FA_u_E0, n2});
-        m->fields.push_back(Field{"FA_u_W0", &m->bt.FA_u_W0, n2});
-        m->fields.push_back(Field{"FA_u_WW", &m->bt.FA_u_WW, n2});
-        m->fields.push_back(Field{"uBT_WW", &m->bt.uBT_WW, n2});
-        m->fields.push_back(Field{"uBT_EE", &m->bt.uBT_EE, n2});
-        m->fields.push_back(Field{"BT_h_u", &m->bt.h_u, n3});
+    // BT_cont: six 2D fields (FP64) and the 3D marginal thickness (FP32 in the mixed form)
+    auto bt_fields = [&](mom6h_bt_cont &bt, const char *const names[7], float **h3_f32) {
+        if (rc) return;
+        if (!m->mixed) {
+            rc = mom6h_bt_cont_alloc(&bt, 1, c->ni + 2, 1, c->nj + 2, c->nk);
+        } else {
+            for (double **p : {&bt.FA_u_EE, &bt.FA_u_E0, &bt.FA_u_W0, &bt.FA_u_WW, &bt.uBT_WW, &bt.uBT_EE})
+                rc = rc ? rc : dev_alloc(p, n2, true);
+        }
+        if (rc) return;
+        double **p2[6] = {&bt.FA_u_EE, &bt.FA_u_E0, &bt.FA_u_W0, &bt.FA_u_WW, &bt.uBT_WW, &bt.uBT_EE};
+        for (int q = 0; q < 6; ++q) m->fields.push_back(Field{names[q], (void **)p2[q], n2, 8, false});
+        if (!m->mixed) m->fields.push_back(Field{names[6], (void **)&bt.h_u, n3, 8, false});
+        else rc = add_field(m, names[6], h3_f32, n3, true);
+    };
+    if (!rc && c->biharmonic) {   // zero outside the Del2 ranges, for good
+        if (!m->mixed) {
+            rc = add_field(m, "del2u", &m->del2u, n3, true);
+            rc = rc ? rc : add_field(m, "del2v", &m->del2v, n3, true);
+        } else {
+            rc = add_field(m, "del2u", &m->f.del2u, n3, true);
+            rc = rc ? rc : add_field(m, "del2v", &m->f.del2v, n3, true);
+        }
+    }
+    static const char *const bt_u_names[7] = {"FA_u_EE", "FA_u_E0", "FA_u_W0", "FA_u_WW", "uBT_WW", "uBT_EE", "BT_h_u"};
+    static const char *const bt_v_names[7] = {"FA_v_NN", "FA_v_N0", "FA_v_S0", "FA_v_SS", "vBT_SS", "vBT_NN", "BT_h_v"};
+    bt_fields(m->bt, bt_u_names, &m->f.bt_h_u);
+    if (!rc && c->meridional) {
+        if (!m->mixed) {
+            rc = add_field(m, "hx", &m->hx, n3, true);
+            rc = rc ? rc : add_field(m, "v_cor", &m->v_cor, n3, true);
+        } else {
+            rc = add_field(m, "hx", &m->f.hx, n3, true);
+            rc = rc ? rc : add_field(m, "v_cor", &m->f.v_cor, n3, true);
+        }
+        rc = rc ? rc : add_field(m, "dv_cor", &m->dv_cor, n2, true);
+        bt_fields(m->bt_v, bt_v_names, &m->f.bt_h_v);
     }
     if (!rc && hipStreamCreateWithFlags(&m->s0, hipStreamNonBlocking) != hipSuccess)
         rc = set_error(MOM6H_ENOMEM, "stream creation failed");
@@ -290,15 +438,14 @@ int mom6h_model_create(mom6h_model **out, const mom6h_config *c)
     return MOM6H_OK;
 }
 
-int mom6h_model_field(mom6h_model *m, const char *name, double **d_ptr, long *count)
+int mom6h_model_field(mom6h_model *m, const char *name, void **d_ptr, long *count)
 {
     MOM6H_REQUIRE(m && name, "mom6h_model_field: NULL argument");
-    for (auto &f : m->fields)
-        if (f.name == name) {
-            if (d_ptr) *d_ptr = *f.p;
-            if (count) *count = (long)f.n;
-            return MOM6H_OK;
-        }
+    if (const Field *f = find_field(m, name)) {
+        if (d_ptr) *d_ptr = *f->p;
+        if (count) *count = (long)f->n;
+        return MOM6H_OK;
+    }
     if (std::string(name) == "vv_taux_bot" || std::string(name) == "vv_tauy_bot" ||
         std::string(name) == "vv_a_u" || std::string(name) == "vv_h_u") {
         MOM6H_REQUIRE(m->vv, "mom6h_model_field: %s exists after prepare", name);
@@ -315,25 +462,66 @@ int mom6h_model_field(mom6h_model *m, const char *name, double **d_ptr, long *co
     return set_error(MOM6H_EINVAL, "mom6h_model_field: unknown field '%s'", name);
 }
 
+int mom6h_model_field_bytes(mom6h_model *m, const char *name, int *elem_bytes)
+{
+    MOM6H_REQUIRE(m && name && elem_bytes, "mom6h_model_field_bytes: NULL argument");
+    const Field *f = find_field(m, name);
+    long n = 0;
+    if (!f) RC(mom6h_model_field(m, name, nullptr, &n));   // the vert_visc arrays: FP64
+    *elem_bytes = f ? f->bytes : 8;
+    return MOM6H_OK;
+}
+
 int mom6h_model_upload(mom6h_model *m, const char *name, const double *src, long count)
 {
-    double *p = nullptr;
+    void *p = nullptr;
     long n = 0;
+    int bytes = 8;
     RC(mom6h_model_field(m, name, &p, &n));
+    RC(mom6h_model_field_bytes(m, name, &bytes));
     MOM6H_REQUIRE(src && count == n, "mom6h_model_upload(%s): expected %ld values, got %ld", name, n, count);
     MOM6H_CHECK(hipDeviceSynchronize());
-    MOM6H_CHECK(hipMemcpy(p, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    if (bytes == 8) {
+        MOM6H_CHECK(hipMemcpy(p, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp((size_t)n);
+        for (size_t q = 0; q < (size_t)n; ++q) tmp[q] = (float)src[q];
+        MOM6H_CHECK(hipMemcpy(p, tmp.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return MOM6H_OK;
+}
+
+int mom6h_model_fill(mom6h_model *m, const char *name, const double *d_src, long count)
+{
+    void *p = nullptr;
+    long n = 0;
+    int bytes = 8;
+    RC(mom6h_model_field(m, name, &p, &n));
+    RC(mom6h_model_field_bytes(m, name, &bytes));
+    MOM6H_REQUIRE(d_src && count == n, "mom6h_model_fill(%s): expected %ld values, got %ld", name, n, count);
+    MOM6H_CHECK(hipDeviceSynchronize());
+    if (bytes == 8) MOM6H_CHECK(hipMemcpy(p, d_src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice));
+    else RC(narrow_to_f32(d_src, (float *)p, n, m->s0));
+    MOM6H_CHECK(hipStreamSynchronize(m->s0));
     return MOM6H_OK;
 }
 
 int mom6h_model_download(mom6h_model *m, const char *name, double *dst, long count)
 {
-    double *p = nullptr;
+    void *p = nullptr;
     long n = 0;
+    int bytes = 8;
     RC(mom6h_model_field(m, name, &p, &n));
+    RC(mom6h_model_field_bytes(m, name, &bytes));
     MOM6H_REQUIRE(dst && count == n, "mom6h_model_download(%s): expected %ld values, got %ld", name, n, count);
     MOM6H_CHECK(hipDeviceSynchronize());
-    MOM6H_CHECK(hipMemcpy(dst, p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    if (bytes == 8) {
+        MOM6H_CHECK(hipMemcpy(dst, p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp((size_t)n);
+        MOM6H_CHECK(hipMemcpy(tmp.data(), p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < (size_t)n; ++q) dst[q] = (double)tmp[q];
+    }
     return MOM6H_OK;
 }
 
@@ -354,11 +542,16 @@ int mom6h_model_prepare(mom6h_model *m)
     RC(mom6h_continuity_init(&m->cont, d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk,
                              m->IareaT.data(), m->IdxT.data(), m->dy_Cu.data(), mT.data(),
                              m->c.monotonic, m->dxT.data(), m->dxCu.data(), mCu.data()));
-    // coriolis_init (mom6_coriolis_serial.F90:48-84)
-    RC(mom6h_coriolis_init(&m->cor, d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk,
-                           m->areaT.data(), m->IareaBu.data(), fq.data(), mBu.data(), m->dyCv.data(),
-                           m->dxCu.data(), m->dyCu.data(), m->dxCv.data(), m->IdxCu.data(),
-                           m->IdyCv.data(), m->c.coriolis_scheme));
+    if (m->c.meridional)
+        RC(mom6h_continuity_set_meridional(m->cont, m->IdyT.data(), m->dxCv.data(), m->dyT.data(), m->dyCv.data(),
+                                           mCv.data()));
+    // coriolis_init (mom6_coriolis_serial.F90:48-84); the mixed form uses
+    // the fused stencil only
+    if (!m->mixed)
+        RC(mom6h_coriolis_init(&m->cor, d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, nk,
+                               m->areaT.data(), m->IareaBu.data(), fq.data(), mBu.data(), m->dyCv.data(),
+                               m->dxCu.data(), m->dyCu.data(), m->dxCv.data(), m->IdxCu.data(),
+                               m->IdyCv.data(), m->c.coriolis_scheme));
     // barotropic_init (mom6_barotropic_serial.F90:63-137); closed v faces carry no transport
     {
         std::vector<double> Datu(n2), Datv(n2), g(n2, kG_EARTH), rem(n2, 0.999), f4(4 * n2);
@@ -379,7 +572,7 @@ int mom6h_model_prepare(mom6h_model *m)
                             mCu.data(), mCv.data(), m->c.Kv, m->c.Kv_ml, m->c.Kv_extra_bbl,
                             m->c.Hmix, m->c.Hbbl));
     // hor_visc_init (mom6_hor_visc_serial.F90:160-286)
-    {
+    if (!m->mixed) {
         std::vector<double> dx2h(n2), dy2h(n2), dx2q(n2), dy2q(n2), DX_dyT(n2), DY_dxT(n2);
         std::vector<double> DX_dyBu(n2), DY_dxBu(n2), I2u(n2), Iu2(n2), I2v(n2), Iv2(n2), one(n2, 1.0);
         for (size_t q = 0; q < n2; ++q) {
@@ -431,11 +624,21 @@ int mom6h_model_prepare(mom6h_model *m)
             g.DX_dyBu = dxBu * IdyBu;
             g.DY_dxBu = dyBu * IdxBu;
             g.Area_q = (g.areaT + g.areaT) + (g.areaT + g.areaT);
+            g.Idx2dyCu = g.IdxCu * g.IdxCu * g.IdyCu;
+            g.Idxdy2u = g.IdxCu * g.IdyCu * g.IdyCu;
+            g.Idx2dyCv = g.IdxCv * g.IdxCv * g.IdyCv;
+            g.Idxdy2v = g.IdxCv * g.IdyCv * g.IdyCv;
         }
         m->uniform = ok;
     }
+    MOM6H_REQUIRE(!m->mixed || m->uniform,
+                  "mom6h_model_prepare: the mixed-precision form needs a grid with uniform metrics");
     // periodic fill of the initial state (rk2_mpi_serial_driver.F90:191-197)
-    RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}, {m->h0, nk}, {m->eta, 1}, {m->ubt, 1}, {m->vbt, 1}}));
+    if (!m->mixed)
+        RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}, {m->h0, nk}}));
+    else
+        RC(fill_halos(m, {{m->f.u, nk}, {m->f.v, nk}, {m->f.h, nk}, {m->f.h0, nk}}));
+    RC(fill_halos(m, {{m->eta, 1}, {m->ubt, 1}, {m->vbt, 1}}));
     MOM6H_CHECK(hipStreamSynchronize(m->s0));
     m->prepared = true;
     return MOM6H_OK;
@@ -449,6 +652,46 @@ int mom6h_model_step(mom6h_model *m, int niter)
     // The fused path consumes vert_visc's coupling coefficients on chip; the
     // module path keeps them in the control structure, as the reference does.
     RC(mom6h_vert_visc_keep_coefficients(m->vv, fused_ok(m) ? 0 : 1));
+    if (m->use_graph && !m->profile && !m->serialize) {
+        // One step (barotropic chain forked onto s1 and joined back) replayed
+        // as a graph; steps then no longer overlap across the join.
+        const int key = fused_ok(m) ? 1 : 0;
+        if (m->gexec && m->graph_fused != key) {
+            (void)hipGraphExecDestroy(m->gexec);
+            (void)hipGraphDestroy(m->graph);
+            m->gexec = nullptr;
+            m->graph = nullptr;
+        }
+        if (!m->gexec) {
+            MOM6H_CHECK(hipStreamBeginCapture(m->s0, hipStreamCaptureModeThreadLocal));
+            int rc = MOM6H_OK;
+            if (hipEventRecord(m->e_start, m->s0) != hipSuccess || hipStreamWaitEvent(m->s1, m->e_start, 0) != hipSuccess)
+                rc = set_error(MOM6H_ESTATE, "mom6h_model_step: fork into the captured step failed");
+            if (!rc) rc = enqueue_barotropic(m);
+            if (!rc) rc = enqueue_step(m);
+            if (!rc && (hipEventRecord(m->e_bt, m->s1) != hipSuccess || hipStreamWaitEvent(m->s0, m->e_bt, 0) != hipSuccess))
+                rc = set_error(MOM6H_ESTATE, "mom6h_model_step: join of the captured step failed");
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(m->s0, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            MOM6H_CHECK(ec);
+            m->graph = g;
+            MOM6H_CHECK(hipGraphInstantiate(&m->gexec, m->graph, nullptr, nullptr, 0));
+            m->graph_fused = key;
+        }
+        for (int it = 0; it < niter; ++it) MOM6H_CHECK(hipGraphLaunch(m->gexec, m->s0));
+        return MOM6H_OK;
+    }
+    if (m->serialize) {
+        for (int it = 0; it < niter; ++it) {
+            RC(enqueue_barotropic(m));
+            RC(enqueue_step(m));
+        }
+        return MOM6H_OK;
+    }
     // The barotropic chain and the 3D chain share no array (the 2D fields the
     // barotropic solver reads and writes are never touched by the 3D stages),
     // so s1 runs all its steps without waiting for s0; the streams join once.
@@ -484,7 +727,11 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
     MOM6H_REQUIRE(m && key, "mom6h_model_set_option: NULL argument");
     const std::string k(key);
     if (k == "graph") m->use_graph = value != 0.0;
-    else if (k == "fused") m->fused = value != 0.0;
+    else if (k == "serialize") m->serialize = value != 0.0;
+    else if (k == "fused") {
+        MOM6H_REQUIRE(!m->mixed || value != 0.0, "mom6h_model_set_option: the mixed-precision form is fused only");
+        m->fused = value != 0.0;
+    }
     else if (k == "profile") {
         m->profile = value != 0.0;
         for (double &x : m->stage_ms) x = 0.0;
@@ -526,8 +773,11 @@ int mom6h_model_destroy(mom6h_model *m)
     mom6h_vert_visc_end(m->vv);
     mom6h_hor_visc_end(m->hv);
     for (auto &f : m->fields)
-        if (f.name.rfind("FA_", 0) != 0 && f.name.rfind("uBT", 0) != 0 && f.name != "BT_h_u") dev_free(*f.p);
+        if (f.owned && *f.p) (void)hipFree(*f.p);
     mom6h_bt_cont_free(&m->bt);
+    mom6h_bt_cont_free(&m->bt_v);
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
     if (m->e_start) (void)hipEventDestroy(m->e_start);
     if (m->e_bt) (void)hipEventDestroy(m->e_bt);
     if (m->s0) (void)hipStreamDestroy(m->s0);

@@ -48,7 +48,8 @@ void dev_free(double *p)
 // One thread per (halo cell, k).  x-fill: columns isd..isc-1 and iec+1..ied of
 // every row; y-fill: rows jsd..jsc-1 and jec+1..jed of every column, run after
 // the x-fill so corners take their x-filled source.
-__global__ void k_halo_x(Dims d, double *a, int nk)
+template <typename T>
+__global__ void k_halo_x(Dims d, T *a, int nk)
 {
     const int hw_w = d.isc - d.isd, hw_e = d.ied - d.iec, hw = hw_w + hw_e;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,7 +65,8 @@ __global__ void k_halo_x(Dims d, double *a, int nk)
     a[IX3(d, i, j, k)] = a[IX3(d, src, j, k)];
 }
 
-__global__ void k_halo_y(Dims d, double *a, int nk)
+template <typename T>
+__global__ void k_halo_y(Dims d, T *a, int nk)
 {
     const int hs = d.jsc - d.jsd, hn = d.jed - d.jec, hw = hs + hn;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,7 +82,8 @@ __global__ void k_halo_y(Dims d, double *a, int nk)
     a[IX3(d, i, j, k)] = a[IX3(d, i, src, k)];
 }
 
-int halo_fill(const Dims &d, const Halo &hm, double *a, int nk, hipStream_t s)
+template <typename T>
+int halo_fill_t(const Dims &d, const Halo &hm, T *a, int nk, hipStream_t s)
 {
     const int bs = 256;
     if (hm.px) {
@@ -97,6 +100,55 @@ int halo_fill(const Dims &d, const Halo &hm, double *a, int nk, hipStream_t s)
             MOM6H_LAUNCH_CHECK();
         }
     }
+    return MOM6H_OK;
+}
+
+int halo_fill(const Dims &d, const Halo &hm, double *a, int nk, hipStream_t s)
+{
+    return halo_fill_t(d, hm, a, nk, s);
+}
+
+int halo_fill(const Dims &d, const Halo &hm, float *a, int nk, hipStream_t s)
+{
+    return halo_fill_t(d, hm, a, nk, s);
+}
+
+// ------------------------------------------------------------ conversions
+__global__ void k_narrow(const double *__restrict__ src, float *__restrict__ dst, long long n)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = (float)src[t];
+}
+
+__global__ void k_widen(const float *__restrict__ src, double *__restrict__ dst, long long n)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = (double)src[t];
+}
+
+int narrow_to_f32(const double *src, float *dst, long long n, hipStream_t s)
+{
+    if (n <= 0) return MOM6H_OK;
+    k_narrow<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, dst, n);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
+}
+
+int widen_to_f64(const float *src, double *dst, long long n, hipStream_t s)
+{
+    if (n <= 0) return MOM6H_OK;
+    k_widen<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, dst, n);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
+}
+
+int dev_alloc_bytes(void **p, size_t bytes, bool zero)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return set_error(MOM6H_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    if (zero) MOM6H_CHECK(hipMemset(*p, 0, bytes));
     return MOM6H_OK;
 }
 

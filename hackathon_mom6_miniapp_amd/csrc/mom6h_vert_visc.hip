@@ -179,10 +179,15 @@ __global__ void k_vert_visc(Dims d, double *__restrict__ u, double *__restrict__
 // levels, so the compiler interleaves them.  Only the Thomas recurrence itself
 // (one division per level) is serial.  Levels that do not fill a chunk take a
 // one-level loop with the same arithmetic.
+// T: storage type of u, v and h (double, or float in the mixed-precision
+// form); the coefficients and the tridiagonal solve are FP64 either way, and
+// between sweeps B and C the forward-eliminated velocities live in the
+// velocity array itself (rounded to T).
+template <typename T>
 struct VVArgs {
     Dims d;
-    double *u, *v;
-    const double *h;
+    T *u, *v;
+    const T *h;
     double dt;
     const double *taux, *tauy, *ray_u, *ray_v, *mask2dCu, *mask2dCv;
     double *a_u, *a_v, *h_u, *h_v, *taux_bot, *tauy_bot;
@@ -192,8 +197,8 @@ struct VVArgs {
 constexpr int VV_C = 8;    // levels per chunk of sweep B (heavy bodies)
 constexpr int VV_A = 16;   // levels per chunk of sweeps A and C (light bodies, latency-bound loads)
 
-template <bool KEEP>
-__global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
+template <typename T, bool KEEP>
+__global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs<T> p)
 {
     extern __shared__ double Zs[];
     const Dims &d = p.d;
@@ -205,8 +210,9 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
     const bool is_u = (row & 1) != 0;
     const long long s = d.s2;
     const int nk = d.nk;
-    const double *hA, *hB;
-    double *vel, *a_out, *hu_out, *tb_out;
+    const T *hA, *hB;
+    T *vel;
+    double *a_out, *hu_out, *tb_out;
     double m, sfc, Ray_bot;
     {
         const double dt_Rho0 = p.dt / 1035.0;
@@ -316,7 +322,7 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
         b1 = 1.0 / (b_denom_1 + dt * a_n);
         d1 = b_denom_1 * b1;
         vprev = b1 * (hu_k * v1 + sfc);
-        vel[0] = vprev;
+        vel[0] = (T)vprev;
     }
     // one interior level k (2..nk-1): the recurrence, given a(k+1) and hvel(k+1)
     auto B_serial = [&](int k, double a_next, double hv_next, double vk_next) {
@@ -330,7 +336,7 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
         b1 = 1.0 / (b_denom_1 + dt * a_next);
         d1 = b_denom_1 * b1;
         vprev = (hu_k * vk_cur + dt * a_k * vprev) * b1;
-        vel[(k - 1) * s] = vprev;
+        vel[(k - 1) * s] = (T)vprev;
         a_n = a_next;
         hv_n = hv_next;
         vk_cur = vk_next;
@@ -390,7 +396,7 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
         b1 = 1.0 / (b_denom_1 + dt * a_n);
         d1 = b_denom_1 * b1;
         vprev = (hu_k * vk_cur + dt * a_k * vprev) * b1;
-        vel[(k - 1) * s] = vprev;
+        vel[(k - 1) * s] = (T)vprev;
     }
     if (KEEP) a_out[nk * s] = a_n;
 
@@ -412,19 +418,38 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs p)
 #pragma unroll
             for (int c = 0; c < VV_A; ++c) {
                 vnext = fv[c] + Z(k0 - c + 1) * vnext;
-                vel[(k0 - c - 1) * s] = vnext;
+                vel[(k0 - c - 1) * s] = (T)vnext;
             }
         }
     }
     for (; k0 >= 1; --k0) {
         vnext = vel[(k0 - 1) * s] + Z(k0 + 1) * vnext;
-        vel[(k0 - 1) * s] = vnext;
+        vel[(k0 - 1) * s] = (T)vnext;
     }
     // bottom stress (:452-458)
     const double RHO_0 = 1035.0;
     double t = RHO_0 * vprev * a_n;
     t = t + RHO_0 * Ray_bot * vprev;
     *tb_out = t;
+}
+
+// The mixed-precision form: FP32 u, v, h in memory, FP64 solve; the
+// coefficients stay on chip (the LDS kernel only).
+int vert_visc_f32(mom6h_vert_visc_cs *cs, float *u, float *v, const float *h, double dt, const double *taux,
+                  const double *tauy, const double *ray_u_bot, const double *ray_v_bot, hipStream_t stream)
+{
+    const Dims &d = cs->d;
+    const int ni = d.iec - d.isc + 2;
+    const int rows = (d.jec - d.jsc + 1) + (d.jec - d.jsc + 2);
+    const size_t lds = (size_t)(d.nk + 1) * 64 * sizeof(double);
+    MOM6H_REQUIRE((long long)lds <= cs->max_lds, "vert_visc (FP32 state): nk = %d exceeds the LDS column (%lld B)",
+                  d.nk, cs->max_lds);
+    VVArgs<float> p{d, u, v, h, dt, taux, tauy, ray_u_bot, ray_v_bot, cs->mask2dCu, cs->mask2dCv,
+                    cs->a_u, cs->a_v, cs->h_u, cs->h_v, cs->taux_bot, cs->tauy_bot, cs->Kv, cs->Kv_ml,
+                    cs->Kv_extra_bbl, cs->Hmix, cs->Hbbl};
+    k_vert_visc_chunked<float, false><<<dim3((ni + 63) / 64, rows), dim3(64), lds, stream>>>(p);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
 }
 
 }  // namespace mom6h
@@ -482,14 +507,14 @@ int mom6h_vert_visc(mom6h_vert_visc_cs *cs, double *u, double *v, const double *
     const int rows = (d.jec - d.jsc + 1) + (d.jec - d.jsc + 2);
     const size_t lds = (size_t)(d.nk + 1) * 64 * sizeof(double);
     if ((long long)lds <= cs->max_lds) {
-        VVArgs p{d, u, v, h, dt, taux, tauy, ray_u_bot, ray_v_bot, cs->mask2dCu, cs->mask2dCv,
+        VVArgs<double> p{d, u, v, h, dt, taux, tauy, ray_u_bot, ray_v_bot, cs->mask2dCu, cs->mask2dCv,
                  cs->a_u, cs->a_v, cs->h_u, cs->h_v, cs->taux_bot, cs->tauy_bot, cs->Kv, cs->Kv_ml,
                  cs->Kv_extra_bbl, cs->Hmix, cs->Hbbl};
         const dim3 g((ni + 63) / 64, rows);
         if (cs->keep_coefs)
-            k_vert_visc_chunked<true><<<g, dim3(64), lds, (hipStream_t)stream>>>(p);
+            k_vert_visc_chunked<double, true><<<g, dim3(64), lds, (hipStream_t)stream>>>(p);
         else
-            k_vert_visc_chunked<false><<<g, dim3(64), lds, (hipStream_t)stream>>>(p);
+            k_vert_visc_chunked<double, false><<<g, dim3(64), lds, (hipStream_t)stream>>>(p);
         MOM6H_LAUNCH_CHECK();
         return MOM6H_OK;
     }

@@ -26,6 +26,10 @@ def to_mconfig(cfg: Config) -> MConfig:
     c.laplacian, c.biharmonic, c.no_slip = int(cfg.laplacian), int(cfg.biharmonic), 0
     c.Kv, c.Kv_ml, c.Kv_extra_bbl, c.Hmix, c.Hbbl = cfg.Kv, cfg.Kv_ml, cfg.Kv_extra_bbl, cfg.Hmix, cfg.Hbbl
     c.coriolis_scheme, c.monotonic = cfg.coriolis_scheme, int(cfg.monotonic)
+    c.meridional = int(cfg.meridional)
+    if cfg.precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', not {cfg.precision!r}")
+    c.precision = 1 if cfg.precision == "mixed" else 0
     return c
 
 
@@ -44,7 +48,7 @@ class Model:
                 self.upload(name, val)
         check(self.L.mom6h_device_synchronize(), "synchronize")
         check(self.L.mom6h_memcpy_d2d(self.device_ptr("h"), self.device_ptr("h0"),
-                                      8 * self._count("h0")), "h = h0")
+                                      self.field_bytes("h0") * self._count("h0")), "h = h0")
         ray = np.full((cfg.n1, cfg.n2), cfg.ray_bottom, order="F")
         self.upload("ray_u", ray)
         self.upload("ray_v", ray)
@@ -76,13 +80,29 @@ class Model:
         a = np.asfortranarray(np.asarray(arr, dtype=np.float64))
         check(self.L.mom6h_model_upload(self.m, name.encode(), a.ctypes.data, a.size), f"upload {name}")
 
+    def field_bytes(self, name: str) -> int:
+        b = C.c_int()
+        check(self.L.mom6h_model_field_bytes(self.m, name.encode(), C.byref(b)), f"field {name}")
+        return b.value
+
     def generate(self, name: str, rec: Seeded):
-        """Fill a 3D field from its seeded recipe on the device."""
+        """Fill a 3D field from its seeded recipe on the device (FP64 values,
+        rounded once for an FP32 field)."""
         n = self._count(name)
         mask = self.device_ptr(rec.mask) if rec.mask else None
-        check(self.L.mom6h_seeded_normal(self.device_ptr(name), n, rec.seed, rec.field,
-                                         int(rec.mean is not None), rec.mean or 0.0, rec.sigma,
-                                         mask, self.cfg.n1 * self.cfg.n2, None), f"generate {name}")
+        args = (n, rec.seed, rec.field, int(rec.mean is not None), rec.mean or 0.0, rec.sigma,
+                mask, self.cfg.n1 * self.cfg.n2, None)
+        if self.field_bytes(name) == 8:
+            check(self.L.mom6h_seeded_normal(self.device_ptr(name), *args), f"generate {name}")
+            return
+        tmp = C.c_void_p()
+        check(self.L.mom6h_malloc(C.byref(tmp), 8 * n), "malloc")
+        try:
+            check(self.L.mom6h_seeded_normal(tmp, *args), f"generate {name}")
+            check(self.L.mom6h_device_synchronize(), "synchronize")
+            check(self.L.mom6h_model_fill(self.m, name.encode(), tmp, n), f"fill {name}")
+        finally:
+            self.L.mom6h_free(tmp)
 
     def get(self, name: str) -> np.ndarray:
         n = self._count(name)

@@ -65,6 +65,12 @@ class Config:
     ray_bottom: float = 1.0e-4
     coriolis_scheme: int = 1
     monotonic: bool = False
+    # continuity also takes a meridional PPM pass after the zonal one (no
+    # counterpart in the reference, whose continuity is zonal only)
+    meridional: bool = False
+    # "fp64": the reference's arithmetic throughout; "mixed": FP32 layer
+    # fields and stencils, FP64 vertical solve, barotropic solver and 2D fields
+    precision: str = "fp64"
 
     # ---- derived bounds (mom6_types.F90:145-148)
     @property

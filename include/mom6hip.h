@@ -76,6 +76,21 @@ int mom6h_continuity_ppm(mom6h_continuity_cs *cs, const double *u, const double 
                          double *uh, double dt, const double *uhbt, const mom6h_bt_cont *bt_cont,
                          double *u_cor, double *du_cor, int hin_halo_to_h, void *stream);
 int mom6h_continuity_end(mom6h_continuity_cs *cs);
+/* Meridional pass.  No counterpart in the reference, whose continuity is
+ * zonal only (continuity_PPM_cuda_c, src/serial/mom6_continuity_serial.F90:
+ * 100-145): MOM6's meridional_mass_flux, i.e. the zonal algorithm with
+ * (i, u, IdxT, dy_Cu, dxT, dxCu, mask2dCu) exchanged for (j, v, IdyT, dx_Cv,
+ * dyT, dyCv, mask2dCv).  set_meridional copies those y metrics (host arrays);
+ * ppm_y then takes v, hin -> h (compute domain), vh on the faces (i, J),
+ * i = isc..iec, J = jsc-1..jec, with the same optional outputs as the zonal
+ * call; bt_cont receives the v-side BT_cont fields in the same slots
+ * (FA_u_EE -> FA_v_NN, FA_u_E0 -> FA_v_N0, FA_u_W0 -> FA_v_S0, FA_u_WW ->
+ * FA_v_SS, uBT_WW -> vBT_SS, uBT_EE -> vBT_NN, h_u -> h_v).  Needs nk <= 80. */
+int mom6h_continuity_set_meridional(mom6h_continuity_cs *cs, const double *h_IdyT, const double *h_dx_Cv,
+                                    const double *h_dyT, const double *h_dyCv, const double *h_mask2dCv);
+int mom6h_continuity_ppm_y(mom6h_continuity_cs *cs, const double *v, const double *hin, double *h,
+                           double *vh, double dt, const double *vhbt, const mom6h_bt_cont *bt_cont,
+                           double *v_cor, double *dv_cor, int hin_halo_to_h, void *stream);
 /* alloc_BT_cont_type_cuda_c / dealloc (:389-435): zeroed device arrays */
 int mom6h_bt_cont_alloc(mom6h_bt_cont *bt, int isd, int ied, int jsd, int jed, int nk);
 int mom6h_bt_cont_free(mom6h_bt_cont *bt);
@@ -207,20 +222,39 @@ typedef struct mom6h_config {
     int laplacian, biharmonic, no_slip;
     double Kv, Kv_ml, Kv_extra_bbl, Hmix, Hbbl;
     int coriolis_scheme, monotonic;
+    int meridional;   /* continuity also takes the meridional pass (mom6h_continuity_ppm_y) */
+    int precision;    /* 0: FP64 throughout (the reference's arithmetic); 1: mixed -- FP32 layer
+                         fields (u v h h0 htmp up vp uh vh u_cor BT_h_u ...) and FP32 stencil
+                         arithmetic (PPM continuity with Kahan-compensated column sums, Coriolis/KE,
+                         Laplacian viscosity); FP64 vertical-viscosity solve, barotropic solver,
+                         2D fields and BT_cont 2D fields.  Needs the fused stencils: uniform
+                         metrics, Sadourny Coriolis, Laplacian viscosity, nk <= 80. */
 } mom6h_config;
 
 typedef struct mom6h_model mom6h_model;
 int mom6h_model_create(mom6h_model **m, const mom6h_config *c);
-int mom6h_model_field(mom6h_model *m, const char *name, double **d_ptr, long *count);
+/* Device address and element count of a field; its element type is FP64,
+ * or FP32 for the layer fields of the mixed-precision form
+ * (mom6h_model_field_bytes gives 8 or 4). */
+int mom6h_model_field(mom6h_model *m, const char *name, void **d_ptr, long *count);
+int mom6h_model_field_bytes(mom6h_model *m, const char *name, int *elem_bytes);
+/* Host FP64 values in and out (converted for FP32 fields), and a fill from
+ * FP64 values already on the device (d_src, count elements). */
 int mom6h_model_upload(mom6h_model *m, const char *name, const double *h_src, long count);
 int mom6h_model_download(mom6h_model *m, const char *name, double *h_dst, long count);
+int mom6h_model_fill(mom6h_model *m, const char *name, const double *d_src, long count);
 /* after the inputs are uploaded: barotropic coefficients from CoriolisBu,
  * wall masks, periodic fill of the initial state */
 int mom6h_model_prepare(mom6h_model *m);
 int mom6h_model_step(mom6h_model *m, int niter);
 int mom6h_model_synchronize(mom6h_model *m);
-/* Options: "graph" (1 = replay a captured hipGraph per step, default 1),
- * "profile" (1 = record per-stage HIP events; disables the graph). */
+/* Options: "graph" (1 = capture one step, both streams, as a hipGraph and
+ * replay it per step; default 0), "fused" (1 = fused momentum stencil and
+ * on-chip vert_visc coefficients where the grid allows, default 1; 0 = the
+ * module-by-module path), "profile" (1 = record per-stage HIP events; the
+ * graph is not used while profiling), "serialize" (1 = run the barotropic
+ * chain on the 3D stream, one step after the other, so that per-stage events
+ * time each kernel without the overlap; default 0). */
 int mom6h_model_set_option(mom6h_model *m, const char *key, double value);
 /* Per-stage GPU time (ms) accumulated while "profile" is on; names are
  * written comma-separated into names (size nbytes). Returns the stage count. */

@@ -91,6 +91,11 @@ struct om_model {
     double *hv_bhstr_xx, *hv_bhstr_xy, *hv_h_u, *hv_h_v, *hv_hq, *hv_Kh, *hv_Ah, *hv_Shear_mag;
     double *hv_Del2u, *hv_Del2v, *hv_hrat_min, *hv_visc_bound_rem;
 
+    /* meridional pass: the transposed model it runs the zonal pass on, and the
+     * meridional outputs (v_cor, dv_cor, BT_cont v-side) */
+    struct om_model *mt;
+    double *v_cor, *dv_cor, *FA_v_NN, *FA_v_N0, *FA_v_S0, *FA_v_SS, *vBT_SS, *vBT_NN, *BT_h_v;
+
     named *tab;
     int ntab, captab;
 };
@@ -751,6 +756,75 @@ void om_continuity_ppm(om_model *m, const double *u, const double *hin, double *
     zonal_edge_thickness(m, hin, m->cont_hW, m->cont_hE);
     zonal_mass_flux(m, u, hin, m->cont_hW, m->cont_hE, uh, dt);
     continuity_zonal_convergence(m, h, uh, dt, hin);
+}
+
+/* Meridional pass (no counterpart in the reference, whose continuity is
+ * zonal only).  MOM6's meridional mass flux is the zonal algorithm with
+ * (i, u, dx, dy_Cu, IdxT, dxT, dxCu, mask2dCu, h_W/h_E) exchanged for
+ * (j, v, dy, dx_Cv, IdyT, dyT, dyCv, mask2dCv, h_S/h_N); here it is exactly
+ * that: the zonal pass above, pinned to the reference, applied to the
+ * transposed grid and state (a second model with ni and nj exchanged). */
+static void transpose3(const om_model *m, const double *a, om_model *t, double *b, int nk)
+{
+    for (int k = 1; k <= nk; ++k)
+        for (int j = m->jsd; j <= m->jed; ++j)
+            for (int i = m->isd; i <= m->ied; ++i)
+                b[(size_t)(k - 1) * t->N2 + (size_t)(i - t->jsd) * t->n1 + (size_t)(j - t->isd)] =
+                    a[(size_t)(k - 1) * m->N2 + (size_t)(j - m->jsd) * m->n1 + (size_t)(i - m->isd)];
+}
+
+static void transpose3_back(const om_model *t, const double *b, om_model *m, double *a, int nk)
+{
+    for (int k = 1; k <= nk; ++k)
+        for (int j = m->jsd; j <= m->jed; ++j)
+            for (int i = m->isd; i <= m->ied; ++i)
+                a[(size_t)(k - 1) * m->N2 + (size_t)(j - m->jsd) * m->n1 + (size_t)(i - m->isd)] =
+                    b[(size_t)(k - 1) * t->N2 + (size_t)(i - t->jsd) * t->n1 + (size_t)(j - t->isd)];
+}
+
+static om_model *transposed_model(om_model *m)
+{
+    if (m->mt) return m->mt;
+    om_config c = m->c;
+    c.ni = m->c.nj;
+    c.nj = m->c.ni;
+    om_model *t = om_create(&c);
+    /* the y metrics of m become the x metrics of t */
+    transpose3(m, m->IareaT, t, t->IareaT, 1);
+    transpose3(m, m->IdyT, t, t->IdxT, 1);
+    transpose3(m, m->dyT, t, t->dxT, 1);
+    transpose3(m, m->dxCv, t, t->dy_Cu, 1);
+    transpose3(m, m->dyCv, t, t->dxCu, 1);
+    transpose3(m, m->mask2dT, t, t->mask2dT, 1);
+    transpose3(m, m->mask2dCv, t, t->mask2dCu, 1);
+    t->monotonic = m->monotonic; t->upwind_1st = m->upwind_1st; t->simple_2nd = m->simple_2nd;
+    m->mt = t;
+    return t;
+}
+
+void om_continuity_ppm_y(om_model *m, const double *v, const double *hin, double *h, double *vh, double dt)
+{
+    om_model *t = transposed_model(m);
+    const int nk = m->nk;
+    double *vT = (double *)malloc(t->N3 * sizeof(double)), *hinT = (double *)malloc(t->N3 * sizeof(double));
+    double *hT = (double *)malloc(t->N3 * sizeof(double)), *vhT = (double *)malloc(t->N3 * sizeof(double));
+    transpose3(m, v, t, vT, nk);
+    transpose3(m, hin, t, hinT, nk);
+    transpose3(m, h, t, hT, nk);
+    transpose3(m, vh, t, vhT, nk);
+    om_continuity_ppm(t, vT, hinT, hT, vhT, dt);
+    transpose3_back(t, hT, m, h, nk);
+    transpose3_back(t, vhT, m, vh, nk);
+    transpose3_back(t, t->u_cor, m, m->v_cor, nk);
+    transpose3_back(t, t->du_cor, m, m->dv_cor, 1);
+    transpose3_back(t, t->FA_u_EE, m, m->FA_v_NN, 1);
+    transpose3_back(t, t->FA_u_E0, m, m->FA_v_N0, 1);
+    transpose3_back(t, t->FA_u_W0, m, m->FA_v_S0, 1);
+    transpose3_back(t, t->FA_u_WW, m, m->FA_v_SS, 1);
+    transpose3_back(t, t->uBT_WW, m, m->vBT_SS, 1);
+    transpose3_back(t, t->uBT_EE, m, m->vBT_NN, 1);
+    transpose3_back(t, t->BT_h_u, m, m->BT_h_v, nk);
+    free(vT); free(hinT); free(hT); free(vhT);
 }
 
 /* ------------------------------------------------------------ coriolis --- */
@@ -1419,6 +1493,18 @@ void om_prepare(om_model *m)
 }
 
 
+/* The meridional pass after the zonal one: its input is h after the zonal
+ * pass with refilled periodic halos; its output keeps that halo. */
+static void meridional_pass(om_model *m, const double *v, double dt)
+{
+    double *hx = (double *)malloc(m->N3 * sizeof(double));
+    memcpy(hx, m->h, m->N3 * sizeof(double));
+    if (periodic(m)) halo_fill(m, hx, m->nk);
+    memcpy(m->h, hx, m->N3 * sizeof(double));
+    om_continuity_ppm_y(m, v, hx, m->h, m->vh, dt);
+    free(hx);
+}
+
 void om_step(om_model *m, int niter)
 {
     const double dt = m->c.dt;
@@ -1435,6 +1521,7 @@ void om_step(om_model *m, int niter)
         om_btstep(m, m->eta, m->ubt, m->vbt, m->ubt_av, m->vbt_av, m->eta_av);
         if (per) { halo_fill(m, m->up, nk); halo_fill(m, m->vp, nk); }
         om_continuity_ppm(m, m->up, m->h0, m->h, m->uh, dt);
+        if (m->c.meridional) meridional_pass(m, m->vp, dt);
         if (per) halo_fill(m, m->h, nk);
         /* corrector */
         om_compute_transports(m, m->up, m->vp, m->h, m->uh, m->vh);
@@ -1448,6 +1535,7 @@ void om_step(om_model *m, int niter)
          * `i=isd,jed` at :375 is read as isd..ied; identical when ni == nj) */
         memcpy(m->htmp, m->h, m->N3 * sizeof(double));
         om_continuity_ppm(m, m->u, m->htmp, m->h, m->uh, 0.5 * dt);
+        if (m->c.meridional) meridional_pass(m, m->v, 0.5 * dt);
         if (per) {
             halo_fill(m, m->u, nk); halo_fill(m, m->v, nk); halo_fill(m, m->h, nk);
             halo_fill(m, m->eta, 1); halo_fill(m, m->ubt, 1); halo_fill(m, m->vbt, 1);
@@ -1503,6 +1591,9 @@ om_model *om_create(const om_config *c)
     R2D(taux); R2D(tauy); R3D(Ray_u); R3D(Ray_v);
     R2D(FA_u_EE); R2D(FA_u_E0); R2D(FA_u_W0); R2D(FA_u_WW); R2D(uBT_WW); R2D(uBT_EE);
     reg(m, "BT_h_u", &m->BT_h_u, N3, 0.0);
+    R3D(v_cor); R2D(dv_cor);
+    R2D(FA_v_NN); R2D(FA_v_N0); R2D(FA_v_S0); R2D(FA_v_SS); R2D(vBT_SS); R2D(vBT_NN);
+    reg(m, "BT_h_v", &m->BT_h_v, N3, 0.0);
     R3D(cont_hW); R3D(cont_hE);
     R2D(cor_dvdx); R2D(cor_dudy); R2D(cor_rel_vort); R2D(cor_abs_vort); R2D(cor_q); R2D(cor_Ih_q);
     R2D(cor_hArea_u); R2D(cor_hArea_v); R2D(cor_Area_q); R2D(cor_KE); R2D(cor_a); R2D(cor_b);
@@ -1553,6 +1644,7 @@ om_model *om_create(const om_config *c)
 void om_destroy(om_model *m)
 {
     if (!m) return;
+    om_destroy(m->mt);
     for (int q = 0; q < m->ntab; ++q) free(*m->tab[q].p);
     free(m->tab);
     free(m);

@@ -52,6 +52,8 @@ typedef struct om_config {
     int coriolis_scheme;     /* 1 Sadourny75, 2 Arakawa-Hsu90, 3 Arakawa-Lamb81 */
     int monotonic;           /* continuity limiter: 0 positive-definite, 1 CW84 */
     int simple_2nd, upwind_1st;
+    int meridional;          /* continuity also takes a meridional pass after the zonal one
+                                (no reference counterpart; see om_continuity_ppm_y) */
 } om_config;
 
 void om_config_default(om_config *c);
@@ -84,6 +86,11 @@ void om_btstep(om_model *m, const double *eta_in, const double *ubt_in, const do
                double *u_av, double *v_av, double *eta_av);
 void om_continuity_ppm(om_model *m, const double *u, const double *hin, double *h, double *uh,
                        double dt);
+/* Meridional PPM pass (no reference counterpart): the zonal pass on the
+ * transposed grid and state; v, hin -> h (compute domain), vh; v_cor, dv_cor
+ * and the BT_cont v-side fields (FA_v_*, vBT_*, BT_h_v) are stored in the model. */
+void om_continuity_ppm_y(om_model *m, const double *v, const double *hin, double *h, double *vh,
+                         double dt);
 /* periodic refill of the 3-wide halo of a (nk layers; no-op for static BCs) */
 void om_halo_fill(om_model *m, double *a, int nk);
 /* verify_state (rk2_serial_driver.F90:563-623): out[7] = mass_init, mass_final,

@@ -30,7 +30,7 @@ class OmConfig(C.Structure):
                 ("Kv", C.c_double), ("Kv_ml", C.c_double), ("Kv_extra_bbl", C.c_double),
                 ("Hmix", C.c_double), ("Hbbl", C.c_double), ("ray_bottom", C.c_double),
                 ("coriolis_scheme", C.c_int), ("monotonic", C.c_int),
-                ("simple_2nd", C.c_int), ("upwind_1st", C.c_int)]
+                ("simple_2nd", C.c_int), ("upwind_1st", C.c_int), ("meridional", C.c_int)]
 
 
 _lib = None
@@ -56,6 +56,7 @@ def lib():
         _lib.om_vert_visc.argtypes = [C.c_void_p, d, d, d, C.c_double]
         _lib.om_btstep.argtypes = [C.c_void_p, d, d, d, d, d, d]
         _lib.om_continuity_ppm.argtypes = [C.c_void_p, d, d, d, d, C.c_double]
+        _lib.om_continuity_ppm_y.argtypes = [C.c_void_p, d, d, d, d, C.c_double]
         _lib.om_halo_fill.argtypes = [C.c_void_p, d, C.c_int]
     return _lib
 
@@ -78,6 +79,7 @@ class Oracle:
         c.laplacian = int(cfg.laplacian)
         c.biharmonic = int(cfg.biharmonic)
         c.monotonic = int(cfg.monotonic)
+        c.meridional = int(getattr(cfg, "meridional", False))
         c.coriolis_mode = 0 if cfg.coriolis == "reference" else 1
         c.f0 = cfg.f0
         c.beta = cfg.beta if cfg.coriolis == "betaplane" else 0.0

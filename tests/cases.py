@@ -32,3 +32,16 @@ COMPARE = ("u", "v", "h", "htmp", "up", "vp", "uh", "vh", "CAu", "CAv", "diffu",
 COMPARE_FUSED = ("u", "v", "h", "htmp", "up", "vp", "eta", "ubt_av", "vbt_av", "eta_av", "u_cor", "du_cor",
                  "FA_u_EE", "FA_u_E0", "FA_u_W0", "FA_u_WW", "uBT_WW", "uBT_EE", "BT_h_u",
                  "vv_taux_bot", "vv_tauy_bot")
+
+# continuity with the meridional pass after the zonal one (no reference
+# counterpart; the oracle's meridional pass is its zonal pass on the
+# transposed grid)
+MERID_CASES = {
+    "merid_zonal_periodic_walls_beta": (Config(ni=28, nj=20, nk=7, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                               coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3,
+                                               meridional=True), "seeded", 3),
+    "merid_doubly_periodic_fplane": (Config(ni=24, nj=26, nk=8, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_PERIODIC,
+                                            coriolis="fplane", f0=1e-4, meridional=True), "seeded", 2),
+    "merid_static_ref": (Config(ni=20, nj=20, nk=5, bt_nsteps=10, meridional=True), "reference", 2),
+}
+MERID_COMPARE = ("v_cor", "dv_cor", "FA_v_NN", "FA_v_N0", "FA_v_S0", "FA_v_SS", "vBT_SS", "vBT_NN", "BT_h_v")

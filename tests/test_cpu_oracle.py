@@ -204,3 +204,60 @@ def test_ppm_edges_reproduce_linear_profiles():
         np.testing.assert_allclose(hW[ii - 1, jsc - 1:jec], 0.5 * (hh[ii - 2, jsc - 1:jec] + hh[ii - 1, jsc - 1:jec]), rtol=1e-15)
     # u = 0 everywhere: no flux, h unchanged exactly
     assert np.array_equal(f("h")[isc - 1:iec, jsc - 1:jec], hh[isc - 1:iec, jsc - 1:jec])
+
+
+# ---------------------------------------------------------------------------
+# The meridional pass (no reference counterpart): pinned by the properties a
+# meridional continuity pass must have.
+def _merid_setup(cfg, seed=3):
+    from oracle.pyoracle import Oracle
+    case = make_case(cfg, init="seeded", seed=seed)
+    o = Oracle(cfg, case)
+    v = np.asfortranarray(case["v"].copy())
+    hin = np.asfortranarray(case["h0"].copy())
+    return o, case, v, hin
+
+
+def test_meridional_rest_state_stays_at_rest():
+    cfg = Config(ni=20, nj=18, nk=4, bt_nsteps=5)
+    o, case, v, hin = _merid_setup(cfg)
+    v[:] = 0.0
+    h = hin.copy(order="F")
+    vh = np.full_like(hin, 7.0, order="F")
+    o.call("om_continuity_ppm_y", v, hin, h, vh, scalar=cfg.dt)
+    isd, ied, jsd, jed, isc, iec, jsc, jec = cfg.bounds
+    assert np.array_equal(h, hin)
+    assert np.all(vh[isc - 1:iec, jsc - 2:jec] == 0.0)
+
+
+def test_meridional_pass_conserves_volume_between_walls():
+    """Closed y faces carry no transport, so the column-integrated volume
+    telescopes: sum(h*area) is unchanged to round-off."""
+    cfg = Config(ni=24, nj=20, nk=5, bt_nsteps=10, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                 coriolis="betaplane", f0=1e-4, beta=2e-11)
+    o, case, v, hin = _merid_setup(cfg)
+    h = hin.copy(order="F")
+    vh = np.zeros_like(hin, order="F")
+    o.call("om_continuity_ppm_y", v, hin, h, vh, scalar=cfg.dt)
+    isd, ied, jsd, jed, isc, iec, jsc, jec = cfg.bounds
+    assert np.abs(h - hin).max() > 1e-3          # it did move mass
+    v0, v1 = hin[isc - 1:iec, jsc - 1:jec].sum(), h[isc - 1:iec, jsc - 1:jec].sum()
+    assert abs(v1 - v0) <= 1e-13 * v0
+    # wall faces (J = jsc-1 and J = jec) carry nothing
+    assert np.all(vh[isc - 1:iec, jsc - 2] == 0.0) and np.all(vh[isc - 1:iec, jec - 1] == 0.0)
+
+
+def test_meridional_pass_is_the_transposed_zonal_pass():
+    """On a square static grid, a state that is symmetric under i <-> j with
+    u(i, j) = v(j, i) gives vh(i, J) = uh(J, i) and the same thickness change."""
+    cfg = Config(ni=18, nj=18, nk=4, bt_nsteps=5)
+    o, case, v, hin = _merid_setup(cfg, seed=11)
+    hs = np.asfortranarray(0.5 * (hin + hin.transpose(1, 0, 2)))
+    vs = np.asfortranarray(v)
+    us = np.asfortranarray(vs.transpose(1, 0, 2))
+    hx, uh = hs.copy(order="F"), np.zeros_like(hs, order="F")
+    o.call("om_continuity_ppm", us, hs, hx, uh, scalar=cfg.dt)
+    hy, vh = hs.copy(order="F"), np.zeros_like(hs, order="F")
+    o.call("om_continuity_ppm_y", vs, hs, hy, vh, scalar=cfg.dt)
+    assert np.array_equal(vh, uh.transpose(1, 0, 2))
+    assert np.array_equal(hy, hx.transpose(1, 0, 2))
