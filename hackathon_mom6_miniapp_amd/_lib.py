@@ -59,6 +59,15 @@ _SIGS = {
     "mom6h_model_stage_times": ([P, D, C.c_int, C.c_char_p, C.c_int], C.c_int),
     "mom6h_model_main_stream": ([P, C.POINTER(P)], C.c_int),
     "mom6h_model_destroy": ([P], C.c_int),
+    "mom6h_model_autotune": ([P, C.c_int, C.c_char_p, C.c_int], C.c_int),
+    "mom6h_ensemble_create": ([C.POINTER(P), C.POINTER(MConfig), C.c_int], C.c_int),
+    "mom6h_ensemble_member": ([P, C.c_int, C.POINTER(P)], C.c_int),
+    "mom6h_ensemble_prepare": ([P], C.c_int),
+    "mom6h_ensemble_step": ([P, C.c_int], C.c_int),
+    "mom6h_ensemble_synchronize": ([P], C.c_int),
+    "mom6h_ensemble_set_option": ([P, C.c_char_p, C.c_double], C.c_int),
+    "mom6h_ensemble_autotune": ([P, C.c_int, C.c_char_p, C.c_int], C.c_int),
+    "mom6h_ensemble_destroy": ([P], C.c_int),
 }
 
 

@@ -18,54 +18,6 @@
 
 namespace mom6h {
 
-struct BT {
-    Dims d;
-    int nstep, first_direction, extended, px, py;
-    double dtbt, bebt, dgeo_de;
-    double *ubt_av, *vbt_av;   // running time averages, accumulated in place
-};
-
-// Metric access.  BTArrays reads every coefficient from its 2D array (the
-// general grid); BTUniform holds the coefficients that are the same at every
-// point as values and keeps only Datv (closed faces of walls) and the
-// Coriolis factors (f_4 = one value per point, equal over the four
-// components) as arrays.  The arithmetic is the same for both.
-struct BTArrays {
-    const double *Datu, *Datv, *gtot_E, *gtot_W, *gtot_N, *gtot_S, *f_4_u, *f_4_v;
-    const double *bt_rem_u, *bt_rem_v, *IareaT, *IdxCu, *IdyCv;
-    __device__ double datu(long long c) const { return Datu[c]; }
-    __device__ double datv(long long c) const { return Datv[c]; }
-    __device__ double gE(long long c) const { return gtot_E[c]; }
-    __device__ double gW(long long c) const { return gtot_W[c]; }
-    __device__ double gN(long long c) const { return gtot_N[c]; }
-    __device__ double gS(long long c) const { return gtot_S[c]; }
-    __device__ double f4u(int comp, long long c) const { return f_4_u[4 * c + (comp - 1)]; }
-    __device__ double f4v(int comp, long long c) const { return f_4_v[4 * c + (comp - 1)]; }
-    __device__ double remu(long long c) const { return bt_rem_u[c]; }
-    __device__ double remv(long long c) const { return bt_rem_v[c]; }
-    __device__ double iareaT(long long c) const { return IareaT[c]; }
-    __device__ double idxCu(long long c) const { return IdxCu[c]; }
-    __device__ double idyCv(long long c) const { return IdyCv[c]; }
-};
-
-struct BTUniform {
-    double Datu, gtot, bt_rem_u, bt_rem_v, IareaT, IdxCu, IdyCv;
-    const double *Datv, *f4;
-    __device__ double datu(long long) const { return Datu; }
-    __device__ double datv(long long c) const { return Datv[c]; }
-    __device__ double gE(long long) const { return gtot; }
-    __device__ double gW(long long) const { return gtot; }
-    __device__ double gN(long long) const { return gtot; }
-    __device__ double gS(long long) const { return gtot; }
-    __device__ double f4u(int, long long c) const { return f4[c]; }
-    __device__ double f4v(int, long long c) const { return f4[c]; }
-    __device__ double remu(long long) const { return bt_rem_u; }
-    __device__ double remv(long long) const { return bt_rem_v; }
-    __device__ double iareaT(long long) const { return IareaT; }
-    __device__ double idxCu(long long) const { return IdxCu; }
-    __device__ double idyCv(long long) const { return IdyCv; }
-};
-
 // ---------------------------------------------------------------------------
 // One substep per launch, 2D tiles staged in LDS.  A block owns the output
 // tile T = [i0, i0+TX-1] x [j0, j0+TY-1] and recomputes the few cells around
@@ -75,13 +27,6 @@ struct BTUniform {
 // are accumulated in place.
 constexpr int BT_TX = 64, BT_TY = 8;
 constexpr int BT_W = BT_TX + 3, BT_H = BT_TY + 3;
-
-struct BTStep {
-    const double *eta_o, *ubt_o, *vbt_o;   // state before the substep
-    double *eta_n, *ubt_n, *vbt_n;         // state after it
-    double *out_u, *out_v, *out_eta;       // written on the last substep only
-    int n;                                 // substep number, 1-based
-};
 
 __global__ void k_bt_copy_in(Dims d, const double *eta_in, const double *ubt_in, const double *vbt_in,
                              double *eta, double *ubt, double *vbt, double *ubt_av, double *vbt_av)
@@ -96,15 +41,14 @@ __global__ void k_bt_copy_in(Dims d, const double *eta_in, const double *ubt_in,
 }
 
 template <typename M>
-__global__ void __launch_bounds__(256) k_bt_substep(BT p, M m, BTStep st)
+__device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &st, int tile_x, int tile_y)
 {
     __shared__ double s_eta[BT_H][BT_W], s_u[BT_H][BT_W], s_v[BT_H][BT_W];
     __shared__ double s_ep[BT_H][BT_W], s_un[BT_H][BT_W], s_vn[BT_H][BT_W];
     __shared__ double s_PFu[BT_H][BT_W], s_PFv[BT_H][BT_W];
     const Dims &d = p.d;
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
-    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int i0 = d.isd + (tile % gridDim.x) * BT_TX, j0 = d.jsd + (tile / gridDim.x) * BT_TY;
+    const int i0 = d.isd + tile_x * BT_TX, j0 = d.jsd + tile_y * BT_TY;
     const int i1 = i0 + BT_TX - 1, j1 = j0 + BT_TY - 1;
     const int bi = i0 - 2, bj = j0 - 2;  // LDS origin
     const int tid = threadIdx.x;
@@ -255,6 +199,90 @@ __global__ void __launch_bounds__(256) k_bt_substep(BT p, M m, BTStep st)
             st.out_eta[c] = e;
         }
     }
+}
+
+template <typename M>
+__global__ void __launch_bounds__(256) k_bt_substep(BT p, M m, BTStep st)
+{
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    bt_body<M>(p, m, st, tile % gridDim.x, tile / gridDim.x);
+}
+
+// ---- the ensemble form: every member's substep n in one launch.  Member
+// state ping-pongs between A (after the copy-in: odd substeps read it) and B.
+__global__ void k_bt_copy_in_batch(const BTMember *__restrict__ args, BatchMap mp)
+{
+    int m, local;
+    batch_locate(mp, blockIdx.x, m, local);
+    const BTMember &a = args[m];
+    const long long q = (long long)local * 256 + threadIdx.x;
+    if (q >= a.p.d.s2) return;
+    a.A[0][q] = a.in[0][q];
+    a.A[1][q] = a.in[1][q];
+    a.A[2][q] = a.in[2][q];
+    a.p.ubt_av[q] = 0.0;
+    a.p.vbt_av[q] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_bt_substep_batch(const BTMember *__restrict__ args, BatchMap mp, int n)
+{
+    int m, local;
+    batch_locate(mp, xcd_tile(blockIdx.x, gridDim.x), m, local);
+    const BTMember a = args[m];
+    const bool odd = (n % 2) == 1;
+    BTStep st;
+    st.eta_o = odd ? a.A[0] : a.B[0]; st.ubt_o = odd ? a.A[1] : a.B[1]; st.vbt_o = odd ? a.A[2] : a.B[2];
+    st.eta_n = odd ? a.B[0] : a.A[0]; st.ubt_n = odd ? a.B[1] : a.A[1]; st.vbt_n = odd ? a.B[2] : a.A[2];
+    st.out_u = a.out[0]; st.out_v = a.out[1]; st.out_eta = a.out[2];
+    st.n = n;
+    bt_body<BTUniform>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
+}
+
+// btstep of every member (all with nstep substeps and uniform coefficients)
+int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s)
+{
+    MOM6H_REQUIRE(n >= 1 && n <= kMaxMembers, "btstep batch: 1..%d members", kMaxMembers);
+    const int nstep = h_args[0].p.nstep;
+    BatchMap mc{}, mt{};
+    mc.n = mt.n = n;
+    for (int m = 0; m < n; ++m) {
+        const Dims &d = h_args[m].p.d;
+        MOM6H_REQUIRE(h_args[m].p.nstep == nstep, "btstep batch: members differ in bt_nsteps");
+        mc.nbx[m] = (int)((d.s2 + 255) / 256);
+        mc.first[m + 1] = mc.first[m] + mc.nbx[m];
+        mt.nbx[m] = (d.n1 + BT_TX - 1) / BT_TX;
+        mt.nby[m] = (d.n2 + BT_TY - 1) / BT_TY;
+        mt.first[m + 1] = mt.first[m] + mt.nbx[m] * mt.nby[m];
+    }
+    k_bt_copy_in_batch<<<(unsigned)mc.first[n], 256, 0, s>>>(d_args, mc);
+    MOM6H_LAUNCH_CHECK();
+    for (int q = 1; q <= nstep; ++q) {
+        k_bt_substep_batch<<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q);
+        MOM6H_LAUNCH_CHECK();
+    }
+    return MOM6H_OK;
+}
+
+// The member record of one btstep call (inputs in, outputs out) on a uniform grid
+int bt_member(const mom6h_barotropic_cs *cs, const double *eta_in, const double *ubt_in, const double *vbt_in,
+              double *u_av, double *v_av, double *eta_av, BTMember *out)
+{
+    MOM6H_REQUIRE(cs->uniform, "btstep batch: members need uniform barotropic coefficients");
+    BTMember a{};
+    a.p.d = cs->d;
+    a.p.nstep = cs->nstep; a.p.first_direction = cs->first_direction;
+    a.p.px = cs->halo.px; a.p.py = cs->halo.py;
+    a.p.extended = (a.p.px || a.p.py) ? 1 : 0;
+    a.p.dtbt = cs->dtbt; a.p.bebt = cs->bebt; a.p.dgeo_de = cs->dgeo_de;
+    a.p.ubt_av = cs->ubt_av; a.p.vbt_av = cs->vbt_av;
+    a.m = BTUniform{cs->u_Datu, cs->u_gtot, cs->u_rem_u, cs->u_rem_v, cs->u_IareaT, cs->u_IdxCu, cs->u_IdyCv,
+                    cs->Datv, cs->f4};
+    a.A[0] = cs->eta; a.A[1] = cs->ubt; a.A[2] = cs->vbt;
+    a.B[0] = cs->eta_b; a.B[1] = cs->ubt_b; a.B[2] = cs->vbt_b;
+    a.in[0] = eta_in; a.in[1] = ubt_in; a.in[2] = vbt_in;
+    a.out[0] = u_av; a.out[1] = v_av; a.out[2] = eta_av;
+    *out = a;
+    return MOM6H_OK;
 }
 
 }  // namespace mom6h

@@ -347,27 +347,6 @@ __global__ void k_mass_flux(Dims d, const double *__restrict__ u, const double *
 // identically and only one writes.
 constexpr int CF = 64;
 
-// Arguments of one pass.  Names are those of the zonal pass; the meridional
-// pass passes v, vh, ... and its own metrics in the same slots:
-//   IdxT -> IdyT, dy_Cu -> dx_Cv, dxT -> dyT, dxCu -> dyCv, mask2dCu -> mask2dCv.
-// T is the type of the 3D layer fields and of the flux arithmetic: double
-// (the reference's FP64) or float (the mixed-precision form); metrics, the
-// 2D BT_cont outputs and du_cor stay FP64 in memory.
-template <typename T>
-struct ContArgsT {
-    Dims d;
-    const T *u, *hin;
-    T *h, *uh;
-    double dt;
-    const double *uhbt;
-    const double *IareaT, *IdxT, *dy_Cu, *dxT, *dxCu, *mask2dCu, *mask2dT;
-    double tol_eta0, tol_vel, CFL_limit_adjust, h_min;
-    T *u_cor;
-    double *du_cor;
-    double *FA_u_EE, *FA_u_E0, *FA_u_W0, *FA_u_WW, *uBT_WW, *uBT_EE;
-    T *bt_h_u;
-};
-using ContArgs = ContArgsT<double>;
 
 __device__ __forceinline__ double vabs(double x) { return fabs(x); }
 __device__ __forceinline__ float vabs(float x) { return fabsf(x); }
@@ -591,21 +570,19 @@ __device__ __forceinline__ float col_sum(const float (*col)[CF], int nk, int lan
 // the first pass); the FP64 tolerance of 1e-12 m lies below what FP32 sums
 // resolve, and the iteration would otherwise run to its 20-pass limit.
 template <typename T, bool Y, int CW, int K>
-__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(1, (CW >= 16) ? 4 : 2)))
-k_continuity_blk(ContArgsT<T> a, int monotonic)
+__device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monotonic, int bx, int by, double *smem_raw)
 {
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int L = ContShared<T, CW, K>::L;
-    extern __shared__ double smem_raw[];
     ContShared<T, CW, K> &S = *reinterpret_cast<ContShared<T, CW, K> *>(smem_raw);
     const Dims &d = a.d;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     // face (fi, fj) between cell 0 = (fi, fj) and cell 1 = (fi+1, fj) [x] or (fi, fj+1) [y]
-    const int f_raw = Y ? d.isc + blockIdx.x * CF + lane : d.isc - 1 + blockIdx.x * (CF - 1) + lane;
+    const int f_raw = Y ? d.isc + bx * CF + lane : d.isc - 1 + bx * (CF - 1) + lane;
     const int f_max = d.iec;
     const int fi = f_raw > f_max ? f_max : f_raw;           // clamped: evaluated, never written
-    const int fj = Y ? d.jsc - 1 + (int)blockIdx.y : d.jsc + (int)blockIdx.y;
+    const int fj = Y ? d.jsc - 1 + by : d.jsc + by;
     const bool live = f_raw <= f_max;
     auto cell = [&](int q) __attribute__((always_inline)) -> long long {   // q-th cell of the stencil, q = 2: cell 0
         return Y ? IX2(d, fi, fj - 2 + q) : IX2(d, fi - 2 + q, fj);
@@ -774,7 +751,7 @@ k_continuity_blk(ContArgsT<T> a, int monotonic)
 
     // outputs at the face: uh, u_cor, du_cor; a face shared with the western
     // neighbour block is written by that block
-    const bool writes_face = live && (Y || lane > 0 || blockIdx.x == 0);
+    const bool writes_face = live && (Y || lane > 0 || bx == 0);
     if (writes_face) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
@@ -872,17 +849,38 @@ k_continuity_blk(ContArgsT<T> a, int monotonic)
                                 : (T(1.5) * (duR - du0)) * ((FAmt_R - FA_avg) / (FAmt_R - FA_0)));
 }
 
+template <typename T, bool Y, int CW, int K>
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(1, (CW >= 16) ? 4 : 2)))
+k_continuity_blk(ContArgsT<T> a, int monotonic)
+{
+    extern __shared__ double smem_raw[];
+    continuity_body<T, Y, CW, K>(a, monotonic, blockIdx.x, blockIdx.y, smem_raw);
+}
+
+// every member of an ensemble in one launch (K for the deepest member)
+template <typename T, bool Y, int CW, int K>
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(1, (CW >= 16) ? 4 : 2)))
+k_continuity_batch(const ContArgsT<T> *__restrict__ args, BatchMap mp)
+{
+    extern __shared__ double smem_raw[];
+    int m, local;
+    batch_locate(mp, blockIdx.x, m, local);
+    const ContArgsT<T> a = args[m];
+    continuity_body<T, Y, CW, K>(a, a.monotonic, local % mp.nbx[m], local / mp.nbx[m], smem_raw);
+}
+
 // the driver's `h = h0` reset for the cells the convergence does not write:
 // rows outside jsc..jec in full, and the isd..isc-1 / iec+1..ied strips of the
 // compute rows (one thread per halo cell of a layer)
 template <typename T>
-__global__ void k_halo_copy(Dims d, const T *__restrict__ hin, T *__restrict__ h)
+__device__ __forceinline__ void halo_copy_body(const Dims &d, const T *__restrict__ hin, T *__restrict__ h, int bx,
+                                               int bz)
 {
-    const int k = 1 + blockIdx.y;
+    const int k = 1 + bz;
     const int rows_out = (d.jsc - d.jsd) + (d.jed - d.jec);
     const int side = (d.isc - d.isd) + (d.ied - d.iec);
     const long long n_rows = (long long)rows_out * d.n1;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)bx * 256 + threadIdx.x;
     int i, j;
     if (t < n_rows) {
         const int r = (int)(t / d.n1);
@@ -899,19 +897,59 @@ __global__ void k_halo_copy(Dims d, const T *__restrict__ hin, T *__restrict__ h
     h[o] = hin[o];
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_halo_copy(Dims d, const T *__restrict__ hin, T *__restrict__ h)
+{
+    halo_copy_body(d, hin, h, blockIdx.x, blockIdx.y);
+}
+
+// halo cells of a layer of the grid
+inline long long halo_cells(const Dims &d)
+{
+    return (long long)((d.jsc - d.jsd) + (d.jed - d.jec)) * d.n1 +
+           (long long)((d.isc - d.isd) + (d.ied - d.iec)) * (d.jec - d.jsc + 1);
+}
+
 // The meridional counterpart of continuity_zonal_convergence:
 // h(i, j) = max(hin - dt*IareaT*(vh(i, J=j) - vh(i, J=j-1)), 0) on the compute domain.
 template <typename T>
-__global__ void k_convergence_y(Dims d, const T *hin, const T *__restrict__ vh, double dt,
-                                const double *__restrict__ IareaT, T *h)
+__device__ __forceinline__ void conv_y_body(const Dims &d, const T *hin, const T *__restrict__ vh, double dt,
+                                            const double *__restrict__ IareaT, T *h, int bx, int by, int bz)
 {
-    const int i = d.isc + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = d.jsc + blockIdx.y;
-    const int k = 1 + blockIdx.z;
+    const int i = d.isc + bx * 128 + threadIdx.x;
+    const int j = d.jsc + by;
+    const int k = 1 + bz;
     if (i > d.iec) return;
     const long long c = IX3(d, i, j, k);
     const T dtIareaT = (T)(dt * IareaT[IX2(d, i, j)]);
     h[c] = vmax(hin[c] - dtIareaT * (vh[c] - vh[c - d.n1]), T(0.0));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128) k_convergence_y(Dims d, const T *hin, const T *__restrict__ vh, double dt,
+                                                       const double *__restrict__ IareaT, T *h)
+{
+    conv_y_body(d, hin, vh, dt, IareaT, h, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// the ensemble forms: one launch over every member (blocks flattened x, y, k)
+template <typename T>
+__global__ void __launch_bounds__(256) k_halo_copy_batch(const ContArgsT<T> *__restrict__ args, BatchMap mp)
+{
+    int m, local;
+    batch_locate(mp, blockIdx.x, m, local);
+    const ContArgsT<T> &a = args[m];
+    halo_copy_body(a.d, a.hin, a.h, local % mp.nbx[m], local / mp.nbx[m]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128) k_convergence_y_batch(const ContArgsT<T> *__restrict__ args, BatchMap mp)
+{
+    int m, local;
+    batch_locate(mp, blockIdx.x, m, local);
+    const ContArgsT<T> &a = args[m];
+    const int r = local / mp.nbx[m];
+    conv_y_body(a.d, a.hin, a.uh, a.dt, a.IareaT, a.h, local % mp.nbx[m], r % mp.nby[m], r / mp.nby[m]);
 }
 
 // continuity_zonal_convergence (:148-176) with hin; optionally copies hin into
@@ -957,18 +995,22 @@ int set_blk_lds(long long max_lds)
                           (const void *)k_continuity_blk<T, true, 8, 4>, (const void *)k_continuity_blk<T, true, 8, 10>};
     const int bytes[] = {(int)sizeof(ContShared<T, 8, 1>), (int)sizeof(ContShared<T, 8, 2>),
                          (int)sizeof(ContShared<T, 8, 4>), (int)sizeof(ContShared<T, 8, 10>)};
+    const void *kb[] = {(const void *)k_continuity_batch<T, false, 8, 1>, (const void *)k_continuity_batch<T, false, 8, 2>,
+                        (const void *)k_continuity_batch<T, false, 8, 4>, (const void *)k_continuity_batch<T, false, 8, 10>,
+                        (const void *)k_continuity_batch<T, true, 8, 1>, (const void *)k_continuity_batch<T, true, 8, 2>,
+                        (const void *)k_continuity_batch<T, true, 8, 4>, (const void *)k_continuity_batch<T, true, 8, 10>};
     for (int q = 0; q < 8; ++q)
-        if (bytes[q % 4] <= max_lds)
+        if (bytes[q % 4] <= max_lds) {
             MOM6H_CHECK(hipFuncSetAttribute(kern[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q % 4]));
+            MOM6H_CHECK(hipFuncSetAttribute(kb[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q % 4]));
+        }
     return MOM6H_OK;
 }
 
 template <typename T>
 void launch_halo_copy(const Dims &d, const T *hin, T *h, hipStream_t s)
 {
-    const long long cells = (long long)((d.jsc - d.jsd) + (d.jed - d.jec)) * d.n1 +
-                            (long long)((d.isc - d.isd) + (d.ied - d.iec)) * (d.jec - d.jsc + 1);
-    const dim3 bh(256), gh((unsigned)((cells + 255) / 256), d.nk);
+    const dim3 bh(256), gh((unsigned)((halo_cells(d) + 255) / 256), d.nk);
     k_halo_copy<T><<<gh, bh, 0, s>>>(d, hin, h);
 }
 
@@ -1025,6 +1067,98 @@ int continuity_blk_y(mom6h_continuity_cs *cs, const T *v, const T *hin, T *h, T 
     }
     return MOM6H_OK;
 }
+
+// One pass (zonal, or meridional with its convergence) of every member of an
+// ensemble: one launch per kernel.  h_args mirrors d_args on the host.
+template <typename T>
+int continuity_batch(const ContArgsT<T> *d_args, const ContArgsT<T> *h_args, int n, bool y_pass, bool halo_copy,
+                     long long max_lds, hipStream_t s)
+{
+    MOM6H_REQUIRE(n >= 1 && n <= kMaxMembers, "continuity batch: 1..%d members", kMaxMembers);
+    BatchMap mp{};
+    mp.n = n;
+    int nk_max = 1;
+    for (int m = 0; m < n; ++m) {
+        const Dims &d = h_args[m].d;
+        mp.nbx[m] = y_pass ? (d.iec - d.isc + 1 + CF - 1) / CF : (d.iec - d.isc + 1 + CF - 2) / (CF - 1);
+        mp.nby[m] = y_pass ? d.jec - d.jsc + 2 : d.jec - d.jsc + 1;
+        mp.first[m + 1] = mp.first[m] + mp.nbx[m] * mp.nby[m];
+        nk_max = d.nk > nk_max ? d.nk : nk_max;
+    }
+    MOM6H_REQUIRE(nk_max <= 80 && (long long)sizeof(ContShared<T, 8, 10>) <= max_lds, "continuity batch: nk <= 80");
+    const unsigned nb = (unsigned)mp.first[n];
+    const dim3 b(64 * 8);
+#define CB(Y)                                                                                                  \
+    if (nk_max <= 8) k_continuity_batch<T, Y, 8, 1><<<nb, b, sizeof(ContShared<T, 8, 1>), s>>>(d_args, mp);     \
+    else if (nk_max <= 16) k_continuity_batch<T, Y, 8, 2><<<nb, b, sizeof(ContShared<T, 8, 2>), s>>>(d_args, mp); \
+    else if (nk_max <= 32) k_continuity_batch<T, Y, 8, 4><<<nb, b, sizeof(ContShared<T, 8, 4>), s>>>(d_args, mp); \
+    else k_continuity_batch<T, Y, 8, 10><<<nb, b, sizeof(ContShared<T, 8, 10>), s>>>(d_args, mp);
+    if (y_pass) { CB(true) } else { CB(false) }
+#undef CB
+    MOM6H_LAUNCH_CHECK();
+    if (y_pass) {
+        BatchMap mc{};
+        mc.n = n;
+        for (int m = 0; m < n; ++m) {
+            const Dims &d = h_args[m].d;
+            mc.nbx[m] = (d.iec - d.isc + 1 + 127) / 128;
+            mc.nby[m] = d.jec - d.jsc + 1;
+            mc.first[m + 1] = mc.first[m] + mc.nbx[m] * mc.nby[m] * d.nk;
+        }
+        k_convergence_y_batch<T><<<(unsigned)mc.first[n], 128, 0, s>>>(d_args, mc);
+        MOM6H_LAUNCH_CHECK();
+    }
+    if (halo_copy) {
+        BatchMap mh{};
+        mh.n = n;
+        for (int m = 0; m < n; ++m) {
+            const Dims &d = h_args[m].d;
+            mh.nbx[m] = (int)((halo_cells(d) + 255) / 256);
+            mh.nby[m] = d.nk;
+            mh.first[m + 1] = mh.first[m] + mh.nbx[m] * d.nk;
+        }
+        k_halo_copy_batch<T><<<(unsigned)mh.first[n], 256, 0, s>>>(d_args, mh);
+        MOM6H_LAUNCH_CHECK();
+    }
+    return MOM6H_OK;
+}
+
+template int continuity_batch<float>(const ContArgsT<float> *, const ContArgsT<float> *, int, bool, bool, long long,
+                                     hipStream_t);
+template int continuity_batch<double>(const ContArgsT<double> *, const ContArgsT<double> *, int, bool, bool, long long,
+                                      hipStream_t);
+
+// The arguments the single-grid block kernels receive (for the ensemble tables)
+template <typename T>
+ContArgsT<T> continuity_args(const mom6h_continuity_cs *cs, bool y_pass, const T *u, const T *hin, T *h, T *uh,
+                             double dt, const mom6h_bt_cont *bt, T *bt_h_u, T *u_cor, double *du_cor)
+{
+    ContArgsT<T> a{};
+    a.d = cs->d; a.u = u; a.hin = hin; a.h = h; a.uh = uh; a.dt = dt; a.uhbt = nullptr;
+    a.IareaT = cs->IareaT;
+    a.IdxT = y_pass ? cs->IdyT : cs->IdxT;
+    a.dy_Cu = y_pass ? cs->dx_Cv : cs->dy_Cu;
+    a.dxT = y_pass ? cs->dyT : cs->dxT;
+    a.dxCu = y_pass ? cs->dyCv : cs->dxCu;
+    a.mask2dCu = y_pass ? cs->mask2dCv : cs->mask2dCu;
+    a.mask2dT = cs->mask2dT;
+    a.tol_eta0 = cs->tol_eta; a.tol_vel = cs->tol_vel; a.CFL_limit_adjust = cs->CFL_limit_adjust;
+    a.h_min = 2.0 * 1.0e-10;
+    a.u_cor = u_cor; a.du_cor = du_cor;
+    if (bt) {
+        a.FA_u_EE = bt->FA_u_EE; a.FA_u_E0 = bt->FA_u_E0; a.FA_u_W0 = bt->FA_u_W0; a.FA_u_WW = bt->FA_u_WW;
+        a.uBT_WW = bt->uBT_WW; a.uBT_EE = bt->uBT_EE; a.bt_h_u = bt_h_u;
+    }
+    a.monotonic = cs->monotonic;
+    return a;
+}
+
+template ContArgsT<float> continuity_args<float>(const mom6h_continuity_cs *, bool, const float *, const float *,
+                                                 float *, float *, double, const mom6h_bt_cont *, float *, float *,
+                                                 double *);
+template ContArgsT<double> continuity_args<double>(const mom6h_continuity_cs *, bool, const double *, const double *,
+                                                   double *, double *, double, const mom6h_bt_cont *, double *,
+                                                   double *, double *);
 
 int continuity_f32(mom6h_continuity_cs *cs, const float *u, const float *hin, float *h, float *uh, double dt,
                    const mom6h_bt_cont *bt, float *bt_h_u, float *u_cor, double *du_cor, int hin_halo_to_h,

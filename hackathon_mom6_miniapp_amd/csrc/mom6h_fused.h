@@ -53,8 +53,13 @@ struct MomArgsT {
 };
 using MomArgs = MomArgsT<double>;
 
+// One grid (tile height ty = 4, 8 or 16 rows), or every member of an ensemble
+// in one launch per kernel (d_args: the members' arguments on the device,
+// h_args: the same on the host)
 template <typename T>
-int launch_momentum(const MomArgsT<T> &a, hipStream_t s);
+int launch_momentum(const MomArgsT<T> &a, hipStream_t s, int ty = 8);
+template <typename T>
+int launch_momentum_batch(const MomArgsT<T> *d_args, const MomArgsT<T> *h_args, int n, hipStream_t s, int ty = 8);
 
 // true and the common value when every element of a[0..n) is identical
 inline bool uniform_value(const double *a, size_t n, double *v)

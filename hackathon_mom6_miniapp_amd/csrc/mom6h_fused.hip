@@ -22,8 +22,10 @@
 
 namespace mom6h {
 
-constexpr int MT_X = 64, MT_Y = 8;             // output tile
-constexpr int MW = MT_X + 2, MH = MT_Y + 2;    // staged region [i0-1, i1+1] x [j0-1, j1+1]
+// Output tile MT_X x TY (TY = 4, 8 or 16 rows, chosen at launch); the staged
+// region is [i0-1, i1+1] x [j0-1, j1+1]
+constexpr int MT_X = 64;
+constexpr int MW = MT_X + 2;
 
 // Biharmonic viscosity, first half: Del2u on u points I = is-1..ie,
 // j = js..je and Del2v on v points i = is..ie, J = js-1..je
@@ -31,13 +33,13 @@ constexpr int MW = MT_X + 2, MH = MT_Y + 2;    // staged region [i0-1, i1+1] x [
 // at allocation and never written outside those ranges, which is what the
 // reference's second half reads there.
 template <typename T>
-__global__ void __launch_bounds__(256) k_del2(MomArgsT<T> a)
+__device__ __forceinline__ void del2_body(const MomArgsT<T> &a, int bx, int by, int bz)
 {
     const Dims &d = a.d;
     const UMetT<T> g(a.g);
-    const int i = d.isc - 1 + blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = d.jsc - 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = 1 + blockIdx.z;
+    const int i = d.isc - 1 + bx * 64 + (threadIdx.x & 63);
+    const int j = d.jsc - 1 + by * 4 + (threadIdx.x >> 6);
+    const int k = 1 + bz;
     if (i > d.iec || j > d.jec) return;
     auto U = [&](int ii, int jj) -> T { return a.u[IX3(d, ii, jj, k)]; };
     auto V = [&](int ii, int jj) -> T { return a.v[IX3(d, ii, jj, k)]; };
@@ -61,29 +63,46 @@ __global__ void __launch_bounds__(256) k_del2(MomArgsT<T> a)
                      g.Idx2dyCv * ((g.dx2h * sh_xx(i, j + 1)) - (g.dx2h * sh_xx(i, j)));
 }
 
-// BIH: the biharmonic stress of the Del2 fields staged like u and v, added
-// to the Laplacian stress (:824-886, :979-1036).
-template <typename T, bool BIH>
-__global__ void __launch_bounds__(256) k_momentum(MomArgsT<T> a)
+template <typename T>
+__global__ void __launch_bounds__(256) k_del2(MomArgsT<T> a)
 {
+    del2_body(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_del2_batch(const MomArgsT<T> *__restrict__ args, BatchMap mp)
+{
+    int m, local;
+    batch_locate(mp, blockIdx.x, m, local);
+    const int r = local / mp.nbx[m];
+    const MomArgsT<T> a = args[m];
+    del2_body(a, local % mp.nbx[m], r % mp.nby[m], r / mp.nby[m]);
+}
+
+// The tile (tile_x, tile_y) of one grid.  BIH: the biharmonic stress of the
+// Del2 fields staged like u and v, added to the Laplacian stress
+// (:824-886, :979-1036).
+template <typename T, bool BIH, int TY>
+__device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, int tile_y)
+{
+    constexpr int MH = TY + 2, RPT = TY / 4;   // staged rows; output rows per thread
     __shared__ T s_u[MH][MW], s_v[MH][MW], s_h[MH][MW];
     __shared__ T s_q[MH][MW], s_KE[MH][MW];
     __shared__ T s_d2u[BIH ? MH : 1][BIH ? MW : 1], s_d2v[BIH ? MH : 1][BIH ? MW : 1];
     const Dims &d = a.d;
     const UMetT<T> g(a.g);
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
-    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int i0 = is + (tile % gridDim.x) * MT_X, j0 = js + (tile / gridDim.x) * MT_Y;
+    const int i0 = is + tile_x * MT_X, j0 = js + tile_y * TY;
     const int bi = i0 - 1, bj = j0 - 1;
     const int tid = threadIdx.x;
-    const int tx = tid % MT_X, ty = tid / MT_X;          // 64 x 4 threads, two rows each
+    const int tx = tid % MT_X, ty = tid / MT_X;          // 64 x 4 threads, RPT rows each
 
     // u, v, h of the staged region [i0-1, i1+1] x [j0-1, j1+1] of one layer go
     // through registers: the next layer's loads are issued before the current
     // layer is computed (NS points per thread), and so are the base velocities
-    // of the thread's two output rows
+    // of the thread's output rows
     constexpr int NS = (MW * MH + 255) / 256;
-    T pu[NS], pv[NS], ph[NS], pbu[2], pbv[2];
+    T pu[NS], pv[NS], ph[NS], pbu[RPT], pbv[RPT];
     T p2u[BIH ? NS : 1], p2v[BIH ? NS : 1];
     auto load_layer = [&](int k) {
 #pragma unroll
@@ -99,7 +118,7 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgsT<T> a)
             }
         }
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < RPT; ++r) {
             const int i = i0 + tx, j = j0 + ty + 4 * r;
             pbu[r] = T(0.0); pbv[r] = T(0.0);
             if (i <= ie && j <= je) {
@@ -119,7 +138,9 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgsT<T> a)
                 if (BIH) { s_d2u[BIH ? lj : 0][BIH ? li : 0] = p2u[BIH ? r : 0]; s_d2v[BIH ? lj : 0][BIH ? li : 0] = p2v[BIH ? r : 0]; }
             }
         }
-        const T bu_r[2] = {pbu[0], pbu[1]}, bv_r[2] = {pbv[0], pbv[1]};
+        T bu_r[RPT], bv_r[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) { bu_r[r] = pbu[r]; bv_r[r] = pbv[r]; }
         __syncthreads();
         if (k < d.nk) load_layer(k + 1);
         // q on q points [i0-1, i1] x [j0-1, j1] (mom6_coriolis_serial.F90:134-168)
@@ -147,7 +168,7 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgsT<T> a)
         }
         __syncthreads();
 #pragma unroll 1
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < RPT; ++r) {
             const int i = i0 + tx, j = j0 + ty + 4 * r;
             if (i > ie || j > je) continue;
             const int li = i - bi, lj = j - bj;
@@ -210,23 +231,92 @@ __global__ void __launch_bounds__(256) k_momentum(MomArgsT<T> a)
     }
 }
 
+template <typename T, bool BIH, int TY>
+__global__ void __launch_bounds__(256) k_momentum(MomArgsT<T> a)
+{
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    momentum_body<T, BIH, TY>(a, tile % gridDim.x, tile / gridDim.x);
+}
+
+// every member's tiles in one launch, renumbered per XCD over the whole batch
+template <typename T, bool BIH, int TY>
+__global__ void __launch_bounds__(256) k_momentum_batch(const MomArgsT<T> *__restrict__ args, BatchMap mp)
+{
+    int m, local;
+    batch_locate(mp, xcd_tile(blockIdx.x, gridDim.x), m, local);
+    const MomArgsT<T> a = args[m];
+    momentum_body<T, BIH, TY>(a, local % mp.nbx[m], local / mp.nbx[m]);
+}
+
+template <typename T, bool BIH>
+void momentum_launch_ty(const MomArgsT<T> &a, int ty, dim3 grid, hipStream_t s)
+{
+    if (ty == 4) k_momentum<T, BIH, 4><<<grid, 256, 0, s>>>(a);
+    else if (ty == 16) k_momentum<T, BIH, 16><<<grid, 256, 0, s>>>(a);
+    else k_momentum<T, BIH, 8><<<grid, 256, 0, s>>>(a);
+}
+
 template <typename T>
-int launch_momentum(const MomArgsT<T> &a, hipStream_t s)
+int launch_momentum(const MomArgsT<T> &a, hipStream_t s, int ty)
 {
     const Dims &d = a.d;
+    MOM6H_REQUIRE(ty == 4 || ty == 8 || ty == 16, "momentum: tile height %d not in {4, 8, 16}", ty);
     if (a.del2u) {
         dim3 g2((d.iec - d.isc + 2 + 63) / 64, (d.jec - d.jsc + 2 + 3) / 4, d.nk);
         k_del2<T><<<g2, dim3(256), 0, s>>>(a);
         MOM6H_LAUNCH_CHECK();
     }
-    dim3 grid((d.iec - d.isc + 1 + MT_X - 1) / MT_X, (d.jec - d.jsc + 1 + MT_Y - 1) / MT_Y), block(256);
-    if (a.del2u) k_momentum<T, true><<<grid, block, 0, s>>>(a);
-    else k_momentum<T, false><<<grid, block, 0, s>>>(a);
+    dim3 grid((d.iec - d.isc + 1 + MT_X - 1) / MT_X, (d.jec - d.jsc + 1 + ty - 1) / ty);
+    if (a.del2u) momentum_launch_ty<T, true>(a, ty, grid, s);
+    else momentum_launch_ty<T, false>(a, ty, grid, s);
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
 }
 
-template int launch_momentum<double>(const MomArgsT<double> &, hipStream_t);
-template int launch_momentum<float>(const MomArgsT<float> &, hipStream_t);
+template <typename T, bool BIH>
+void momentum_batch_ty(const MomArgsT<T> *d_args, const BatchMap &mp, int ty, hipStream_t s)
+{
+    const unsigned nb = (unsigned)mp.first[mp.n];
+    if (ty == 4) k_momentum_batch<T, BIH, 4><<<nb, 256, 0, s>>>(d_args, mp);
+    else if (ty == 16) k_momentum_batch<T, BIH, 16><<<nb, 256, 0, s>>>(d_args, mp);
+    else k_momentum_batch<T, BIH, 8><<<nb, 256, 0, s>>>(d_args, mp);
+}
+
+template <typename T>
+int launch_momentum_batch(const MomArgsT<T> *d_args, const MomArgsT<T> *h_args, int n, hipStream_t s, int ty)
+{
+    MOM6H_REQUIRE(n >= 1 && n <= kMaxMembers, "momentum batch: 1..%d members", kMaxMembers);
+    MOM6H_REQUIRE(ty == 4 || ty == 8 || ty == 16, "momentum: tile height %d not in {4, 8, 16}", ty);
+    const bool bih = h_args[0].del2u != nullptr;
+    for (int m = 1; m < n; ++m)
+        MOM6H_REQUIRE((h_args[m].del2u != nullptr) == bih, "momentum batch: members differ in biharmonic");
+    BatchMap mp{};
+    mp.n = n;
+    if (bih) {
+        for (int m = 0; m < n; ++m) {
+            const Dims &d = h_args[m].d;
+            mp.nbx[m] = (d.iec - d.isc + 2 + 63) / 64;
+            mp.nby[m] = (d.jec - d.jsc + 2 + 3) / 4;
+            mp.first[m + 1] = mp.first[m] + mp.nbx[m] * mp.nby[m] * d.nk;
+        }
+        k_del2_batch<T><<<(unsigned)mp.first[n], 256, 0, s>>>(d_args, mp);
+        MOM6H_LAUNCH_CHECK();
+    }
+    for (int m = 0; m < n; ++m) {
+        const Dims &d = h_args[m].d;
+        mp.nbx[m] = (d.iec - d.isc + 1 + MT_X - 1) / MT_X;
+        mp.nby[m] = (d.jec - d.jsc + 1 + ty - 1) / ty;
+        mp.first[m + 1] = mp.first[m] + mp.nbx[m] * mp.nby[m];
+    }
+    if (bih) momentum_batch_ty<T, true>(d_args, mp, ty, s);
+    else momentum_batch_ty<T, false>(d_args, mp, ty, s);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
+}
+
+template int launch_momentum<double>(const MomArgsT<double> &, hipStream_t, int);
+template int launch_momentum<float>(const MomArgsT<float> &, hipStream_t, int);
+template int launch_momentum_batch<double>(const MomArgsT<double> *, const MomArgsT<double> *, int, hipStream_t, int);
+template int launch_momentum_batch<float>(const MomArgsT<float> *, const MomArgsT<float> *, int, hipStream_t, int);
 
 }  // namespace mom6h

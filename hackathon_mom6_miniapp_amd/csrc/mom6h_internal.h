@@ -55,6 +55,25 @@ __device__ inline int xcd_tile(int b, int nb)
     return x < r ? x * (m + 1) + q : r * (m + 1) + (x - r) * m + q;
 }
 
+// Several grids (the members of an ensemble) in one launch: the blocks of
+// member m are first[m] .. first[m+1]-1, laid out as that member's own
+// launch would be (nbx x nby tiles, x fastest, then any further dimension).
+constexpr int kMaxMembers = 64;
+struct BatchMap {
+    int n;                          // members
+    int first[kMaxMembers + 1];     // first block of each member; first[n] = total
+    int nbx[kMaxMembers], nby[kMaxMembers];
+};
+
+// member of global block b and the block's index within that member
+__device__ inline void batch_locate(const BatchMap &mp, int b, int &m, int &local)
+{
+    int q = 0;
+    while (q + 1 < mp.n && b >= mp.first[q + 1]) ++q;
+    m = q;
+    local = b - mp.first[q];
+}
+
 __device__ inline double dmax(double a, double b) { return a > b ? a : b; }
 __device__ inline double dmin(double a, double b) { return a < b ? a : b; }
 
@@ -87,15 +106,150 @@ void dev_free(double *p);
 int halo_fill(const Dims &d, const Halo &hm, double *a, int nk, hipStream_t s);
 int halo_fill(const Dims &d, const Halo &hm, float *a, int nk, hipStream_t s);
 
+// Halo fill of many arrays (the members of an ensemble) in one launch per
+// direction; a job with px = py = 0 is skipped.
+template <typename T>
+struct HaloJob {
+    Dims d;
+    T *a;
+    int nk, px, py;
+};
+template <typename T>
+int halo_batch(const HaloJob<T> *d_jobs, const HaloJob<T> *h_jobs, int n, hipStream_t s);
+
 // Element-wise FP64 <-> FP32 copies on a stream (round to nearest / exact)
 int narrow_to_f32(const double *src, float *dst, long long n, hipStream_t s);
 int widen_to_f64(const float *src, double *dst, long long n, hipStream_t s);
 int dev_alloc_bytes(void **p, size_t bytes, bool zero);
 
+// ------------------------------------------------- kernel argument blocks
+// vert_visc of one grid.
+// T: storage type of u, v and h (double, or float in the mixed-precision
+// form); the coefficients and the tridiagonal solve are FP64 either way, and
+// between sweeps B and C the forward-eliminated velocities live in the
+// velocity array itself (rounded to T).
+template <typename T>
+struct VVArgs {
+    Dims d;
+    T *u, *v;
+    const T *h;
+    double dt;
+    const double *taux, *tauy, *ray_u, *ray_v, *mask2dCu, *mask2dCv;
+    double *a_u, *a_v, *h_u, *h_v, *taux_bot, *tauy_bot;
+    double Kv, Kv_ml, Kv_extra_bbl, Hmix, Hbbl;
+};
+
+// Arguments of one pass.  Names are those of the zonal pass; the meridional
+// pass passes v, vh, ... and its own metrics in the same slots:
+//   IdxT -> IdyT, dy_Cu -> dx_Cv, dxT -> dyT, dxCu -> dyCv, mask2dCu -> mask2dCv.
+// T is the type of the 3D layer fields and of the flux arithmetic: double
+// (the reference's FP64) or float (the mixed-precision form); metrics, the
+// 2D BT_cont outputs and du_cor stay FP64 in memory.
+template <typename T>
+struct ContArgsT {
+    Dims d;
+    const T *u, *hin;
+    T *h, *uh;
+    double dt;
+    const double *uhbt;
+    const double *IareaT, *IdxT, *dy_Cu, *dxT, *dxCu, *mask2dCu, *mask2dT;
+    double tol_eta0, tol_vel, CFL_limit_adjust, h_min;
+    T *u_cor;
+    double *du_cor;
+    double *FA_u_EE, *FA_u_E0, *FA_u_W0, *FA_u_WW, *uBT_WW, *uBT_EE;
+    T *bt_h_u;
+    int monotonic;   // read by the batched launch (the single one takes it as an argument)
+};
+using ContArgs = ContArgsT<double>;
+
+// btstep: the scalars of one grid, its coefficients and one substep's buffers.
+struct BT {
+    Dims d;
+    int nstep, first_direction, extended, px, py;
+    double dtbt, bebt, dgeo_de;
+    double *ubt_av, *vbt_av;   // running time averages, accumulated in place
+};
+
+// Metric access.  BTArrays reads every coefficient from its 2D array (the
+// general grid); BTUniform holds the coefficients that are the same at every
+// point as values and keeps only Datv (closed faces of walls) and the
+// Coriolis factors (f_4 = one value per point, equal over the four
+// components) as arrays.  The arithmetic is the same for both.
+struct BTArrays {
+    const double *Datu, *Datv, *gtot_E, *gtot_W, *gtot_N, *gtot_S, *f_4_u, *f_4_v;
+    const double *bt_rem_u, *bt_rem_v, *IareaT, *IdxCu, *IdyCv;
+    __device__ double datu(long long c) const { return Datu[c]; }
+    __device__ double datv(long long c) const { return Datv[c]; }
+    __device__ double gE(long long c) const { return gtot_E[c]; }
+    __device__ double gW(long long c) const { return gtot_W[c]; }
+    __device__ double gN(long long c) const { return gtot_N[c]; }
+    __device__ double gS(long long c) const { return gtot_S[c]; }
+    __device__ double f4u(int comp, long long c) const { return f_4_u[4 * c + (comp - 1)]; }
+    __device__ double f4v(int comp, long long c) const { return f_4_v[4 * c + (comp - 1)]; }
+    __device__ double remu(long long c) const { return bt_rem_u[c]; }
+    __device__ double remv(long long c) const { return bt_rem_v[c]; }
+    __device__ double iareaT(long long c) const { return IareaT[c]; }
+    __device__ double idxCu(long long c) const { return IdxCu[c]; }
+    __device__ double idyCv(long long c) const { return IdyCv[c]; }
+};
+
+struct BTUniform {
+    double Datu, gtot, bt_rem_u, bt_rem_v, IareaT, IdxCu, IdyCv;
+    const double *Datv, *f4;
+    __device__ double datu(long long) const { return Datu; }
+    __device__ double datv(long long c) const { return Datv[c]; }
+    __device__ double gE(long long) const { return gtot; }
+    __device__ double gW(long long) const { return gtot; }
+    __device__ double gN(long long) const { return gtot; }
+    __device__ double gS(long long) const { return gtot; }
+    __device__ double f4u(int, long long c) const { return f4[c]; }
+    __device__ double f4v(int, long long c) const { return f4[c]; }
+    __device__ double remu(long long) const { return bt_rem_u; }
+    __device__ double remv(long long) const { return bt_rem_v; }
+    __device__ double iareaT(long long) const { return IareaT; }
+    __device__ double idxCu(long long) const { return IdxCu; }
+    __device__ double idyCv(long long) const { return IdyCv; }
+};
+
+struct BTStep {
+    const double *eta_o, *ubt_o, *vbt_o;   // state before the substep
+    double *eta_n, *ubt_n, *vbt_n;         // state after it
+    double *out_u, *out_v, *out_eta;       // written on the last substep only
+    int n;                                 // substep number, 1-based
+};
+
+
+// One member of a batched btstep call: its grid and coefficients, its two
+// state buffer sets, and the call's inputs and outputs.
+struct BTMember {
+    BT p;
+    BTUniform m;
+    double *A[3], *B[3];        // eta, ubt, vbt: A after the copy-in, B the other set
+    const double *in[3];        // eta_in, ubt_in, vbt_in
+    double *out[3];             // u_av, v_av, eta_av
+};
+int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s);
+
+template <typename T>
+int vert_visc_batch(const VVArgs<T> *d_args, const VVArgs<T> *h_args, int n, long long max_lds, hipStream_t s);
+template <typename T>
+int continuity_batch(const ContArgsT<T> *d_args, const ContArgsT<T> *h_args, int n, bool y_pass, bool halo_copy,
+                     long long max_lds, hipStream_t s);
+
 }  // namespace mom6h
 
 struct mom6h_vert_visc_cs;
 struct mom6h_continuity_cs;
+
+struct mom6h_barotropic_cs;
+
+namespace mom6h {
+int bt_member(const mom6h_barotropic_cs *cs, const double *eta_in, const double *ubt_in, const double *vbt_in,
+              double *u_av, double *v_av, double *eta_av, BTMember *out);
+template <typename T>
+ContArgsT<T> continuity_args(const mom6h_continuity_cs *cs, bool y_pass, const T *u, const T *hin, T *h, T *uh,
+                             double dt, const mom6h_bt_cont *bt, T *bt_h_u, T *u_cor, double *du_cor);
+}  // namespace mom6h
 
 namespace mom6h {
 

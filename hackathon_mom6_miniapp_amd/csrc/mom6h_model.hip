@@ -24,7 +24,7 @@
 #include <string>
 #include <vector>
 
-#include "mom6h_fused.h"
+#include "mom6h_model.h"
 
 using namespace mom6h;
 
@@ -33,69 +33,13 @@ namespace {
 const double kG_EARTH = 9.80;     // src/common/mom6_types.F90:80
 const double kDepth = 4000.0;     // mom6_barotropic_serial.F90:76
 
-struct Field {
-    std::string name;
-    void **p;
-    size_t n;
-    int bytes;    // 8: FP64, 4: FP32 (the layer fields of the mixed-precision form)
-    bool owned;   // freed with the field list (BT_cont 2D arrays are freed with their struct)
-};
+}  // namespace
 
-enum Stage {
-    ST_TRANSPORTS, ST_HOR_VISC, ST_CORIOLIS, ST_VEL_UPDATE, ST_VERT_VISC, ST_CONTINUITY,
-    ST_HALO, ST_BTSTEP, ST_MOMENTUM, ST_COUNT
-};
+namespace mom6h {
 const char *kStageNames[ST_COUNT] = {"transports", "hor_visc",   "coriolis", "vel_update", "vert_visc",
                                      "continuity", "halo",       "btstep",   "momentum"};
 
-}  // namespace
-
-struct mom6h_model {
-    mom6h_config c;
-    Dims d;
-    Halo halo;
-    bool prepared = false;
-    int use_graph = 0, profile = 0;
-    int serialize = 0;       // barotropic chain on the 3D stream (per-kernel timing without overlap)
-    int fused = 1;           // fused momentum stencil (module-by-module when 0)
-    bool uniform = false;    // every metric array constant (fused kernels allowed)
-    UMetrics ug{};
-    hipStream_t s0 = nullptr, s1 = nullptr;
-    hipEvent_t e_start = nullptr, e_bt = nullptr;
-    // state and driver arrays
-    double *u, *v, *h, *h0, *htmp, *uh, *vh, *CAu, *CAv, *up, *vp, *diffu, *diffv;
-    double *eta, *ubt, *vbt, *ubt_av, *vbt_av, *eta_av, *u_cor, *du_cor;
-    double *hx = nullptr, *v_cor = nullptr, *dv_cor = nullptr;   // meridional pass
-    mom6h_bt_cont bt_v{};                                          // its v-side BT_cont
-    double *taux, *tauy, *ray_u, *ray_v, *CoriolisBu, *mask2dT, *mask2dCu, *mask2dCv, *mask2dBu;
-    mom6h_bt_cont bt{};
-    // mixed-precision form: FP32 layer fields (the FP64 pointers above that
-    // name them stay NULL); 2D fields, BT_cont 2D and du_cor stay FP64
-    bool mixed = false;
-    struct {
-        float *u, *v, *h, *h0, *htmp, *uh, *vh, *up, *vp, *u_cor, *bt_h_u;
-        float *hx, *v_cor, *bt_h_v;   // meridional pass
-        float *del2u, *del2v;         // biharmonic scratch of the fused stencil
-    } f{};
-    double *del2u = nullptr, *del2v = nullptr;   // biharmonic scratch of the fused stencil (FP64)
-    std::vector<Field> fields;
-    // host copies of the uniform metrics (init_ocean_grid)
-    std::vector<double> IareaT, areaT, dxT, dyT, IdxT, IdyT, dxCu, dyCu, dy_Cu, IdxCu, IdyCu;
-    std::vector<double> dxCv, dyCv, IdxCv, IdyCv, IareaBu, IareaCu, IareaCv, dxBu, dyBu, IdxBu, IdyBu;
-    // module control structures
-    mom6h_continuity_cs *cont = nullptr;
-    mom6h_coriolis_cs *cor = nullptr;
-    mom6h_barotropic_cs *bt_cs = nullptr;
-    mom6h_vert_visc_cs *vv = nullptr;
-    mom6h_hor_visc_cs *hv = nullptr;
-    // one step captured as a graph (both streams, fork and join inside)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_fused = -1;    // fused_ok() at capture time
-    // profiling
-    std::vector<hipEvent_t> ev;
-    double stage_ms[ST_COUNT] = {0};
-};
+}  // namespace mom6h
 
 namespace {
 
@@ -213,10 +157,6 @@ int enqueue_barotropic(mom6h_model *m)
     return MOM6H_OK;
 }
 
-bool fused_ok(const mom6h_model *m)
-{
-    return m->fused && m->uniform && m->c.laplacian && m->c.coriolis_scheme == 1;
-}
 
 // transports + hor_visc + CorAdCalc + velocity update of one phase
 int momentum_phase(mom6h_model *m, const double *u, const double *v, const double *h, double *ou,
@@ -232,7 +172,7 @@ int momentum_phase(mom6h_model *m, const double *u, const double *v, const doubl
         a.u = u; a.v = v; a.h = h; a.bu = m->u; a.bv = m->v; a.ou = ou; a.ov = ov;
         a.mask2dT = m->mask2dT; a.mask2dBu = m->mask2dBu; a.CoriolisBu = m->CoriolisBu;
         a.Ah = m->c.Ah; a.del2u = m->del2u; a.del2v = m->del2v;
-        return launch_momentum(a, s0);
+        return launch_momentum(a, s0, m->momentum_tile);
     }
     { StageTimer t(m, ST_TRANSPORTS, s0);
       RC(mom6h_transports(d.isd, d.ied, d.jsd, d.jed, nk, u, v, h, m->cor->dyCu, m->cor->dxCv, m->uh, m->vh, s0)); }
@@ -273,7 +213,7 @@ int momentum_phase_f32(mom6h_model *m, const float *u, const float *v, const flo
     a.u = u; a.v = v; a.h = h; a.bu = m->f.u; a.bv = m->f.v; a.ou = ou; a.ov = ov;
     a.mask2dT = m->mask2dT; a.mask2dBu = m->mask2dBu; a.CoriolisBu = m->CoriolisBu;
     a.Ah = (float)m->c.Ah; a.del2u = m->f.del2u; a.del2v = m->f.del2v;
-    return launch_momentum(a, m->s0);
+    return launch_momentum(a, m->s0, m->momentum_tile);
 }
 
 int continuity_phase_f32(mom6h_model *m, const float *u, const float *v, const float *hin, float *hout, double dt)
@@ -728,6 +668,17 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
     const std::string k(key);
     if (k == "graph") m->use_graph = value != 0.0;
     else if (k == "serialize") m->serialize = value != 0.0;
+    else if (k == "momentum_tile") {
+        const int ty = (int)value;
+        MOM6H_REQUIRE(ty == 4 || ty == 8 || ty == 16, "mom6h_model_set_option: momentum_tile must be 4, 8 or 16");
+        m->momentum_tile = ty;
+        if (m->gexec) {   // the captured step holds the old launch shape
+            (void)hipGraphExecDestroy(m->gexec);
+            (void)hipGraphDestroy(m->graph);
+            m->gexec = nullptr;
+            m->graph = nullptr;
+        }
+    }
     else if (k == "fused") {
         MOM6H_REQUIRE(!m->mixed || value != 0.0, "mom6h_model_set_option: the mixed-precision form is fused only");
         m->fused = value != 0.0;
@@ -762,6 +713,60 @@ int mom6h_model_main_stream(mom6h_model *m, void **stream)
     return MOM6H_OK;
 }
 
+int mom6h_model_autotune(mom6h_model *m, int reps, char *report, int nbytes)
+{
+    MOM6H_REQUIRE(m != nullptr, "mom6h_model_autotune: NULL model");
+    MOM6H_REQUIRE(m->prepared, "mom6h_model_autotune: call mom6h_model_prepare first");
+    MOM6H_REQUIRE(fused_ok(m), "mom6h_model_autotune: only the fused stencil has tile shapes to choose");
+    MOM6H_CHECK(hipDeviceSynchronize());
+    const int saved_profile = m->profile;
+    m->profile = 0;
+    // the predictor's momentum stencil (u, v, h0 -> up, vp): every step rewrites its outputs
+    auto launch = [&]() -> int {
+        return m->mixed ? momentum_phase_f32(m, m->f.u, m->f.v, m->f.h0, m->f.up, m->f.vp, m->c.dt)
+                        : momentum_phase(m, m->u, m->v, m->h0, m->up, m->vp, m->c.dt);
+    };
+    hipEvent_t a, b;
+    MOM6H_CHECK(hipEventCreate(&a));
+    MOM6H_CHECK(hipEventCreate(&b));
+    const int cand[3] = {4, 8, 16};
+    float best = 0.f;
+    int pick = 8, rc = 0;
+    std::string rep = "momentum tile rows:";
+    for (int q = 0; q < 3 && !rc; ++q) {
+        m->momentum_tile = cand[q];
+        rc = launch();   // warm
+        if (!rc && hipEventRecord(a, m->s0) != hipSuccess) rc = set_error(MOM6H_ESTATE, "event record failed");
+        for (int r = 0; r < reps && !rc; ++r) rc = launch();
+        if (!rc && hipEventRecord(b, m->s0) != hipSuccess) rc = set_error(MOM6H_ESTATE, "event record failed");
+        if (!rc && hipEventSynchronize(b) != hipSuccess) rc = set_error(MOM6H_ESTATE, "event wait failed");
+        float ms = 0.f;
+        if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = set_error(MOM6H_ESTATE, "event timing failed");
+        ms /= (float)(reps > 0 ? reps : 1);
+        char buf[64];
+        snprintf(buf, sizeof(buf), " %d: %.4f ms;", cand[q], ms);
+        rep += buf;
+        if (q == 0 || ms < best) {
+            best = ms;
+            pick = cand[q];
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    m->profile = saved_profile;
+    m->momentum_tile = pick;
+    if (m->gexec) {
+        (void)hipGraphExecDestroy(m->gexec);
+        (void)hipGraphDestroy(m->graph);
+        m->gexec = nullptr;
+        m->graph = nullptr;
+    }
+    if (rc) return rc;
+    rep += " chosen " + std::to_string(pick);
+    if (report && nbytes > 0) snprintf(report, (size_t)nbytes, "%s", rep.c_str());
+    return MOM6H_OK;
+}
+
 int mom6h_model_destroy(mom6h_model *m)
 {
     if (!m) return MOM6H_OK;
@@ -787,3 +792,10 @@ int mom6h_model_destroy(mom6h_model *m)
 }
 
 }  // extern "C"
+
+namespace mom6h {
+bool fused_ok(const mom6h_model *m)
+{
+    return m->fused && m->uniform && m->c.laplacian && m->c.coriolis_scheme == 1;
+}
+}  // namespace mom6h

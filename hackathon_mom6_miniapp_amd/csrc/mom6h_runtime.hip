@@ -49,10 +49,9 @@ void dev_free(double *p)
 // every row; y-fill: rows jsd..jsc-1 and jec+1..jed of every column, run after
 // the x-fill so corners take their x-filled source.
 template <typename T>
-__global__ void k_halo_x(Dims d, T *a, int nk)
+__device__ __forceinline__ void halo_x_body(const Dims &d, T *a, int nk, long long t)
 {
     const int hw_w = d.isc - d.isd, hw_e = d.ied - d.iec, hw = hw_w + hw_e;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long per_k = (long long)hw * d.n2;
     if (t >= per_k * nk) return;
     const int k = (int)(t / per_k) + 1;
@@ -66,10 +65,15 @@ __global__ void k_halo_x(Dims d, T *a, int nk)
 }
 
 template <typename T>
-__global__ void k_halo_y(Dims d, T *a, int nk)
+__global__ void k_halo_x(Dims d, T *a, int nk)
+{
+    halo_x_body(d, a, nk, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T>
+__device__ __forceinline__ void halo_y_body(const Dims &d, T *a, int nk, long long t)
 {
     const int hs = d.jsc - d.jsd, hn = d.jed - d.jec, hw = hs + hn;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long per_k = (long long)hw * d.n1;
     if (t >= per_k * nk) return;
     const int k = (int)(t / per_k) + 1;
@@ -81,6 +85,51 @@ __global__ void k_halo_y(Dims d, T *a, int nk)
     const int src = (c < hs) ? j + Q : j - Q;
     a[IX3(d, i, j, k)] = a[IX3(d, i, src, k)];
 }
+
+template <typename T>
+__global__ void k_halo_y(Dims d, T *a, int nk)
+{
+    halo_y_body(d, a, nk, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// the ensemble form: every (member, array) job of one direction in one launch
+template <typename T, bool Y>
+__global__ void __launch_bounds__(256) k_halo_batch(const HaloJob<T> *__restrict__ jobs, BatchMap mp)
+{
+    int m, local;
+    batch_locate(mp, blockIdx.x, m, local);
+    const HaloJob<T> &j = jobs[m];
+    const long long t = (long long)local * 256 + threadIdx.x;
+    if (Y) halo_y_body(j.d, j.a, j.nk, t);
+    else halo_x_body(j.d, j.a, j.nk, t);
+}
+
+// jobs: device copy of h_jobs; x jobs are those with px, y jobs those with py
+template <typename T>
+int halo_batch(const HaloJob<T> *d_jobs, const HaloJob<T> *h_jobs, int n, hipStream_t s)
+{
+    MOM6H_REQUIRE(n <= kMaxMembers, "halo batch: at most %d jobs", kMaxMembers);
+    for (int dir = 0; dir < 2; ++dir) {
+        BatchMap mp{};
+        mp.n = n;
+        for (int m = 0; m < n; ++m) {
+            const Dims &d = h_jobs[m].d;
+            const bool on = dir == 0 ? h_jobs[m].px : h_jobs[m].py;
+            const long long cells = dir == 0 ? (long long)((d.isc - d.isd) + (d.ied - d.iec)) * d.n2
+                                             : (long long)((d.jsc - d.jsd) + (d.jed - d.jec)) * d.n1;
+            mp.nbx[m] = on ? (int)((cells * h_jobs[m].nk + 255) / 256) : 0;
+            mp.first[m + 1] = mp.first[m] + mp.nbx[m];
+        }
+        if (mp.first[n] == 0) continue;
+        if (dir == 0) k_halo_batch<T, false><<<(unsigned)mp.first[n], 256, 0, s>>>(d_jobs, mp);
+        else k_halo_batch<T, true><<<(unsigned)mp.first[n], 256, 0, s>>>(d_jobs, mp);
+        MOM6H_LAUNCH_CHECK();
+    }
+    return MOM6H_OK;
+}
+
+template int halo_batch<float>(const HaloJob<float> *, const HaloJob<float> *, int, hipStream_t);
+template int halo_batch<double>(const HaloJob<double> *, const HaloJob<double> *, int, hipStream_t);
 
 template <typename T>
 int halo_fill_t(const Dims &d, const Halo &hm, T *a, int nk, hipStream_t s)

@@ -179,34 +179,18 @@ __global__ void k_vert_visc(Dims d, double *__restrict__ u, double *__restrict__
 // levels, so the compiler interleaves them.  Only the Thomas recurrence itself
 // (one division per level) is serial.  Levels that do not fill a chunk take a
 // one-level loop with the same arithmetic.
-// T: storage type of u, v and h (double, or float in the mixed-precision
-// form); the coefficients and the tridiagonal solve are FP64 either way, and
-// between sweeps B and C the forward-eliminated velocities live in the
-// velocity array itself (rounded to T).
-template <typename T>
-struct VVArgs {
-    Dims d;
-    T *u, *v;
-    const T *h;
-    double dt;
-    const double *taux, *tauy, *ray_u, *ray_v, *mask2dCu, *mask2dCv;
-    double *a_u, *a_v, *h_u, *h_v, *taux_bot, *tauy_bot;
-    double Kv, Kv_ml, Kv_extra_bbl, Hmix, Hbbl;
-};
 
 constexpr int VV_C = 8;    // levels per chunk of sweep B (heavy bodies)
 constexpr int VV_A = 16;   // levels per chunk of sweeps A and C (light bodies, latency-bound loads)
 
+// Block (bx, row) of one grid: 64 columns of one row; Zs = (nk+1)*64 doubles of LDS.
+// Rows are interleaved v, u, v, u, ... (v row J = js-1+r/2, u row j = js+(r-1)/2)
+// so that consecutive rows share their h rows.
 template <typename T, bool KEEP>
-__global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs<T> p)
+__device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, double *Zs)
 {
-    extern __shared__ double Zs[];
     const Dims &d = p.d;
     const int lane = threadIdx.x;
-    // rows interleaved v, u, v, u, ... (v row J = js-1+r/2, u row j = js+(r-1)/2)
-    // so that consecutive rows share their h rows, tiles renumbered per XCD
-    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int bx = tile % gridDim.x, row = tile / gridDim.x;
     const bool is_u = (row & 1) != 0;
     const long long s = d.s2;
     const int nk = d.nk;
@@ -433,6 +417,26 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs<T> p)
     *tb_out = t;
 }
 
+// one grid, tiles renumbered per XCD
+template <typename T, bool KEEP>
+__global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs<T> p)
+{
+    extern __shared__ double Zs[];
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    vv_body<T, KEEP>(p, tile % gridDim.x, tile / gridDim.x, Zs);
+}
+
+// every member of an ensemble in one launch (LDS sized for the deepest member)
+template <typename T>
+__global__ void __launch_bounds__(64, 2) k_vert_visc_batch(const VVArgs<T> *__restrict__ args, BatchMap mp)
+{
+    extern __shared__ double Zs[];
+    int m, local;
+    batch_locate(mp, xcd_tile(blockIdx.x, gridDim.x), m, local);
+    const VVArgs<T> p = args[m];
+    vv_body<T, false>(p, local % mp.nbx[m], local / mp.nbx[m], Zs);
+}
+
 // The mixed-precision form: FP32 u, v, h in memory, FP64 solve; the
 // coefficients stay on chip (the LDS kernel only).
 int vert_visc_f32(mom6h_vert_visc_cs *cs, float *u, float *v, const float *h, double dt, const double *taux,
@@ -451,6 +455,30 @@ int vert_visc_f32(mom6h_vert_visc_cs *cs, float *u, float *v, const float *h, do
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
 }
+
+template <typename T>
+int vert_visc_batch(const VVArgs<T> *d_args, const VVArgs<T> *h_args, int n, long long max_lds, hipStream_t s)
+{
+    MOM6H_REQUIRE(n >= 1 && n <= kMaxMembers, "vert_visc batch: 1..%d members", kMaxMembers);
+    BatchMap mp{};
+    mp.n = n;
+    int nk_max = 1;
+    for (int m = 0; m < n; ++m) {
+        const Dims &d = h_args[m].d;
+        mp.nbx[m] = (d.iec - d.isc + 2 + 63) / 64;
+        mp.nby[m] = (d.jec - d.jsc + 1) + (d.jec - d.jsc + 2);
+        mp.first[m + 1] = mp.first[m] + mp.nbx[m] * mp.nby[m];
+        nk_max = d.nk > nk_max ? d.nk : nk_max;
+    }
+    const size_t lds = (size_t)(nk_max + 1) * 64 * sizeof(double);
+    MOM6H_REQUIRE((long long)lds <= max_lds, "vert_visc batch: nk = %d exceeds the LDS column", nk_max);
+    k_vert_visc_batch<T><<<(unsigned)mp.first[n], 64, lds, s>>>(d_args, mp);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
+}
+
+template int vert_visc_batch<float>(const VVArgs<float> *, const VVArgs<float> *, int, long long, hipStream_t);
+template int vert_visc_batch<double>(const VVArgs<double> *, const VVArgs<double> *, int, long long, hipStream_t);
 
 }  // namespace mom6h
 

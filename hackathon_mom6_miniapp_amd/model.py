@@ -34,12 +34,25 @@ def to_mconfig(cfg: Config) -> MConfig:
 
 
 class Model:
-    def __init__(self, cfg: Config, case: dict, graph: bool = False):
+    def __init__(self, cfg: Config, case: dict | None, graph: bool = False, handle=None):
+        """A model run on the device.  ``handle``: wrap a member of an
+        ensemble (created and prepared by the ensemble) instead of creating one."""
         self.cfg = cfg
         self.L = lib()
+        self.owned = handle is None
         self.m = C.c_void_p()
+        if handle is not None:
+            self.m = handle
+            return
         mc = to_mconfig(cfg)
         check(self.L.mom6h_model_create(C.byref(self.m), C.byref(mc)), "mom6h_model_create")
+        self.fill_inputs(case)
+        check(self.L.mom6h_model_set_option(self.m, b"graph", float(graph)), "set_option")
+        check(self.L.mom6h_model_prepare(self.m), "mom6h_model_prepare")
+
+    def fill_inputs(self, case: dict):
+        """Upload (or generate on the device) the run's inputs; before prepare."""
+        cfg = self.cfg
         for name in _UPLOAD:
             val = case[name]
             if isinstance(val, Seeded):
@@ -52,13 +65,11 @@ class Model:
         ray = np.full((cfg.n1, cfg.n2), cfg.ray_bottom, order="F")
         self.upload("ray_u", ray)
         self.upload("ray_v", ray)
-        check(self.L.mom6h_model_set_option(self.m, b"graph", float(graph)), "set_option")
-        check(self.L.mom6h_model_prepare(self.m), "mom6h_model_prepare")
 
     def close(self):
-        if self.m:
+        if self.m and self.owned:
             self.L.mom6h_model_destroy(self.m)
-            self.m = C.c_void_p()
+        self.m = C.c_void_p()
 
     def __del__(self):
         try:
@@ -132,3 +143,56 @@ class Model:
         s = C.c_void_p()
         check(self.L.mom6h_model_main_stream(self.m, C.byref(s)), "main_stream")
         return s.value or 0
+
+    def autotune(self, reps: int = 5) -> str:
+        buf = C.create_string_buffer(512)
+        check(self.L.mom6h_model_autotune(self.m, reps, buf, 512), "mom6h_model_autotune")
+        return buf.value.decode()
+
+
+class Ensemble:
+    """Members of different sizes advanced together: one launch per stage over
+    every member (mom6h_ensemble_*).  ``members`` are Model views on the
+    members' device state."""
+
+    def __init__(self, cfgs: list, cases: list, graph: bool = False):
+        self.L = lib()
+        self.e = C.c_void_p()
+        arr = (MConfig * len(cfgs))(*[to_mconfig(c) for c in cfgs])
+        check(self.L.mom6h_ensemble_create(C.byref(self.e), arr, len(cfgs)), "mom6h_ensemble_create")
+        self.members = []
+        for q, (cfg, case) in enumerate(zip(cfgs, cases)):
+            h = C.c_void_p()
+            check(self.L.mom6h_ensemble_member(self.e, q, C.byref(h)), "mom6h_ensemble_member")
+            mv = Model(cfg, None, handle=h)
+            mv.fill_inputs(case)
+            self.members.append(mv)
+        check(self.L.mom6h_ensemble_set_option(self.e, b"graph", float(graph)), "set_option")
+        check(self.L.mom6h_ensemble_prepare(self.e), "mom6h_ensemble_prepare")
+
+    def step(self, niter: int = 1):
+        check(self.L.mom6h_ensemble_step(self.e, niter), "mom6h_ensemble_step")
+
+    def synchronize(self):
+        check(self.L.mom6h_ensemble_synchronize(self.e), "mom6h_ensemble_synchronize")
+
+    def set_option(self, key: str, value: float):
+        check(self.L.mom6h_ensemble_set_option(self.e, key.encode(), float(value)), f"option {key}")
+
+    def autotune(self, reps: int = 5) -> str:
+        buf = C.create_string_buffer(512)
+        check(self.L.mom6h_ensemble_autotune(self.e, reps, buf, 512), "mom6h_ensemble_autotune")
+        return buf.value.decode()
+
+    def close(self):
+        for mv in self.members:
+            mv.m = C.c_void_p()
+        if self.e:
+            self.L.mom6h_ensemble_destroy(self.e)
+            self.e = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -249,7 +249,8 @@ int mom6h_model_prepare(mom6h_model *m);
 int mom6h_model_step(mom6h_model *m, int niter);
 int mom6h_model_synchronize(mom6h_model *m);
 /* Options: "graph" (1 = capture one step, both streams, as a hipGraph and
- * replay it per step; default 0), "fused" (1 = fused momentum stencil and
+ * replay it per step; default 0), "momentum_tile" (rows of the fused momentum
+ * tile: 4, 8 or 16; default 8), "fused" (1 = fused momentum stencil and
  * on-chip vert_visc coefficients where the grid allows, default 1; 0 = the
  * module-by-module path), "profile" (1 = record per-stage HIP events; the
  * graph is not used while profiling), "serialize" (1 = run the barotropic
@@ -260,7 +261,37 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value);
  * written comma-separated into names (size nbytes). Returns the stage count. */
 int mom6h_model_stage_times(mom6h_model *m, double *ms, int max_stages, char *names, int nbytes);
 int mom6h_model_main_stream(mom6h_model *m, void **stream);
+/* Times the fused momentum stencil (reps launches of the predictor's, whose
+ * outputs every step rewrites) for each tile shape (4, 8, 16 rows), keeps the
+ * fastest as option "momentum_tile" and writes the timings into report. */
+int mom6h_model_autotune(mom6h_model *m, int reps, char *report, int nbytes);
 int mom6h_model_destroy(mom6h_model *m);
+
+/* ---------------------------------------------------------- the ensemble --
+ * Several models (members: own grid size, nk and state) advanced together:
+ * each stage of the step is one launch over the tiles of every member, the
+ * barotropic chains of all members share a second stream.  Members must share
+ * bt_nsteps, precision, meridional and biharmonic, and need the fused
+ * stencils (uniform metrics, Sadourny Coriolis, Laplacian viscosity).  Fill a
+ * member's fields through the model API on the handle mom6h_ensemble_member
+ * returns (upload / fill / download), then prepare once.  A member's results
+ * equal those of the same model stepped alone.  No counterpart in the
+ * reference (its drivers run one grid). */
+typedef struct mom6h_ensemble mom6h_ensemble;
+int mom6h_ensemble_create(mom6h_ensemble **e, const mom6h_config *configs, int n_members);
+int mom6h_ensemble_member(mom6h_ensemble *e, int index, mom6h_model **m);
+int mom6h_ensemble_prepare(mom6h_ensemble *e);
+int mom6h_ensemble_step(mom6h_ensemble *e, int niter);
+int mom6h_ensemble_synchronize(mom6h_ensemble *e);
+/* Options: "graph" (1 = capture one step, replay it per step), "momentum_tile"
+ * (rows of the fused momentum tile: 4, 8 or 16; default 8). */
+int mom6h_ensemble_set_option(mom6h_ensemble *e, const char *key, double value);
+/* Times the fused momentum launch of the ensemble (reps launches) for every
+ * tile shape, keeps the fastest and writes the timings into report.  It runs
+ * only the predictor's momentum stencil, whose outputs every step rewrites,
+ * so the state is unchanged. */
+int mom6h_ensemble_autotune(mom6h_ensemble *e, int reps, char *report, int nbytes);
+int mom6h_ensemble_destroy(mom6h_ensemble *e);
 
 #ifdef __cplusplus
 }
