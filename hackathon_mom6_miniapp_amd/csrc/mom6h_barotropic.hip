@@ -25,8 +25,8 @@ namespace mom6h {
 // so no grid-wide barrier is needed: consecutive launches on one stream order
 // the substeps.  State ping-pongs between two buffer sets; the time averages
 // are accumulated in place.
-constexpr int BT_TX = 64, BT_TY = 8;
-constexpr int BT_W = BT_TX + 3, BT_H = BT_TY + 3;
+constexpr int BT_TX = 64;
+constexpr int BT_W = BT_TX + 3;
 
 __global__ void k_bt_copy_in(Dims d, const double *eta_in, const double *ubt_in, const double *vbt_in,
                              double *eta, double *ubt, double *vbt, double *ubt_av, double *vbt_av)
@@ -40,12 +40,21 @@ __global__ void k_bt_copy_in(Dims d, const double *eta_in, const double *ubt_in,
     vbt_av[q] = 0.0;
 }
 
-template <typename M>
+// Output tile BT_TX x BT_TY (BT_TY = 4, 8 or 16 rows: the launch's choice),
+// 256 threads, BT_TY/4 outputs per thread.  Three LDS arrays hold eta, ubt
+// and vbt of the staged region and are updated in place: eta becomes the
+// predicted eta (each point reads only its own eta), each velocity update
+// reads the other component and its own old value only, and the old values
+// the transports of a thread's outputs need are kept in registers before the
+// first update.  The pressure forces are recomputed from the predicted eta
+// where each update needs them (same operands, same values).  At 8 rows the
+// block needs 17.7 KB of LDS, and the whole 720x720 grid fits the chip in
+// one round of blocks.
+template <typename M, int BT_TY>
 __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &st, int tile_x, int tile_y)
 {
-    __shared__ double s_eta[BT_H][BT_W], s_u[BT_H][BT_W], s_v[BT_H][BT_W];
-    __shared__ double s_ep[BT_H][BT_W], s_un[BT_H][BT_W], s_vn[BT_H][BT_W];
-    __shared__ double s_PFu[BT_H][BT_W], s_PFv[BT_H][BT_W];
+    constexpr int BT_H = BT_TY + 3, NOUT = BT_TY / 4;
+    __shared__ double s_e[BT_H][BT_W], s_u[BT_H][BT_W], s_v[BT_H][BT_W];
     const Dims &d = p.d;
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
     const int i0 = d.isd + tile_x * BT_TX, j0 = d.jsd + tile_y * BT_TY;
@@ -56,88 +65,84 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
     const double dtbt = p.dtbt;
 
     // L: old state on [i0-2, i1+1] x [j0-2, j1+1]
-    for (int t = tid; t < BT_W * BT_H; t += blockDim.x) {
+    for (int t = tid; t < BT_W * BT_H; t += 256) {
         const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
         double e = 0.0, uu = 0.0, vv = 0.0;
         if (i >= d.isd && i <= d.ied && j >= d.jsd && j <= d.jed) {
             const long long c = IX2(d, i, j);
             e = st.eta_o[c]; uu = st.ubt_o[c]; vv = st.vbt_o[c];
         }
-        s_eta[lj][li] = e; s_u[lj][li] = uu; s_v[lj][li] = vv;
+        s_e[lj][li] = e; s_u[lj][li] = uu; s_v[lj][li] = vv;
     }
     __syncthreads();
-    // P1: eta predictor (:224-231) on [i0-1, i1+1] x [j0-1, j1+1]
-    for (int t = tid; t < BT_W * BT_H; t += blockDim.x) {
+    // old values the transports of this thread's outputs need
+    double e_old[NOUT], u_old[NOUT], uw_old[NOUT], v_old[NOUT], vs_old[NOUT];
+#pragma unroll
+    for (int r = 0; r < NOUT; ++r) {
+        const int t = tid + 256 * r;
+        const int li = 2 + t % BT_TX, lj = 2 + t / BT_TX;
+        e_old[r] = s_e[lj][li];
+        u_old[r] = s_u[lj][li]; uw_old[r] = s_u[lj][li - 1];
+        v_old[r] = s_v[lj][li]; vs_old[r] = s_v[lj - 1][li];
+    }
+    __syncthreads();
+    // P1: eta predictor (:224-231) on [i0-1, i1+1] x [j0-1, j1+1], in place
+    for (int t = tid; t < BT_W * BT_H; t += 256) {
         const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
         if (i < i0 - 1 || j < j0 - 1 || i < is || i > ie || j < js || j > je) continue;
         const long long c = IX2(d, i, j);
-        s_ep[lj][li] = s_eta[lj][li] + (dtbt * m.iareaT(c)) *
+        s_e[lj][li] = s_e[lj][li] + (dtbt * m.iareaT(c)) *
             (((m.datu(IX2(d, i - 1, j)) * s_u[lj][li - 1]) - (m.datu(c) * s_u[lj][li])) +
              ((m.datv(IX2(d, i, j - 1)) * s_v[lj - 1][li]) - (m.datv(c) * s_v[lj][li])));
     }
     __syncthreads();
-    // P2: pressure forces (:234-247) and the first velocity update (:250-260)
-    for (int t = tid; t < BT_W * BT_H; t += blockDim.x) {
+    // pressure forces (:234-247) from the predicted eta
+    auto pf_u = [&](int lj, int li, int i, int j, long long c) -> double {
+        return ((s_e[lj][li] * m.gE(c)) - (s_e[lj][li + 1] * m.gW(IX2(d, i + 1, j)))) * p.dgeo_de * m.idxCu(c);
+    };
+    auto pf_v = [&](int lj, int li, int i, int j, long long c) -> double {
+        return ((s_e[lj][li] * m.gN(c)) - (s_e[lj + 1][li] * m.gS(IX2(d, i, j + 1)))) * p.dgeo_de * m.idyCv(c);
+    };
+    // the velocity updates (:250-260), with the other component as it stands in LDS
+    auto new_v = [&](int lj, int li, int i, int j, long long c) -> double {
+        double vn = s_v[lj][li];
+        if (i >= is && i <= ie && j >= js && j <= je - 1) {
+            const double Cor = -1.0 * (((m.f4v(1, c) * s_u[lj][li - 1]) + (m.f4v(4, c) * s_u[lj + 1][li])) +
+                                       ((m.f4v(2, c) * s_u[lj][li]) + (m.f4v(3, c) * s_u[lj + 1][li - 1])));
+            vn = m.remv(c) * (vn + dtbt * (Cor + pf_v(lj, li, i, j, c)));
+        }
+        return vn;
+    };
+    auto new_u = [&](int lj, int li, int i, int j, long long c) -> double {
+        double un = s_u[lj][li];
+        if (i >= is && i <= ie - 1 && j >= js && j <= je) {
+            const double Cor = (((m.f4u(4, c) * s_v[lj][li + 1]) + (m.f4u(1, c) * s_v[lj - 1][li])) +
+                                ((m.f4u(3, c) * s_v[lj][li]) + (m.f4u(2, c) * s_v[lj - 1][li + 1])));
+            un = m.remu(c) * (un + dtbt * (Cor + pf_u(lj, li, i, j, c)));
+        }
+        return un;
+    };
+    // P2: the first update -- vbt on [i0-1, i1+1] x [j0-1, j1] or ubt on [i0-1, i1] x [j0-1, j1+1]
+    for (int t = tid; t < BT_W * BT_H; t += 256) {
         const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
         if (i < i0 - 1 || j < j0 - 1) continue;
         const long long c = IX2(d, i, j);
-        const bool in_u = i >= is && i <= ie - 1 && j >= js && j <= je;
-        const bool in_v = i >= is && i <= ie && j >= js && j <= je - 1;
-        double PFu = 0.0, PFv = 0.0;
-        if (in_u && i <= i1) {
-            PFu = ((s_ep[lj][li] * m.gE(c)) - (s_ep[lj][li + 1] * m.gW(IX2(d, i + 1, j)))) * p.dgeo_de * m.idxCu(c);
-            s_PFu[lj][li] = PFu;
-        }
-        if (in_v && j <= j1) {
-            PFv = ((s_ep[lj][li] * m.gN(c)) - (s_ep[lj + 1][li] * m.gS(IX2(d, i, j + 1)))) * p.dgeo_de * m.idyCv(c);
-            s_PFv[lj][li] = PFv;
-        }
-        if (v_first) {  // vbt on [i0-1, i1+1] x [j0-1, j1]
-            if (j <= j1) {
-                double vn = s_v[lj][li];
-                if (in_v) {
-                    const double Cor = -1.0 * (((m.f4v(1, c) * s_u[lj][li - 1]) + (m.f4v(4, c) * s_u[lj + 1][li])) +
-                                               ((m.f4v(2, c) * s_u[lj][li]) + (m.f4v(3, c) * s_u[lj + 1][li - 1])));
-                    vn = m.remv(c) * (vn + dtbt * (Cor + PFv));
-                }
-                s_vn[lj][li] = vn;
-            }
-        } else {        // ubt on [i0-1, i1] x [j0-1, j1+1]
-            if (i <= i1) {
-                double un = s_u[lj][li];
-                if (in_u) {
-                    const double Cor = (((m.f4u(4, c) * s_v[lj][li + 1]) + (m.f4u(1, c) * s_v[lj - 1][li])) +
-                                        ((m.f4u(3, c) * s_v[lj][li]) + (m.f4u(2, c) * s_v[lj - 1][li + 1])));
-                    un = m.remu(c) * (un + dtbt * (Cor + PFu));
-                }
-                s_un[lj][li] = un;
-            }
+        if (v_first) {
+            if (j <= j1) s_v[lj][li] = new_v(lj, li, i, j, c);
+        } else {
+            if (i <= i1) s_u[lj][li] = new_u(lj, li, i, j, c);
         }
     }
     __syncthreads();
-    // P3: the second velocity update
-    for (int t = tid; t < BT_W * BT_H; t += blockDim.x) {
+    // P3: the second update -- ubt on [i0-1, i1] x [j0, j1] or vbt on [i0, i1] x [j0-1, j1]
+    for (int t = tid; t < BT_W * BT_H; t += 256) {
         const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
         if (i < i0 - 1 || j < j0 - 1 || i > i1 || j > j1) continue;
         const long long c = IX2(d, i, j);
-        if (v_first) {  // ubt on [i0-1, i1] x [j0, j1]
-            if (j < j0) continue;
-            double un = s_u[lj][li];
-            if (i >= is && i <= ie - 1 && j >= js && j <= je) {
-                const double Cor = (((m.f4u(4, c) * s_vn[lj][li + 1]) + (m.f4u(1, c) * s_vn[lj - 1][li])) +
-                                    ((m.f4u(3, c) * s_vn[lj][li]) + (m.f4u(2, c) * s_vn[lj - 1][li + 1])));
-                un = m.remu(c) * (un + dtbt * (Cor + s_PFu[lj][li]));
-            }
-            s_un[lj][li] = un;
-        } else {        // vbt on [i0, i1] x [j0-1, j1]
-            if (i < i0) continue;
-            double vn = s_v[lj][li];
-            if (i >= is && i <= ie && j >= js && j <= je - 1) {
-                const double Cor = -1.0 * (((m.f4v(1, c) * s_un[lj][li - 1]) + (m.f4v(4, c) * s_un[lj + 1][li])) +
-                                           ((m.f4v(2, c) * s_un[lj][li]) + (m.f4v(3, c) * s_un[lj + 1][li - 1])));
-                vn = m.remv(c) * (vn + dtbt * (Cor + s_PFv[lj][li]));
-            }
-            s_vn[lj][li] = vn;
+        if (v_first) {
+            if (j >= j0) s_u[lj][li] = new_u(lj, li, i, j, c);
+        } else {
+            if (i >= i0) s_v[lj][li] = new_v(lj, li, i, j, c);
         }
     }
     __syncthreads();
@@ -149,15 +154,17 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
     const bool refill = (p.px || p.py) && st.n < p.nstep;
     const bool last = st.n == p.nstep;
     const int P = ie - is + 1, Q = je - js + 1;
-    auto uhbt = [&](int i, int j, int li, int lj) -> double {
+    auto uhbt = [&](int i, int j, double un, double uo) -> double {
         if (i < iu0 || i > iu1 || j < js || j > je) return 0.0;
-        return m.datu(IX2(d, i, j)) * (trans_wt1 * s_un[lj][li] + trans_wt2 * s_u[lj][li]);
+        return m.datu(IX2(d, i, j)) * (trans_wt1 * un + trans_wt2 * uo);
     };
-    auto vhbt = [&](int i, int j, int li, int lj) -> double {
+    auto vhbt = [&](int i, int j, double vn, double vo) -> double {
         if (i < is || i > ie || j < jv0 || j > jv1) return 0.0;
-        return m.datv(IX2(d, i, j)) * (trans_wt1 * s_vn[lj][li] + trans_wt2 * s_v[lj][li]);
+        return m.datv(IX2(d, i, j)) * (trans_wt1 * vn + trans_wt2 * vo);
     };
-    for (int t = tid; t < BT_TX * BT_TY; t += blockDim.x) {
+#pragma unroll
+    for (int r = 0; r < NOUT; ++r) {
+        const int t = tid + 256 * r;
         const int i = i0 + t % BT_TX, j = j0 + t / BT_TX;
         if (i > d.ied || j > d.jed) continue;
         const int li = i - bi, lj = j - bj;
@@ -165,13 +172,13 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
         const bool in_eta = i >= is && i <= ie && j >= js && j <= je;
         const bool in_u = i >= is && i <= ie - 1 && j >= js && j <= je;
         const bool in_v = i >= is && i <= ie && j >= js && j <= je - 1;
-        double e = s_eta[lj][li];
+        // ubt/vbt carry their new value only on the update ranges (the
+        // updates leave the old value elsewhere)
+        const double un = s_u[lj][li], vn = s_v[lj][li];
+        double e = e_old[r];
         if (in_eta)
-            e = e - dtbt * m.iareaT(c) * ((uhbt(i, j, li, lj) - uhbt(i - 1, j, li - 1, lj)) +
-                                          (vhbt(i, j, li, lj) - vhbt(i, j - 1, li, lj - 1)));
-        // time averages; ubt/vbt carry their new value only on the update ranges
-        const double un = in_u ? s_un[lj][li] : s_u[lj][li];
-        const double vn = in_v ? s_vn[lj][li] : s_v[lj][li];
+            e = e - dtbt * m.iareaT(c) * ((uhbt(i, j, un, u_old[r]) - uhbt(i - 1, j, s_u[lj][li - 1], uw_old[r])) +
+                                          (vhbt(i, j, vn, v_old[r]) - vhbt(i, j - 1, s_v[lj - 1][li], vs_old[r])));
         double ua = p.ubt_av[c], va = p.vbt_av[c];
         if (in_u) { ua = ua + un * inv_nstep; p.ubt_av[c] = ua; }
         if (in_v) { va = va + vn * inv_nstep; p.vbt_av[c] = va; }
@@ -201,11 +208,11 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
     }
 }
 
-template <typename M>
+template <typename M, int TY>
 __global__ void __launch_bounds__(256) k_bt_substep(BT p, M m, BTStep st)
 {
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    bt_body<M>(p, m, st, tile % gridDim.x, tile / gridDim.x);
+    bt_body<M, TY>(p, m, st, tile % gridDim.x, tile / gridDim.x);
 }
 
 // ---- the ensemble form: every member's substep n in one launch.  Member
@@ -224,6 +231,7 @@ __global__ void k_bt_copy_in_batch(const BTMember *__restrict__ args, BatchMap m
     a.p.vbt_av[q] = 0.0;
 }
 
+template <int TY>
 __global__ void __launch_bounds__(256) k_bt_substep_batch(const BTMember *__restrict__ args, BatchMap mp, int n)
 {
     int m, local;
@@ -235,12 +243,14 @@ __global__ void __launch_bounds__(256) k_bt_substep_batch(const BTMember *__rest
     st.eta_n = odd ? a.B[0] : a.A[0]; st.ubt_n = odd ? a.B[1] : a.A[1]; st.vbt_n = odd ? a.B[2] : a.A[2];
     st.out_u = a.out[0]; st.out_v = a.out[1]; st.out_eta = a.out[2];
     st.n = n;
-    bt_body<BTUniform>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
+    bt_body<BTUniform, TY>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
 }
 
 // btstep of every member (all with nstep substeps and uniform coefficients)
-int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s)
+int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s, int ty)
 {
+    MOM6H_REQUIRE(ty == 4 || ty == 8 || ty == 16, "btstep batch: tile rows must be 4, 8 or 16");
+    const int BT_TY = ty;
     MOM6H_REQUIRE(n >= 1 && n <= kMaxMembers, "btstep batch: 1..%d members", kMaxMembers);
     const int nstep = h_args[0].p.nstep;
     BatchMap mc{}, mt{};
@@ -257,7 +267,9 @@ int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStrea
     k_bt_copy_in_batch<<<(unsigned)mc.first[n], 256, 0, s>>>(d_args, mc);
     MOM6H_LAUNCH_CHECK();
     for (int q = 1; q <= nstep; ++q) {
-        k_bt_substep_batch<<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q);
+        if (ty == 4) k_bt_substep_batch<4><<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q);
+        else if (ty == 16) k_bt_substep_batch<16><<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q);
+        else k_bt_substep_batch<8><<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q);
         MOM6H_LAUNCH_CHECK();
     }
     return MOM6H_OK;
@@ -311,6 +323,7 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
                       bt_rem_v && IareaT && IdxCu && IdyCv,
                   "mom6h_barotropic_init: NULL array");
     auto *cs = new mom6h_barotropic_cs();
+    cs->tile_rows = 8;
     cs->d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, 1);
     cs->halo = Halo{periodic_x, periodic_y};
     cs->nstep = nstep;
@@ -382,6 +395,7 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
                                                                          cs->vbt, cs->ubt_av, cs->vbt_av);
         MOM6H_LAUNCH_CHECK();
     }
+    const int BT_TY = cs->tile_rows;
     dim3 grid((d.n1 + BT_TX - 1) / BT_TX, (d.n2 + BT_TY - 1) / BT_TY), block(256);
     double *A[3] = {cs->eta, cs->ubt, cs->vbt}, *B[3] = {cs->eta_b, cs->ubt_b, cs->vbt_b};
     for (int n = 1; n <= cs->nstep; ++n) {
@@ -393,11 +407,15 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
         if (cs->uniform) {
             BTUniform m{cs->u_Datu, cs->u_gtot, cs->u_rem_u, cs->u_rem_v, cs->u_IareaT, cs->u_IdxCu, cs->u_IdyCv,
                         cs->Datv, cs->f4};
-            k_bt_substep<BTUniform><<<grid, block, 0, strm>>>(p, m, st);
+            if (BT_TY == 4) k_bt_substep<BTUniform, 4><<<grid, block, 0, strm>>>(p, m, st);
+            else if (BT_TY == 16) k_bt_substep<BTUniform, 16><<<grid, block, 0, strm>>>(p, m, st);
+            else k_bt_substep<BTUniform, 8><<<grid, block, 0, strm>>>(p, m, st);
         } else {
             BTArrays m{cs->Datu, cs->Datv, cs->gtot_E, cs->gtot_W, cs->gtot_N, cs->gtot_S, cs->f_4_u, cs->f_4_v,
                        cs->bt_rem_u, cs->bt_rem_v, cs->IareaT, cs->IdxCu, cs->IdyCv};
-            k_bt_substep<BTArrays><<<grid, block, 0, strm>>>(p, m, st);
+            if (BT_TY == 4) k_bt_substep<BTArrays, 4><<<grid, block, 0, strm>>>(p, m, st);
+            else if (BT_TY == 16) k_bt_substep<BTArrays, 16><<<grid, block, 0, strm>>>(p, m, st);
+            else k_bt_substep<BTArrays, 8><<<grid, block, 0, strm>>>(p, m, st);
         }
         MOM6H_LAUNCH_CHECK();
         for (int q = 0; q < 3; ++q) std::swap(A[q], B[q]);

@@ -850,7 +850,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 }
 
 template <typename T, bool Y, int CW, int K>
-__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(1, (CW >= 16) ? 4 : 2)))
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, (CW >= 16 || sizeof(T) == 4) ? 4 : 2)))
 k_continuity_blk(ContArgsT<T> a, int monotonic)
 {
     extern __shared__ double smem_raw[];
@@ -859,7 +859,7 @@ k_continuity_blk(ContArgsT<T> a, int monotonic)
 
 // every member of an ensemble in one launch (K for the deepest member)
 template <typename T, bool Y, int CW, int K>
-__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(1, (CW >= 16) ? 4 : 2)))
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, (CW >= 16 || sizeof(T) == 4) ? 4 : 2)))
 k_continuity_batch(const ContArgsT<T> *__restrict__ args, BatchMap mp)
 {
     extern __shared__ double smem_raw[];

@@ -84,6 +84,7 @@ struct mom6h_ensemble {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int momentum_tile = 8;
+    int bt_tile = 8;         // rows of the barotropic substep tile (4 or 8)
     long long max_lds = 0;
     StepTables<float> tf;
     StepTables<double> td;
@@ -117,6 +118,7 @@ int build_tables(mom6h_ensemble *e, StepTables<T> &t, Layers<T> (*layers)(mom6h_
         a.mask2dT = m->mask2dT; a.mask2dBu = m->mask2dBu; a.CoriolisBu = m->CoriolisBu;
         a.bu = L.u; a.bv = L.v;
         a.Ah = (T)m->c.Ah; a.del2u = m->c.biharmonic ? L.del2u : nullptr; a.del2v = m->c.biharmonic ? L.del2v : nullptr;
+        a.nkz = 1;
         // predictor: u, v, h0 -> up, vp; corrector: up, vp, htmp -> u, v (rk2_serial_driver.F90:236-350)
         a.u = L.u; a.v = L.v; a.h = L.h0; a.ou = L.up; a.ov = L.vp; a.fdt = (T)dt;
         t.mom_pred.h.push_back(a);
@@ -194,8 +196,8 @@ int enqueue_3d(mom6h_ensemble *e, StepTables<T> &t)
 int enqueue_bt(mom6h_ensemble *e)
 {
     const int n = (int)e->members.size();
-    RC(btstep_batch(e->bt_pred.d, e->bt_pred.h.data(), n, e->s1));
-    RC(btstep_batch(e->bt_corr.d, e->bt_corr.h.data(), n, e->s1));
+    RC(btstep_batch(e->bt_pred.d, e->bt_pred.h.data(), n, e->s1, e->bt_tile));
+    RC(btstep_batch(e->bt_corr.d, e->bt_corr.h.data(), n, e->s1, e->bt_tile));
     return halo(e->h2d, e->s1);
 }
 
@@ -207,6 +209,17 @@ int enqueue_step(mom6h_ensemble *e)
     RC(e->mixed ? enqueue_3d<float>(e, e->tf) : enqueue_3d<double>(e, e->td));
     MOM6H_CHECK(hipEventRecord(e->e_bt, e->s1));
     MOM6H_CHECK(hipStreamWaitEvent(e->s0, e->e_bt, 0));
+    return MOM6H_OK;
+}
+
+// layer splits of every member's momentum launch (capped at its nk), both phases
+template <typename T>
+int set_splits(StepTables<T> &t, int nkz)
+{
+    for (auto *x : {&t.mom_pred, &t.mom_corr}) {
+        for (auto &a : x->h) a.nkz = nkz < a.d.nk ? nkz : a.d.nk;
+        if (x->d) MOM6H_CHECK(hipMemcpy(x->d, x->h.data(), x->h.size() * sizeof(x->h[0]), hipMemcpyHostToDevice));
+    }
     return MOM6H_OK;
 }
 
@@ -361,6 +374,16 @@ int mom6h_ensemble_set_option(mom6h_ensemble *e, const char *key, double value)
         MOM6H_REQUIRE(ty == 4 || ty == 8 || ty == 16, "mom6h_ensemble_set_option: momentum_tile must be 4, 8 or 16");
         e->momentum_tile = ty;
         drop_graph(e);
+    } else if (k == "bt_tile") {
+        MOM6H_REQUIRE(value == 4.0 || value == 8.0 || value == 16.0, "mom6h_ensemble_set_option: bt_tile must be 4, 8 or 16");
+        e->bt_tile = (int)value;
+        drop_graph(e);
+    } else if (k == "momentum_splits") {
+        MOM6H_REQUIRE(e->prepared, "mom6h_ensemble_set_option: momentum_splits after prepare");
+        const int v = (int)value;
+        MOM6H_REQUIRE(v >= 1, "mom6h_ensemble_set_option: momentum_splits >= 1");
+        RC(e->mixed ? set_splits<float>(e->tf, v) : set_splits<double>(e->td, v));
+        drop_graph(e);
     } else {
         return set_error(MOM6H_EINVAL, "mom6h_ensemble_set_option: unknown option '%s'", key);
     }
@@ -372,25 +395,28 @@ int mom6h_ensemble_autotune(mom6h_ensemble *e, int reps, char *report, int nbyte
     MOM6H_REQUIRE(e != nullptr, "mom6h_ensemble_autotune: NULL ensemble");
     MOM6H_REQUIRE(e->prepared, "mom6h_ensemble_autotune: call mom6h_ensemble_prepare first");
     MOM6H_CHECK(hipDeviceSynchronize());
-    const int cand[3] = {4, 8, 16};
+    const int rows[3] = {4, 8, 16}, splits[4] = {1, 2, 4, 8};
     float best = 0.f;
-    int pick = 8;
-    std::string rep = "momentum tile rows:";
-    for (int q = 0; q < 3; ++q) {
+    int pick = 8, pick_s = 1;
+    std::string rep = "momentum tile rows x layer splits (ms):";
+    for (int q = 0; q < 12; ++q) {
         float ms = 0.f;
-        RC(e->mixed ? time_momentum<float>(e, e->tf, cand[q], reps, &ms)
-                    : time_momentum<double>(e, e->td, cand[q], reps, &ms));
+        RC(e->mixed ? set_splits<float>(e->tf, splits[q % 4]) : set_splits<double>(e->td, splits[q % 4]));
+        RC(e->mixed ? time_momentum<float>(e, e->tf, rows[q / 4], reps, &ms)
+                    : time_momentum<double>(e, e->td, rows[q / 4], reps, &ms));
         char buf[64];
-        snprintf(buf, sizeof(buf), " %d: %.4f ms;", cand[q], ms);
+        snprintf(buf, sizeof(buf), " %dx%d %.4f;", rows[q / 4], splits[q % 4], ms);
         rep += buf;
         if (q == 0 || ms < best) {
             best = ms;
-            pick = cand[q];
+            pick = rows[q / 4];
+            pick_s = splits[q % 4];
         }
     }
     e->momentum_tile = pick;
+    RC(e->mixed ? set_splits<float>(e->tf, pick_s) : set_splits<double>(e->td, pick_s));
     drop_graph(e);
-    rep += " chosen " + std::to_string(pick);
+    rep += " chosen " + std::to_string(pick) + "x" + std::to_string(pick_s);
     if (report && nbytes > 0) snprintf(report, (size_t)nbytes, "%s", rep.c_str());
     return MOM6H_OK;
 }

@@ -50,6 +50,7 @@ struct MomArgsT {
     // biharmonic viscosity: Ah and the Del2 scratch (NULL: Laplacian only)
     T Ah;
     T *del2u, *del2v;
+    int nkz;   // the layers split over this many blocks per tile (1..nk)
 };
 using MomArgs = MomArgsT<double>;
 

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256) k_del2_batch(const MomArgsT<T> *__restric
 // Del2 fields staged like u and v, added to the Laplacian stress
 // (:824-886, :979-1036).
 template <typename T, bool BIH, int TY>
-__device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, int tile_y)
+__device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, int tile_y, int k_first, int k_last)
 {
     constexpr int MH = TY + 2, RPT = TY / 4;   // staged rows; output rows per thread
     __shared__ T s_u[MH][MW], s_v[MH][MW], s_h[MH][MW];
@@ -127,8 +127,8 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             }
         }
     };
-    load_layer(1);
-    for (int k = 1; k <= d.nk; ++k) {
+    load_layer(k_first);
+    for (int k = k_first; k <= k_last; ++k) {
 #pragma unroll
         for (int r = 0; r < NS; ++r) {
             const int t = tid + 256 * r;
@@ -142,7 +142,7 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
 #pragma unroll
         for (int r = 0; r < RPT; ++r) { bu_r[r] = pbu[r]; bv_r[r] = pbv[r]; }
         __syncthreads();
-        if (k < d.nk) load_layer(k + 1);
+        if (k < k_last) load_layer(k + 1);
         // q on q points [i0-1, i1] x [j0-1, j1] (mom6_coriolis_serial.F90:134-168)
         // and KE on h points [i0, i1+1] x [j0, j1+1] (:196-203)
         for (int t = tid; t < MW * MH; t += blockDim.x) {
@@ -231,11 +231,25 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
     }
 }
 
+// layers k_first..k_last of split kz of nkz (the layers are independent)
+__device__ inline void layer_range(int nk, int kz, int nkz, int &k_first, int &k_last)
+{
+    const int per = (nk + nkz - 1) / nkz;
+    k_first = 1 + kz * per;
+    k_last = k_first + per - 1 < nk ? k_first + per - 1 : nk;
+}
+
+// grid (tiles x, tiles y, layer splits); each block walks its split's layers
 template <typename T, bool BIH, int TY>
 __global__ void __launch_bounds__(256) k_momentum(MomArgsT<T> a)
 {
-    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    momentum_body<T, BIH, TY>(a, tile % gridDim.x, tile / gridDim.x);
+    const int nxy = gridDim.x * gridDim.y;
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nxy * gridDim.z);
+    const int xy = tile % nxy;
+    int k0, k1;
+    layer_range(a.d.nk, tile / nxy, gridDim.z, k0, k1);
+    if (k0 > k1) return;
+    momentum_body<T, BIH, TY>(a, xy % gridDim.x, xy / gridDim.x, k0, k1);
 }
 
 // every member's tiles in one launch, renumbered per XCD over the whole batch
@@ -245,7 +259,11 @@ __global__ void __launch_bounds__(256) k_momentum_batch(const MomArgsT<T> *__res
     int m, local;
     batch_locate(mp, xcd_tile(blockIdx.x, gridDim.x), m, local);
     const MomArgsT<T> a = args[m];
-    momentum_body<T, BIH, TY>(a, local % mp.nbx[m], local / mp.nbx[m]);
+    const int nxy = mp.nbx[m] * mp.nby[m], xy = local % nxy;
+    int k0, k1;
+    layer_range(a.d.nk, local / nxy, a.nkz, k0, k1);
+    if (k0 > k1) return;
+    momentum_body<T, BIH, TY>(a, xy % mp.nbx[m], xy / mp.nbx[m], k0, k1);
 }
 
 template <typename T, bool BIH>
@@ -266,7 +284,8 @@ int launch_momentum(const MomArgsT<T> &a, hipStream_t s, int ty)
         k_del2<T><<<g2, dim3(256), 0, s>>>(a);
         MOM6H_LAUNCH_CHECK();
     }
-    dim3 grid((d.iec - d.isc + 1 + MT_X - 1) / MT_X, (d.jec - d.jsc + 1 + ty - 1) / ty);
+    const int nkz = a.nkz < 1 ? 1 : (a.nkz > d.nk ? d.nk : a.nkz);
+    dim3 grid((d.iec - d.isc + 1 + MT_X - 1) / MT_X, (d.jec - d.jsc + 1 + ty - 1) / ty, nkz);
     if (a.del2u) momentum_launch_ty<T, true>(a, ty, grid, s);
     else momentum_launch_ty<T, false>(a, ty, grid, s);
     MOM6H_LAUNCH_CHECK();
@@ -304,9 +323,10 @@ int launch_momentum_batch(const MomArgsT<T> *d_args, const MomArgsT<T> *h_args, 
     }
     for (int m = 0; m < n; ++m) {
         const Dims &d = h_args[m].d;
+        MOM6H_REQUIRE(h_args[m].nkz >= 1 && h_args[m].nkz <= d.nk, "momentum batch: layer splits 1..nk");
         mp.nbx[m] = (d.iec - d.isc + 1 + MT_X - 1) / MT_X;
         mp.nby[m] = (d.jec - d.jsc + 1 + ty - 1) / ty;
-        mp.first[m + 1] = mp.first[m] + mp.nbx[m] * mp.nby[m];
+        mp.first[m + 1] = mp.first[m] + mp.nbx[m] * mp.nby[m] * h_args[m].nkz;
     }
     if (bih) momentum_batch_ty<T, true>(d_args, mp, ty, s);
     else momentum_batch_ty<T, false>(d_args, mp, ty, s);

@@ -228,7 +228,7 @@ struct BTMember {
     const double *in[3];        // eta_in, ubt_in, vbt_in
     double *out[3];             // u_av, v_av, eta_av
 };
-int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s);
+int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s, int ty);
 
 template <typename T>
 int vert_visc_batch(const VVArgs<T> *d_args, const VVArgs<T> *h_args, int n, long long max_lds, hipStream_t s);
@@ -300,6 +300,7 @@ struct mom6h_barotropic_cs {
     int uniform;
     double u_Datu, u_gtot, u_rem_u, u_rem_v, u_IareaT, u_IdxCu, u_IdyCv;
     double *f4;
+    int tile_rows;           // rows of the substep tile (4 or 8)
     // state (two sets, ping-pong across substeps) and time averages; zero at
     // init, as the reference's never-written cells
     double *eta, *ubt, *vbt, *eta_b, *ubt_b, *vbt_b, *ubt_av, *vbt_av;

@@ -36,6 +36,7 @@ struct mom6h_model {
     bool prepared = false;
     int use_graph = 0, profile = 0;
     int momentum_tile = 8;   // rows of the fused momentum tile (4, 8 or 16)
+    int momentum_splits = 1; // blocks per tile over the layers (1..nk)
     int serialize = 0;       // barotropic chain on the 3D stream (per-kernel timing without overlap)
     int fused = 1;           // fused momentum stencil (module-by-module when 0)
     bool uniform = false;    // every metric array constant (fused kernels allowed)

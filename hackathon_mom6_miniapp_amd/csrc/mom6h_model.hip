@@ -171,7 +171,7 @@ int momentum_phase(mom6h_model *m, const double *u, const double *v, const doubl
         a.d = d; a.g = m->ug; a.Kh = m->c.Kh; a.h_neglect = 1.0e-3; a.fdt = fdt; a.no_slip = m->c.no_slip;
         a.u = u; a.v = v; a.h = h; a.bu = m->u; a.bv = m->v; a.ou = ou; a.ov = ov;
         a.mask2dT = m->mask2dT; a.mask2dBu = m->mask2dBu; a.CoriolisBu = m->CoriolisBu;
-        a.Ah = m->c.Ah; a.del2u = m->del2u; a.del2v = m->del2v;
+        a.Ah = m->c.Ah; a.del2u = m->del2u; a.del2v = m->del2v; a.nkz = m->momentum_splits;
         return launch_momentum(a, s0, m->momentum_tile);
     }
     { StageTimer t(m, ST_TRANSPORTS, s0);
@@ -212,7 +212,7 @@ int momentum_phase_f32(mom6h_model *m, const float *u, const float *v, const flo
     a.no_slip = m->c.no_slip;
     a.u = u; a.v = v; a.h = h; a.bu = m->f.u; a.bv = m->f.v; a.ou = ou; a.ov = ov;
     a.mask2dT = m->mask2dT; a.mask2dBu = m->mask2dBu; a.CoriolisBu = m->CoriolisBu;
-    a.Ah = (float)m->c.Ah; a.del2u = m->f.del2u; a.del2v = m->f.del2v;
+    a.Ah = (float)m->c.Ah; a.del2u = m->f.del2u; a.del2v = m->f.del2v; a.nkz = m->momentum_splits;
     return launch_momentum(a, m->s0, m->momentum_tile);
 }
 
@@ -668,10 +668,26 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
     const std::string k(key);
     if (k == "graph") m->use_graph = value != 0.0;
     else if (k == "serialize") m->serialize = value != 0.0;
-    else if (k == "momentum_tile") {
-        const int ty = (int)value;
-        MOM6H_REQUIRE(ty == 4 || ty == 8 || ty == 16, "mom6h_model_set_option: momentum_tile must be 4, 8 or 16");
-        m->momentum_tile = ty;
+    else if (k == "bt_tile") {
+        MOM6H_REQUIRE(m->bt_cs, "mom6h_model_set_option: bt_tile after prepare");
+        MOM6H_REQUIRE(value == 4.0 || value == 8.0 || value == 16.0, "mom6h_model_set_option: bt_tile must be 4, 8 or 16");
+        m->bt_cs->tile_rows = (int)value;
+        if (m->gexec) {
+            (void)hipGraphExecDestroy(m->gexec);
+            (void)hipGraphDestroy(m->graph);
+            m->gexec = nullptr;
+            m->graph = nullptr;
+        }
+    }
+    else if (k == "momentum_tile" || k == "momentum_splits") {
+        const int v = (int)value;
+        if (k == "momentum_tile") {
+            MOM6H_REQUIRE(v == 4 || v == 8 || v == 16, "mom6h_model_set_option: momentum_tile must be 4, 8 or 16");
+            m->momentum_tile = v;
+        } else {
+            MOM6H_REQUIRE(v >= 1 && v <= m->c.nk, "mom6h_model_set_option: momentum_splits must be 1..nk");
+            m->momentum_splits = v;
+        }
         if (m->gexec) {   // the captured step holds the old launch shape
             (void)hipGraphExecDestroy(m->gexec);
             (void)hipGraphDestroy(m->graph);
@@ -729,12 +745,13 @@ int mom6h_model_autotune(mom6h_model *m, int reps, char *report, int nbytes)
     hipEvent_t a, b;
     MOM6H_CHECK(hipEventCreate(&a));
     MOM6H_CHECK(hipEventCreate(&b));
-    const int cand[3] = {4, 8, 16};
+    const int rows[3] = {4, 8, 16}, splits[4] = {1, 2, 4, 8};
     float best = 0.f;
-    int pick = 8, rc = 0;
-    std::string rep = "momentum tile rows:";
-    for (int q = 0; q < 3 && !rc; ++q) {
-        m->momentum_tile = cand[q];
+    int pick = 8, pick_s = 1, rc = 0;
+    std::string rep = "momentum tile rows x layer splits (ms):";
+    for (int q = 0; q < 12 && !rc; ++q) {
+        m->momentum_tile = rows[q / 4];
+        m->momentum_splits = splits[q % 4] < m->c.nk ? splits[q % 4] : m->c.nk;
         rc = launch();   // warm
         if (!rc && hipEventRecord(a, m->s0) != hipSuccess) rc = set_error(MOM6H_ESTATE, "event record failed");
         for (int r = 0; r < reps && !rc; ++r) rc = launch();
@@ -744,17 +761,19 @@ int mom6h_model_autotune(mom6h_model *m, int reps, char *report, int nbytes)
         if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = set_error(MOM6H_ESTATE, "event timing failed");
         ms /= (float)(reps > 0 ? reps : 1);
         char buf[64];
-        snprintf(buf, sizeof(buf), " %d: %.4f ms;", cand[q], ms);
+        snprintf(buf, sizeof(buf), " %dx%d %.4f;", m->momentum_tile, m->momentum_splits, ms);
         rep += buf;
         if (q == 0 || ms < best) {
             best = ms;
-            pick = cand[q];
+            pick = m->momentum_tile;
+            pick_s = m->momentum_splits;
         }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     m->profile = saved_profile;
     m->momentum_tile = pick;
+    m->momentum_splits = pick_s;
     if (m->gexec) {
         (void)hipGraphExecDestroy(m->gexec);
         (void)hipGraphDestroy(m->graph);
@@ -762,7 +781,7 @@ int mom6h_model_autotune(mom6h_model *m, int reps, char *report, int nbytes)
         m->graph = nullptr;
     }
     if (rc) return rc;
-    rep += " chosen " + std::to_string(pick);
+    rep += " chosen " + std::to_string(pick) + "x" + std::to_string(pick_s);
     if (report && nbytes > 0) snprintf(report, (size_t)nbytes, "%s", rep.c_str());
     return MOM6H_OK;
 }

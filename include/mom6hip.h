@@ -250,7 +250,9 @@ int mom6h_model_step(mom6h_model *m, int niter);
 int mom6h_model_synchronize(mom6h_model *m);
 /* Options: "graph" (1 = capture one step, both streams, as a hipGraph and
  * replay it per step; default 0), "momentum_tile" (rows of the fused momentum
- * tile: 4, 8 or 16; default 8), "fused" (1 = fused momentum stencil and
+ * tile: 4, 8 or 16; default 8), "momentum_splits" (blocks per tile sharing its
+ * layers, 1..nk; default 1), "bt_tile" (rows of the barotropic substep tile,
+ * 4, 8 or 16; after prepare; default 8), "fused" (1 = fused momentum stencil and
  * on-chip vert_visc coefficients where the grid allows, default 1; 0 = the
  * module-by-module path), "profile" (1 = record per-stage HIP events; the
  * graph is not used while profiling), "serialize" (1 = run the barotropic
@@ -262,8 +264,9 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value);
 int mom6h_model_stage_times(mom6h_model *m, double *ms, int max_stages, char *names, int nbytes);
 int mom6h_model_main_stream(mom6h_model *m, void **stream);
 /* Times the fused momentum stencil (reps launches of the predictor's, whose
- * outputs every step rewrites) for each tile shape (4, 8, 16 rows), keeps the
- * fastest as option "momentum_tile" and writes the timings into report. */
+ * outputs every step rewrites) for each launch shape -- tile rows 4, 8, 16 x
+ * layer splits 1, 2, 4, 8 -- keeps the fastest as options "momentum_tile" and
+ * "momentum_splits" and writes the timings into report. */
 int mom6h_model_autotune(mom6h_model *m, int reps, char *report, int nbytes);
 int mom6h_model_destroy(mom6h_model *m);
 
@@ -284,10 +287,12 @@ int mom6h_ensemble_prepare(mom6h_ensemble *e);
 int mom6h_ensemble_step(mom6h_ensemble *e, int niter);
 int mom6h_ensemble_synchronize(mom6h_ensemble *e);
 /* Options: "graph" (1 = capture one step, replay it per step), "momentum_tile"
- * (rows of the fused momentum tile: 4, 8 or 16; default 8). */
+ * (rows of the fused momentum tile: 4, 8 or 16; default 8), "momentum_splits"
+ * (blocks per tile sharing its layers; after prepare; default 1), "bt_tile" (4, 8 or 16). */
 int mom6h_ensemble_set_option(mom6h_ensemble *e, const char *key, double value);
 /* Times the fused momentum launch of the ensemble (reps launches) for every
- * tile shape, keeps the fastest and writes the timings into report.  It runs
+ * launch shape (tile rows x layer splits), keeps the fastest and writes the
+ * timings into report.  It runs
  * only the predictor's momentum stencil, whose outputs every step rewrites,
  * so the state is unchanged. */
 int mom6h_ensemble_autotune(mom6h_ensemble *e, int reps, char *report, int nbytes);

@@ -129,3 +129,28 @@ def test_meridional_pass_bit_exact(name, mode):
             bad.append(f"{f}: max|diff|={d.max():.3e} at {np.unravel_index(d.argmax(), d.shape)} n={(d > 0).sum()}")
     gpu.close()
     assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("shape", [(4, 1, 8), (16, 2, 16), (8, 4, 4), (8, 3, 4)])
+def test_launch_shapes_give_identical_results(shape):
+    """Every momentum tile / layer split / barotropic tile choice computes the
+    same values (the autotuner only picks among equals)."""
+    from hackathon_mom6_miniapp_amd.model import Model
+
+    ty, nkz, bt = shape
+    print(f"[gpu shapes] momentum {ty}x{nkz}, bt tile {bt}", flush=True)
+    cfg, init, _ = CASES["zonal_periodic_walls_beta"]
+    case = make_case(cfg, init=init, seed=12)
+    ref = Model(cfg, case)
+    alt = Model(cfg, case)
+    alt.set_option("momentum_tile", ty)
+    alt.set_option("momentum_splits", nkz)
+    alt.set_option("bt_tile", bt)
+    ref.step(2)
+    alt.step(2)
+    ref.synchronize()
+    alt.synchronize()
+    for f in ("u", "v", "h", "up", "vp", "eta", "ubt_av", "vbt_av", "eta_av"):
+        assert np.array_equal(ref.get(f), alt.get(f)), f
+    ref.close()
+    alt.close()
