@@ -139,7 +139,7 @@ def mixed_form(cfg, args, t_step_fp64):
     from hackathon_mom6_miniapp_amd.state import make_case
 
     mcfg = replace(cfg, precision="mixed")
-    m = Model(mcfg, make_case(mcfg, init="seeded", seed=2024, lazy3d=True))
+    m = Model(mcfg, make_case(mcfg, init="seeded", seed=2024, lazy3d=True, bump=args.bump), graph=args.graph)
     tuned = m.autotune(3)
     m.step(args.warmup)
     m.synchronize()
@@ -155,8 +155,8 @@ def mixed_form(cfg, args, t_step_fp64):
     n = args.mixed_steps
     if n > 0:
         errs = {}
-        ref = Model(cfg, make_case(cfg, init="seeded", seed=2024, lazy3d=True))
-        mix = Model(mcfg, make_case(mcfg, init="seeded", seed=2024, lazy3d=True))
+        ref = Model(cfg, make_case(cfg, init="seeded", seed=2024, lazy3d=True, bump=args.bump), graph=args.graph)
+        mix = Model(mcfg, make_case(mcfg, init="seeded", seed=2024, lazy3d=True, bump=args.bump), graph=args.graph)
         ref.step(n)
         mix.step(n)
         ref.synchronize()
@@ -179,6 +179,75 @@ def mixed_form(cfg, args, t_step_fp64):
     return res
 
 
+def ensemble_sizes(seed=5, n=16):
+    """Member grids of the ensemble workload: ni x nj drawn between 120x120 and
+    480x360 and nk from {25, 50, 75}, from a seeded generator."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(120, 481)), int(rng.integers(120, 361)), int(rng.choice([25, 50, 75])))
+            for _ in range(n)]
+
+
+def ensemble_bench(args):
+    """16 members of seeded sizes, one launch per stage, mixed precision,
+    launch shape from the autotuner; throughput of the whole ensemble and each
+    member's error against the FP64 ensemble after --mixed-steps steps."""
+    from dataclasses import replace
+
+    import numpy as np
+
+    from hackathon_mom6_miniapp_amd.model import Ensemble
+    from hackathon_mom6_miniapp_amd.state import make_case
+
+    sizes = ensemble_sizes()
+    cfgs = [workload_config(ni, nj, nk, args.bt) for ni, nj, nk in sizes]
+    mcfgs = [replace(c, precision="mixed") for c in cfgs]
+    cases = [make_case(c, init="seeded", seed=300 + q, lazy3d=True) for q, c in enumerate(mcfgs)]
+    cells = sum(c.cells for c in cfgs)
+    ens = Ensemble(mcfgs, cases, graph=True)
+    tuned = ens.autotune(3)
+    ens.step(args.warmup)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(args.steps)
+    ens.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    ens.close()
+    print(f"[bench] ensemble of {len(cfgs)}: {t_step * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+    out = {"metric": "ensemble cell-updates/s (16 members, one launch per stage)", "value": cells / t_step,
+           "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 layer fields and stencils; f64 vertical solve, barotropic solver, 2D fields",
+           "data": "synthetic (seeded SplitMix64/Irwin-Hall state per member)",
+           "config": {"workload": "ensemble of 16 members, beta-plane, zonally periodic, walls in y, Kh=1e3, "
+                                  f"bt_nsteps={args.bt}, mixed precision, graph-captured step",
+                      "members": [f"{ni}x{nj}x{nk}" for ni, nj, nk in sizes], "cells": cells},
+           "autotune": tuned}
+    n = args.mixed_steps
+    if n > 0:
+        mix = Ensemble(mcfgs, [make_case(c, init="seeded", seed=300 + q, lazy3d=True) for q, c in enumerate(mcfgs)])
+        ref = Ensemble(cfgs, [make_case(c, init="seeded", seed=300 + q, lazy3d=True) for q, c in enumerate(cfgs)])
+        mix.step(n)
+        ref.step(n)
+        mix.synchronize()
+        ref.synchronize()
+        worst = {}
+        per_member = []
+        for q in range(len(cfgs)):
+            e = {}
+            for f in ("h", "u", "v"):
+                a, b = mix.members[q].get(f), ref.members[q].get(f)
+                e[f] = float(np.abs(a - b).max() / np.abs(b).max())
+                worst[f] = max(worst.get(f, 0.0), e[f])
+            per_member.append(e)
+        mix.close()
+        ref.close()
+        out["errors_vs_fp64"] = {"steps": n, "normalised_by": "max|fp64| of each member's field",
+                                 "worst": worst, "members": per_member}
+        print(f"[bench] ensemble mixed vs fp64 after {n} steps, worst: {worst}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,7 +263,20 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
     ap.add_argument("--mixed-steps", type=int, default=100,
                     help="steps of the FP64-vs-mixed error check (0: timing only, -1: skip the mixed form)")
+    ap.add_argument("--config", choices=("dropin", "large", "ensemble"), default="dropin",
+                    help="dropin: 720x720x75 bt 50 (default); large: 1440x1440x75, bt 100, Gaussian SSH bump "
+                         "(0.5 m, 100 km), graph-captured step; ensemble: 16 members of seeded sizes in one "
+                         "launch per stage")
     args = ap.parse_args()
+    if args.config == "large":
+        args.ni = args.nj = 1440
+        args.bt = 100
+    if args.config == "ensemble":
+        if args.gpus != 1:
+            print("bench.py: one GPU only", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps(ensemble_bench(args)))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not code)", file=sys.stderr)
@@ -205,10 +287,12 @@ def main():
     from hackathon_mom6_miniapp_amd._lib import lib, check
 
     cfg = workload_config(args.ni, args.nj, args.nk, args.bt, args.biharmonic)
+    args.bump = (0.5, 100.0e3) if args.config == "large" else None
+    args.graph = args.config == "large"
     t0 = time.perf_counter()
-    case = make_case(cfg, init="seeded", seed=2024, lazy3d=True)
+    case = make_case(cfg, init="seeded", seed=2024, lazy3d=True, bump=args.bump)
     print(f"[bench] state generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    model = Model(cfg, case)
+    model = Model(cfg, case, graph=args.graph)
     del case
     L = lib()
     # launch shape of the fused momentum stencil from the autotuner (set-up,
@@ -272,7 +356,9 @@ def main():
         "data": "synthetic (seeded SplitMix64/Irwin-Hall state)",
         "config": {"workload": f"split-RK2 step FP64 {cfg.ni}x{cfg.nj}x{cfg.nk} bt_nsteps={cfg.bt_nsteps}, "
                                "beta-plane, zonally periodic, walls in y, Kh=1e3" +
-                               (f", biharmonic Kh4={cfg.Ah:g}" if cfg.biharmonic else ""),
+                               (f", biharmonic Kh4={cfg.Ah:g}" if cfg.biharmonic else "") +
+                               (", Gaussian SSH bump 0.5 m / 100 km" if args.bump else "") +
+                               (", graph-captured step" if args.graph else ""),
                    "ni": cfg.ni, "nj": cfg.nj, "nk": cfg.nk, "bt_nsteps": cfg.bt_nsteps, "dt": cfg.dt,
                    "parallelism": "single GPU"},
         "step_algorithmic_GBps": step_bytes(cfg) / t_step / 1e9,

@@ -46,6 +46,8 @@ _SIGS = {
     "mom6h_stream_synchronize": ([P], C.c_int),
     "mom6h_seeded_normal": ([P, C.c_long, C.c_ulonglong, C.c_int, C.c_int, C.c_double, C.c_double,
                              P, C.c_long, P], C.c_int),
+    "mom6h_bt_cfl": ([C.c_double, C.c_int, C.c_double, C.c_double, D, D, C.POINTER(C.c_int),
+                      C.POINTER(C.c_int)], C.c_int),
     "mom6h_model_create": ([C.POINTER(P), C.POINTER(MConfig)], C.c_int),
     "mom6h_model_field": ([P, C.c_char_p, C.POINTER(P), C.POINTER(C.c_long)], C.c_int),
     "mom6h_model_field_bytes": ([P, C.c_char_p, C.POINTER(C.c_int)], C.c_int),

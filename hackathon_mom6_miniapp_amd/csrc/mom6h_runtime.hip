@@ -1,5 +1,6 @@
 // mom6h_runtime.hip -- runtime helpers, error reporting and the driver's
 // inline loops (transports, velocity updates, periodic halo refill).
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -381,6 +382,22 @@ int mom6h_halo_fill(int isd, int ied, int jsd, int jed, int isc, int iec, int js
     Dims d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, nk);
     Halo hm{periodic_x, periodic_y};
     return halo_fill(d, hm, a, nk, (hipStream_t)stream);
+}
+
+int mom6h_bt_cfl(double dt, int nsteps, double dx_km, double depth, double *cfl, double *cfl_2d, int *min_nsteps,
+                 int *status)
+{
+    MOM6H_REQUIRE(cfl && cfl_2d && min_nsteps && status, "mom6h_bt_cfl: NULL output");
+    MOM6H_REQUIRE(dt > 0.0 && nsteps >= 1 && dx_km > 0.0 && depth > 0.0, "mom6h_bt_cfl: bad arguments");
+    // check_bt_cfl (app/rk2_serial_driver.F90:625-662), G_EARTH = 9.80 (mom6_types.F90:80)
+    const double dx_m = dx_km * 1000.0;
+    const double dtbt = dt / (double)nsteps;
+    const double c_grav = sqrt(9.80 * depth);
+    *cfl = c_grav * dtbt / dx_m;
+    *cfl_2d = *cfl * sqrt(2.0);
+    *min_nsteps = (int)ceil(dt * c_grav / dx_m * 2.5);
+    *status = *cfl_2d > 0.9 ? 2 : (*cfl_2d > 0.7 ? 1 : 0);
+    return MOM6H_OK;
 }
 
 int mom6h_seeded_normal(double *out, long count, unsigned long long seed, int field, int add_mean,

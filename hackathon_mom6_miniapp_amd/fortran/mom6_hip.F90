@@ -201,6 +201,14 @@ module mom6_hip_capi
             integer(c_int), value :: isd, ied, jsd, jed, isc, iec, jsc, jec, nk, periodic_x, periodic_y
             type(c_ptr), value :: a, stream
         end function
+        integer(c_int) function mom6h_bt_cfl(dt, nsteps, dx_km, depth, cfl, cfl_2d, min_nsteps, status) &
+                bind(c, name='mom6h_bt_cfl')
+            import :: c_int, c_double
+            real(c_double), value :: dt, dx_km, depth
+            integer(c_int), value :: nsteps
+            real(c_double), intent(out) :: cfl, cfl_2d
+            integer(c_int), intent(out) :: min_nsteps, status
+        end function
     end interface
 end module mom6_hip_capi
 

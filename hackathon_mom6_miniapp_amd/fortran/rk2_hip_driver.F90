@@ -140,6 +140,8 @@ program rk2_hip_driver
     tauy = 0.0_dp
     ray = 1.0e-4_dp
 
+    call check_bt_cfl(dt, bt_nsteps, 10.0_dp, 4000.0_dp)
+
     ! control structures (the module inits of :130-175)
     call hip_check(mom6h_set_device(0_c_int), 'mom6h_set_device')
     call continuity_init_hip(cont_CS, isd, ied, jsd, jed, isc, iec, jsc, jec, nk, IareaT, Idxv, dxv, one, &
@@ -260,4 +262,35 @@ program rk2_hip_driver
     call hip_free(d_eta); call hip_free(d_ubt); call hip_free(d_vbt)
     call hip_free(d_ubt_av); call hip_free(d_vbt_av); call hip_free(d_eta_av)
     call hip_free(d_taux); call hip_free(d_tauy); call hip_free(d_ray); call hip_free(d_dyCu); call hip_free(d_dxCv)
+
+contains
+
+    !> The reference driver's barotropic CFL report (rk2_serial_driver.F90:625-662),
+    !> numbers from mom6h_bt_cfl
+    subroutine check_bt_cfl(dt, nsteps, dx_km, depth)
+        real(dp), intent(in) :: dt, dx_km, depth
+        integer, intent(in) :: nsteps
+        real(c_double) :: cfl, cfl_2d
+        integer(c_int) :: min_nsteps, status
+        call hip_check(mom6h_bt_cfl(dt, int(nsteps, c_int), dx_km, depth, cfl, cfl_2d, min_nsteps, status), &
+                       'mom6h_bt_cfl')
+        print '(A)', ''
+        print '(A)', 'Barotropic CFL Check:'
+        print '(A,F8.1,A)', '  Gravity wave speed: ', sqrt(9.80_dp * depth), ' m/s'
+        print '(A,F8.3,A)', '  BT timestep:        ', dt / real(nsteps, dp), ' s'
+        print '(A,F8.4)', '  CFL (1D):           ', cfl
+        print '(A,F8.4)', '  CFL (2D diagonal):  ', cfl_2d
+        if (status == 2) then
+            print '(A)', ''
+            print '(A)', '  *** WARNING: CFL > 0.9 - UNSTABLE! ***'
+            print '(A,I4,A)', '  Recommend at least ', min_nsteps, ' substeps'
+        else if (status == 1) then
+            print '(A)', ''
+            print '(A)', '  * CAUTION: CFL > 0.7 - marginally stable'
+            print '(A,I4,A)', '  Recommend at least ', min_nsteps, ' substeps'
+        else
+            print '(A)', '  CFL OK'
+        end if
+    end subroutine check_bt_cfl
+
 end program rk2_hip_driver

@@ -198,6 +198,14 @@ int mom6h_vel_update(int isd, int ied, int jsd, int jed, int isc, int iec, int j
 int mom6h_halo_fill(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk,
                     int periodic_x, int periodic_y, double *a, void *stream);
 
+/* Barotropic CFL of the split time step (check_bt_cfl,
+ * app/rk2_serial_driver.F90:625-662): 1D and 2D-diagonal CFL of the gravity
+ * wave sqrt(g*depth) at dt/nsteps, the substep count the reference recommends
+ * (ceiling(2.5*dt*c/dx)), status 0 = OK, 1 = CFL_2d > 0.7, 2 = CFL_2d > 0.9.
+ * Host only. */
+int mom6h_bt_cfl(double dt, int nsteps, double dx_km, double depth, double *cfl, double *cfl_2d,
+                 int *min_nsteps, int *status);
+
 /* Seeded initial state on the device (no counterpart in the reference, whose
  * driver uses an analytic state): N(mean, sigma) samples from SplitMix64 and
  * Irwin-Hall sums of 12 uniforms, bit-identical to the host generator.

@@ -82,3 +82,22 @@ def test_bench_refuses_more_than_one_gpu():
     import sys
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+@pytest.mark.parametrize("dt,nsteps,status", [(300.0, 50, 0), (300.0, 4, 2), (300.0, 10, 1)])
+def test_bt_cfl_matches_closed_form(dt, nsteps, status):
+    """mom6h_bt_cfl against the closed form of check_bt_cfl
+    (app/rk2_serial_driver.F90:625-662): c = sqrt(9.8*4000), CFL = c*dt/n/dx,
+    CFL_2d = sqrt(2)*CFL, recommended substeps ceil(2.5*dt*c/dx)."""
+    import math
+
+    from hackathon_mom6_miniapp_amd._lib import lib
+    L = lib()
+    cfl, cfl2, nmin, st = ctypes.c_double(), ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+    assert L.mom6h_bt_cfl(ctypes.c_double(dt), nsteps, ctypes.c_double(10.0), ctypes.c_double(4000.0),
+                          ctypes.byref(cfl), ctypes.byref(cfl2), ctypes.byref(nmin), ctypes.byref(st)) == 0
+    c = math.sqrt(9.8 * 4000.0)
+    assert cfl.value == pytest.approx(c * dt / nsteps / 1.0e4, rel=1e-15)
+    assert cfl2.value == pytest.approx(cfl.value * math.sqrt(2.0), rel=1e-15)
+    assert nmin.value == math.ceil(dt * c / 1.0e4 * 2.5) == 15
+    assert st.value == status
