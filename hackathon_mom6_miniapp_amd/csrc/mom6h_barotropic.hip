@@ -12,6 +12,7 @@
 // reference never writes (uhbt on I = is-1 and ie, vhbt on J = js-1 and je
 // with the narrow ranges) read as zero, as they do there.
 #include <utility>
+#include <type_traits>
 #include <vector>
 
 #include "mom6h_internal.h"
@@ -54,7 +55,9 @@ template <typename M, int BT_TY>
 __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &st, int tile_x, int tile_y)
 {
     constexpr int BT_H = BT_TY + 3, NOUT = BT_TY / 4;
+    constexpr bool UNI = std::is_same<M, BTUniform>::value;   // Datv and f4 staged with the state
     __shared__ double s_e[BT_H][BT_W], s_u[BT_H][BT_W], s_v[BT_H][BT_W];
+    __shared__ double s_dv[UNI ? BT_H : 1][UNI ? BT_W : 1], s_f4[UNI ? BT_H : 1][UNI ? BT_W : 1];
     const Dims &d = p.d;
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
     const int i0 = d.isd + tile_x * BT_TX, j0 = d.jsd + tile_y * BT_TY;
@@ -67,14 +70,41 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
     // L: old state on [i0-2, i1+1] x [j0-2, j1+1]
     for (int t = tid; t < BT_W * BT_H; t += 256) {
         const int li = t % BT_W, lj = t / BT_W, i = bi + li, j = bj + lj;
-        double e = 0.0, uu = 0.0, vv = 0.0;
+        double e = 0.0, uu = 0.0, vv = 0.0, dv = 0.0, f4 = 0.0;
         if (i >= d.isd && i <= d.ied && j >= d.jsd && j <= d.jed) {
             const long long c = IX2(d, i, j);
             e = st.eta_o[c]; uu = st.ubt_o[c]; vv = st.vbt_o[c];
+            if (UNI) { dv = m.datv(c); f4 = m.f4u(1, c); }
         }
         s_e[lj][li] = e; s_u[lj][li] = uu; s_v[lj][li] = vv;
+        if (UNI) { s_dv[UNI ? lj : 0][UNI ? li : 0] = dv; s_f4[UNI ? lj : 0][UNI ? li : 0] = f4; }
+    }
+    // the running averages of this thread's outputs
+    double ua_r[NOUT], va_r[NOUT];
+#pragma unroll
+    for (int r = 0; r < NOUT; ++r) {
+        const int t = tid + 256 * r;
+        const int i = i0 + t % BT_TX, j = j0 + t / BT_TX;
+        ua_r[r] = 0.0; va_r[r] = 0.0;
+        if (i <= d.ied && j <= d.jed) {
+            ua_r[r] = p.ubt_av[IX2(d, i, j)];
+            va_r[r] = p.vbt_av[IX2(d, i, j)];
+        }
     }
     __syncthreads();
+    // coefficient access: staged values (uniform grids) or the arrays
+    auto DV = [&](int lj, int li, long long c) -> double {
+        if constexpr (UNI) return s_dv[lj][li];
+        else return m.datv(c);
+    };
+    auto F4U = [&](int comp, int lj, int li, long long c) -> double {
+        if constexpr (UNI) return s_f4[lj][li];
+        else return m.f4u(comp, c);
+    };
+    auto F4V = [&](int comp, int lj, int li, long long c) -> double {
+        if constexpr (UNI) return s_f4[lj][li];
+        else return m.f4v(comp, c);
+    };
     // old values the transports of this thread's outputs need
     double e_old[NOUT], u_old[NOUT], uw_old[NOUT], v_old[NOUT], vs_old[NOUT];
 #pragma unroll
@@ -93,7 +123,7 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
         const long long c = IX2(d, i, j);
         s_e[lj][li] = s_e[lj][li] + (dtbt * m.iareaT(c)) *
             (((m.datu(IX2(d, i - 1, j)) * s_u[lj][li - 1]) - (m.datu(c) * s_u[lj][li])) +
-             ((m.datv(IX2(d, i, j - 1)) * s_v[lj - 1][li]) - (m.datv(c) * s_v[lj][li])));
+             ((DV(lj - 1, li, IX2(d, i, j - 1)) * s_v[lj - 1][li]) - (DV(lj, li, c) * s_v[lj][li])));
     }
     __syncthreads();
     // pressure forces (:234-247) from the predicted eta
@@ -107,8 +137,8 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
     auto new_v = [&](int lj, int li, int i, int j, long long c) -> double {
         double vn = s_v[lj][li];
         if (i >= is && i <= ie && j >= js && j <= je - 1) {
-            const double Cor = -1.0 * (((m.f4v(1, c) * s_u[lj][li - 1]) + (m.f4v(4, c) * s_u[lj + 1][li])) +
-                                       ((m.f4v(2, c) * s_u[lj][li]) + (m.f4v(3, c) * s_u[lj + 1][li - 1])));
+            const double Cor = -1.0 * (((F4V(1, lj, li, c) * s_u[lj][li - 1]) + (F4V(4, lj, li, c) * s_u[lj + 1][li])) +
+                                       ((F4V(2, lj, li, c) * s_u[lj][li]) + (F4V(3, lj, li, c) * s_u[lj + 1][li - 1])));
             vn = m.remv(c) * (vn + dtbt * (Cor + pf_v(lj, li, i, j, c)));
         }
         return vn;
@@ -116,8 +146,8 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
     auto new_u = [&](int lj, int li, int i, int j, long long c) -> double {
         double un = s_u[lj][li];
         if (i >= is && i <= ie - 1 && j >= js && j <= je) {
-            const double Cor = (((m.f4u(4, c) * s_v[lj][li + 1]) + (m.f4u(1, c) * s_v[lj - 1][li])) +
-                                ((m.f4u(3, c) * s_v[lj][li]) + (m.f4u(2, c) * s_v[lj - 1][li + 1])));
+            const double Cor = (((F4U(4, lj, li, c) * s_v[lj][li + 1]) + (F4U(1, lj, li, c) * s_v[lj - 1][li])) +
+                                ((F4U(3, lj, li, c) * s_v[lj][li]) + (F4U(2, lj, li, c) * s_v[lj - 1][li + 1])));
             un = m.remu(c) * (un + dtbt * (Cor + pf_u(lj, li, i, j, c)));
         }
         return un;
@@ -158,9 +188,9 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
         if (i < iu0 || i > iu1 || j < js || j > je) return 0.0;
         return m.datu(IX2(d, i, j)) * (trans_wt1 * un + trans_wt2 * uo);
     };
-    auto vhbt = [&](int i, int j, double vn, double vo) -> double {
+    auto vhbt = [&](int i, int j, int li, int lj, double vn, double vo) -> double {
         if (i < is || i > ie || j < jv0 || j > jv1) return 0.0;
-        return m.datv(IX2(d, i, j)) * (trans_wt1 * vn + trans_wt2 * vo);
+        return DV(lj, li, IX2(d, i, j)) * (trans_wt1 * vn + trans_wt2 * vo);
     };
 #pragma unroll
     for (int r = 0; r < NOUT; ++r) {
@@ -178,8 +208,9 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
         double e = e_old[r];
         if (in_eta)
             e = e - dtbt * m.iareaT(c) * ((uhbt(i, j, un, u_old[r]) - uhbt(i - 1, j, s_u[lj][li - 1], uw_old[r])) +
-                                          (vhbt(i, j, vn, v_old[r]) - vhbt(i, j - 1, s_v[lj - 1][li], vs_old[r])));
-        double ua = p.ubt_av[c], va = p.vbt_av[c];
+                                          (vhbt(i, j, li, lj, vn, v_old[r]) -
+                                           vhbt(i, j - 1, li, lj - 1, s_v[lj - 1][li], vs_old[r])));
+        double ua = ua_r[r], va = va_r[r];
         if (in_u) { ua = ua + un * inv_nstep; p.ubt_av[c] = ua; }
         if (in_v) { va = va + vn * inv_nstep; p.vbt_av[c] = va; }
         const bool halo_cell = i < is || i > ie || j < js || j > je;
@@ -201,8 +232,8 @@ __device__ __forceinline__ void bt_body(const BT &p, const M &m, const BTStep &s
                 }
         }
         if (last) {  // outputs (:296-302)
-            st.out_u[c] = p.ubt_av[c];
-            st.out_v[c] = p.vbt_av[c];
+            st.out_u[c] = ua;
+            st.out_v[c] = va;
             st.out_eta[c] = e;
         }
     }
@@ -323,7 +354,7 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
                       bt_rem_v && IareaT && IdxCu && IdyCv,
                   "mom6h_barotropic_init: NULL array");
     auto *cs = new mom6h_barotropic_cs();
-    cs->tile_rows = 8;
+    cs->tile_rows = 4;
     cs->d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, 1);
     cs->halo = Halo{periodic_x, periodic_y};
     cs->nstep = nstep;

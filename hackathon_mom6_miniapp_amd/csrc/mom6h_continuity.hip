@@ -570,8 +570,9 @@ __device__ __forceinline__ float col_sum(const float (*col)[CF], int nk, int lan
 // the first pass); the FP64 tolerance of 1e-12 m lies below what FP32 sums
 // resolve, and the iteration would otherwise run to its 20-pass limit.
 template <typename T, bool Y, int CW, int K>
-__device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monotonic, int bx, int by, double *smem_raw)
+__device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monotonic, unsigned bx, unsigned by)
 {
+    extern __shared__ double smem_raw[];
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int L = ContShared<T, CW, K>::L;
     ContShared<T, CW, K> &S = *reinterpret_cast<ContShared<T, CW, K> *>(smem_raw);
@@ -582,7 +583,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     const int f_raw = Y ? d.isc + bx * CF + lane : d.isc - 1 + bx * (CF - 1) + lane;
     const int f_max = d.iec;
     const int fi = f_raw > f_max ? f_max : f_raw;           // clamped: evaluated, never written
-    const int fj = Y ? d.jsc - 1 + by : d.jsc + by;
+    const int fj = Y ? d.jsc - 1 + (int)by : d.jsc + (int)by;
     const bool live = f_raw <= f_max;
     auto cell = [&](int q) __attribute__((always_inline)) -> long long {   // q-th cell of the stencil, q = 2: cell 0
         return Y ? IX2(d, fi, fj - 2 + q) : IX2(d, fi - 2 + q, fj);
@@ -853,8 +854,7 @@ template <typename T, bool Y, int CW, int K>
 __global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, (CW >= 16 || sizeof(T) == 4) ? 4 : 2)))
 k_continuity_blk(ContArgsT<T> a, int monotonic)
 {
-    extern __shared__ double smem_raw[];
-    continuity_body<T, Y, CW, K>(a, monotonic, blockIdx.x, blockIdx.y, smem_raw);
+    continuity_body<T, Y, CW, K>(a, monotonic, blockIdx.x, blockIdx.y);
 }
 
 // every member of an ensemble in one launch (K for the deepest member)
@@ -862,11 +862,10 @@ template <typename T, bool Y, int CW, int K>
 __global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, (CW >= 16 || sizeof(T) == 4) ? 4 : 2)))
 k_continuity_batch(const ContArgsT<T> *__restrict__ args, BatchMap mp)
 {
-    extern __shared__ double smem_raw[];
     int m, local;
     batch_locate(mp, blockIdx.x, m, local);
     const ContArgsT<T> a = args[m];
-    continuity_body<T, Y, CW, K>(a, a.monotonic, local % mp.nbx[m], local / mp.nbx[m], smem_raw);
+    continuity_body<T, Y, CW, K>(a, a.monotonic, local % mp.nbx[m], local / mp.nbx[m]);
 }
 
 // the driver's `h = h0` reset for the cells the convergence does not write:

@@ -84,7 +84,7 @@ struct mom6h_ensemble {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int momentum_tile = 8;
-    int bt_tile = 8;         // rows of the barotropic substep tile (4 or 8)
+    int bt_tile = 4;         // rows of the barotropic substep tile (4, 8 or 16)
     long long max_lds = 0;
     StepTables<float> tf;
     StepTables<double> td;

@@ -231,24 +231,31 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
         return;
     }
     auto Z = [&](int k) -> double & { return Zs[(k - 1) * 64 + lane]; };   // level k = 1..nk+1
-    const double Kv = p.Kv, Kv_ml = p.Kv_ml, Kv_extra_bbl = p.Kv_extra_bbl, Hmix = p.Hmix, dt = p.dt;
-    const double I_Hbbl = 1.0 / (p.Hbbl + h_neglect);
+    // C: the arithmetic type of the coefficients (z_i, bottom taper, hvel, the
+    // coupling a); the tridiagonal solve is FP64 whatever C is
+    using C = T;
+    const C hn = (C)h_neglect;
+    const C Kv = (C)p.Kv, Kv_ml = (C)p.Kv_ml, Kv_extra_bbl = (C)p.Kv_extra_bbl, Hmix = (C)p.Hmix;
+    const double dt = p.dt;
+    const C I_Hbbl = C(1.0) / ((C)p.Hbbl + hn);
+    auto botfn = [&](C z2) -> C { return C(1.0) / (C(1.0) + C(0.09) * z2 * z2 * z2 * z2 * z2 * z2); };
+    auto ZC = [&](int k) -> C { return (C)Z(k); };
     Z(nk + 1) = 0.0;
 
     // ---- sweep A, bottom-up (:160-179): z_i(k) ----
-    double zi = 0.0;
-    auto A_level = [&](int k, double ha, double hb) {
-        const double h_harm = 2.0 * ha * hb / (ha + hb + h_neglect);
+    C zi = C(0.0);
+    auto A_level = [&](int k, C ha, C hb) {
+        const C h_harm = C(2.0) * ha * hb / (ha + hb + hn);
         zi = zi + h_harm * I_Hbbl;
-        Z(k) = zi;
+        Z(k) = (double)zi;
     };
     int k0 = nk;
     if (k0 >= VV_A) {
-        double na[VV_A], nb[VV_A];
+        C na[VV_A], nb[VV_A];
 #pragma unroll
         for (int c = 0; c < VV_A; ++c) { na[c] = hA[(k0 - 1 - c) * s]; nb[c] = hB[(k0 - 1 - c) * s]; }
         for (; k0 >= VV_A; k0 -= VV_A) {   // levels k0 .. k0-VV_A+1, all >= 1
-            double ha[VV_A], hb[VV_A];
+            C ha[VV_A], hb[VV_A];
 #pragma unroll
             for (int c = 0; c < VV_A; ++c) { ha[c] = na[c]; hb[c] = nb[c]; }
             if (k0 - VV_A >= VV_A) {
@@ -265,55 +272,56 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
     for (; k0 >= 1; --k0) A_level(k0, hA[(k0 - 1) * s], hB[(k0 - 1) * s]);
 
     // hvel (:165-176) given botfn(z_i(k+1)); the upwind blend is selected, not branched
-    auto hvel = [&](double ha, double hb, double vk, double bf) -> double {
-        const double h_harm = 2.0 * ha * hb / (ha + hb + h_neglect);
-        const double h_arith = 0.5 * (hb + ha);
-        const double h_delta = hb - ha;
-        const double blend = (1.0 - bf) * h_harm + bf * h_arith;
-        return (vk * h_delta < 0.0) ? blend : h_harm;
+    auto hvel = [&](C ha, C hb, C vk, C bf) -> C {
+        const C h_harm = C(2.0) * ha * hb / (ha + hb + hn);
+        const C h_arith = C(0.5) * (hb + ha);
+        const C h_delta = hb - ha;
+        const C blend = (C(1.0) - bf) * h_harm + bf * h_arith;
+        return (vk * h_delta < C(0.0)) ? blend : h_harm;
     };
     // coupling coefficient a(K2) (:184-202) from z_top (already advanced),
     // botfn(z_i(K2)) and hvel at K2-1 and K2
-    auto coefA = [&](double z_top, double bfK, double hv_m1, double hv_0) -> double {
-        const double topfn = 1.0 - z_top / Hmix;
-        double Kv_tot = (z_top < Hmix) ? Kv + (Kv_ml - Kv) * topfn : Kv;
+    auto coefA = [&](C z_top, C bfK, C hv_m1, C hv_0) -> C {
+        const C topfn = C(1.0) - z_top / Hmix;
+        C Kv_tot = (z_top < Hmix) ? Kv + (Kv_ml - Kv) * topfn : Kv;
         Kv_tot = Kv_tot + Kv_extra_bbl * bfK;
-        return Kv_tot / (0.5 * (hv_0 + hv_m1 + h_neglect));
+        return Kv_tot / (C(0.5) * (hv_0 + hv_m1 + hn));
     };
 
     // ---- sweep B, top-down (:430-444) ----
-    double z_top, hv_n, a_n, vk_cur, b1, d1, vprev;
+    C z_top, hv_n;
+    double a_n, vk_cur, b1, d1, vprev;
     {   // level 1
-        const double v1 = vel[0];
-        const double hv1 = hvel(hA[0], hB[0], v1, botfn_of(Z(2)));
-        hv_n = 0.0;
-        z_top = 0.0;
+        const C v1 = vel[0];
+        const C hv1 = hvel(hA[0], hB[0], v1, botfn(ZC(2)));
+        hv_n = C(0.0);
+        z_top = C(0.0);
         if (nk >= 2) {
-            const double v2 = vel[s];
-            hv_n = hvel(hA[s], hB[s], v2, botfn_of(Z(3)));
-            vk_cur = v2;
+            const C v2 = vel[s];
+            hv_n = hvel(hA[s], hB[s], v2, botfn(ZC(3)));
+            vk_cur = (double)v2;
             z_top = z_top + hv1;
-            a_n = coefA(z_top, botfn_of(Z(2)), hv1, hv_n);
+            a_n = (double)coefA(z_top, botfn(ZC(2)), hv1, hv_n);
         } else {
             vk_cur = 0.0;
-            a_n = (Kv + Kv_extra_bbl) / (0.5 * hv1 + h_neglect);
+            a_n = (double)((Kv + Kv_extra_bbl) / (C(0.5) * hv1 + hn));
         }
         if (KEEP) a_out[0] = 0.0;
-        const double hu_k = hv1 + h_neglect;
+        const double hu_k = (double)hv1 + h_neglect;
         if (KEEP) hu_out[0] = hu_k;
         const double Ray = (nk == 1) ? Ray_bot : 0.0;
         const double b_denom_1 = hu_k + dt * (Ray + 0.0);
         b1 = 1.0 / (b_denom_1 + dt * a_n);
         d1 = b_denom_1 * b1;
-        vprev = b1 * (hu_k * v1 + sfc);
+        vprev = b1 * (hu_k * (double)v1 + sfc);
         vel[0] = (T)vprev;
     }
     // one interior level k (2..nk-1): the recurrence, given a(k+1) and hvel(k+1)
-    auto B_serial = [&](int k, double a_next, double hv_next, double vk_next) {
+    auto B_serial = [&](int k, double a_next, C hv_next, double vk_next) {
         const double a_k = a_n;
         if (KEEP) a_out[(k - 1) * s] = a_k;
-        const double hv_k = hv_n;
-        const double hu_k = hv_k + h_neglect;
+        const C hv_k = hv_n;
+        const double hu_k = (double)hv_k + h_neglect;
         if (KEEP) hu_out[(k - 1) * s] = hu_k;
         Z(k) = dt * a_k * b1;                              // c1(k); z_i(k) is no longer read
         const double b_denom_1 = hu_k + dt * (0.0 + a_k * d1);
@@ -327,13 +335,13 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
     };
     k0 = 2;
     if (k0 + VV_C - 1 <= nk - 1) {
-        double na[VV_C], nb[VV_C], nv[VV_C];
+        C na[VV_C], nb[VV_C], nv[VV_C];
 #pragma unroll
         for (int c = 0; c < VV_C; ++c) {   // h and u at k+1 for the chunk's levels
             na[c] = hA[(k0 + c) * s]; nb[c] = hB[(k0 + c) * s]; nv[c] = vel[(k0 + c) * s];
         }
         for (; k0 + VV_C - 1 <= nk - 1; k0 += VV_C) {   // levels k0 .. k0+VV_C-1
-            double ha[VV_C], hb[VV_C], vv[VV_C];
+            C ha[VV_C], hb[VV_C], vv[VV_C];
 #pragma unroll
             for (int c = 0; c < VV_C; ++c) { ha[c] = na[c]; hb[c] = nb[c]; vv[c] = nv[c]; }
             if (k0 + 2 * VV_C - 1 <= nk - 1) {
@@ -345,35 +353,35 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
                 }
             }
             // off-recurrence terms: bf[c] = botfn(z_i(k0+1+c)), hvel(k+1), z_top, a(k+1)
-            double bf[VV_C + 1], hvx[VV_C], ax[VV_C];
+            C bf[VV_C + 1], hvx[VV_C], ax[VV_C];
 #pragma unroll
-            for (int c = 0; c <= VV_C; ++c) bf[c] = botfn_of(Z(k0 + 1 + c));
+            for (int c = 0; c <= VV_C; ++c) bf[c] = botfn(ZC(k0 + 1 + c));
 #pragma unroll
             for (int c = 0; c < VV_C; ++c) hvx[c] = hvel(ha[c], hb[c], vv[c], bf[c + 1]);
 #pragma unroll
             for (int c = 0; c < VV_C; ++c) {
-                const double hv_k = (c == 0) ? hv_n : hvx[c - 1];
+                const C hv_k = (c == 0) ? hv_n : hvx[c - 1];
                 z_top = z_top + hv_k;
                 ax[c] = coefA(z_top, bf[c], hv_k, hvx[c]);
             }
 #pragma unroll
-            for (int c = 0; c < VV_C; ++c) B_serial(k0 + c, ax[c], hvx[c], vv[c]);
+            for (int c = 0; c < VV_C; ++c) B_serial(k0 + c, (double)ax[c], hvx[c], (double)vv[c]);
         }
     }
     for (; k0 <= nk - 1; ++k0) {
-        const double vnx = vel[k0 * s];
-        const double hvx = hvel(hA[k0 * s], hB[k0 * s], vnx, botfn_of(Z(k0 + 2)));
+        const C vnx = vel[k0 * s];
+        const C hvx = hvel(hA[k0 * s], hB[k0 * s], vnx, botfn(ZC(k0 + 2)));
         z_top = z_top + hv_n;
-        const double ax = coefA(z_top, botfn_of(Z(k0 + 1)), hv_n, hvx);
-        B_serial(k0, ax, hvx, vnx);
+        const C ax = coefA(z_top, botfn(ZC(k0 + 1)), hv_n, hvx);
+        B_serial(k0, (double)ax, hvx, (double)vnx);
     }
     if (nk >= 2) {   // bottom level: a(nk+1) from the bottom formula (:205-206), Rayleigh drag
         const int k = nk;
         const double a_k = a_n;
         if (KEEP) a_out[(k - 1) * s] = a_k;
-        const double hv_k = hv_n;
-        a_n = (Kv + Kv_extra_bbl) / (0.5 * hv_k + h_neglect);
-        const double hu_k = hv_k + h_neglect;
+        const C hv_k = hv_n;
+        a_n = (double)((Kv + Kv_extra_bbl) / (C(0.5) * hv_k + hn));
+        const double hu_k = (double)hv_k + h_neglect;
         if (KEEP) hu_out[(k - 1) * s] = hu_k;
         Z(k) = dt * a_k * b1;
         const double b_denom_1 = hu_k + dt * (Ray_bot + a_k * d1);

@@ -260,7 +260,7 @@ int mom6h_model_synchronize(mom6h_model *m);
  * replay it per step; default 0), "momentum_tile" (rows of the fused momentum
  * tile: 4, 8 or 16; default 8), "momentum_splits" (blocks per tile sharing its
  * layers, 1..nk; default 1), "bt_tile" (rows of the barotropic substep tile,
- * 4, 8 or 16; after prepare; default 8), "fused" (1 = fused momentum stencil and
+ * 4, 8 or 16; after prepare; default 4), "fused" (1 = fused momentum stencil and
  * on-chip vert_visc coefficients where the grid allows, default 1; 0 = the
  * module-by-module path), "profile" (1 = record per-stage HIP events; the
  * graph is not used while profiling), "serialize" (1 = run the barotropic
