@@ -580,7 +580,9 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     // face (fi, fj) between cell 0 = (fi, fj) and cell 1 = (fi+1, fj) [x] or (fi, fj+1) [y]
-    const int f_raw = Y ? d.isc + bx * CF + lane : d.isc - 1 + bx * (CF - 1) + lane;
+    // x: a block advances by CF-2 faces (its last lane only lends its cell to the
+    // lane below; its first lane's face feeds the convergence of the second)
+    const int f_raw = Y ? d.isc + bx * CF + lane : d.isc - 1 + bx * (CF - 2) + lane;
     const int f_max = d.iec;
     const int fi = f_raw > f_max ? f_max : f_raw;           // clamped: evaluated, never written
     const int fj = Y ? d.jsc - 1 + (int)by : d.jsc + (int)by;
@@ -602,9 +604,39 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     col.IdxT1 = (T)a.IdxT[cell(3)];
     const T mCu = (T)a.mask2dCu[c2];
     const long long cu = IX3(d, fi, fj, 1);
-    // PPM edges of cells 0 and 1 from h and mask2dT on the 6-cell stencil
-    // (zonal_edge_thickness :185-207); levels past nk repeat level nk
-    {
+    // PPM edges of cells 0 and 1 (zonal_edge_thickness :185-207); levels past
+    // nk repeat level nk.  x: each lane reconstructs its own cell 0 from the
+    // 5-cell stencil and takes cell 1 from the next lane (whose cell 0 it is),
+    // so every cell is reconstructed once.  y: both cells from the 6-cell
+    // stencil along j.
+    if (!Y) {
+        const int ci = f_raw < d.iec + 1 ? f_raw : d.iec + 1;   // this lane's cell (in bounds)
+        auto ccell = [&](int q) __attribute__((always_inline)) -> long long { return IX2(d, ci - 2 + q, fj); };
+        T m5[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) m5[q] = (T)a.mask2dT[ccell(q)];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const int kr = k0 + kk + 1;
+            const int k = kr < nk ? kr : nk;
+            const long long o = (long long)(k - 1) * s;
+            S.U[k0 + kk][lane] = a.u[cu + o];
+            T h5[5], sl[3];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) h5[q] = a.hin[ccell(q) + o];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) sl[q] = ppm_slope_v(m5[q], m5[q + 1], m5[q + 2], h5[q], h5[q + 1], h5[q + 2]);
+            ppm_edges_v(m5, h5, sl, 2, col.wW[kk], col.wE[kk]);
+            col.w0[kk] = h5[2];
+        }
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            ppm_limit(monotonic, col.w0[kk], (T)a.h_min, col.wW[kk], col.wE[kk]);
+            col.eW[kk] = __shfl_down(col.wW[kk], 1, 64);
+            col.eE[kk] = __shfl_down(col.wE[kk], 1, 64);
+            col.e0[kk] = __shfl_down(col.w0[kk], 1, 64);
+        }
+    } else {
         T m6[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) m6[q] = (T)a.mask2dT[cell(q)];
@@ -689,7 +721,9 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         T du = T(0.0), du_max = du_max_CFL, du_min = du_min_CFL;
         T uh_err = uh_tot_0 - uhbt, duhdu_tot = duhdu_tot_0;
         T uh_err_best = vabs(uh_err);
-        bool do_I = true;
+        // x: lanes past the last face and the block's last lane (no cell 1 of
+        // its own) take no Newton passes
+        bool do_I = Y || (live && lane < CF - 1);
         *du_eval = T(0.0);   // the first pass evaluated at du = 0
         for (int itt = 1; itt <= max_itts; ++itt) {
             if (do_I) {
@@ -752,7 +786,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 
     // outputs at the face: uh, u_cor, du_cor; a face shared with the western
     // neighbour block is written by that block
-    const bool writes_face = live && (Y || lane > 0 || bx == 0);
+    const bool writes_face = live && (Y || ((lane > 0 || bx == 0) && lane < CF - 1));
     if (writes_face) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
@@ -767,7 +801,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // continuity_zonal_convergence (:148-176) for cell i = I (faces I-1 and I)
     if (!Y) {
         const T dtIareaT = (T)(a.dt * a.IareaT[IX2(d, i, j)]);
-        const bool writes_cell = live && lane > 0 && i >= d.isc;
+        const bool writes_cell = live && lane > 0 && lane < CF - 1 && i >= d.isc;
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
             const T fw = __shfl_up(fk[kk], 1, 64);   // face I-1, same level
@@ -1031,7 +1065,7 @@ int continuity_blk_x(mom6h_continuity_cs *cs, const T *u, const T *hin, T *h, T 
                    bt ? bt->FA_u_EE : nullptr, bt ? bt->FA_u_E0 : nullptr, bt ? bt->FA_u_W0 : nullptr,
                    bt ? bt->FA_u_WW : nullptr, bt ? bt->uBT_WW : nullptr, bt ? bt->uBT_EE : nullptr,
                    bt ? bt_h_u : nullptr};
-    const dim3 g((d.iec - d.isc + 1 + CF - 2) / (CF - 1), d.jec - d.jsc + 1);
+    const dim3 g((d.iec - d.isc + 1 + CF - 3) / (CF - 2), d.jec - d.jsc + 1);
     int rc = launch_blk<T, false>(a, cs->monotonic, g, s);
     if (rc) return rc;
     if (hin_halo_to_h) {
@@ -1079,7 +1113,7 @@ int continuity_batch(const ContArgsT<T> *d_args, const ContArgsT<T> *h_args, int
     int nk_max = 1;
     for (int m = 0; m < n; ++m) {
         const Dims &d = h_args[m].d;
-        mp.nbx[m] = y_pass ? (d.iec - d.isc + 1 + CF - 1) / CF : (d.iec - d.isc + 1 + CF - 2) / (CF - 1);
+        mp.nbx[m] = y_pass ? (d.iec - d.isc + 1 + CF - 1) / CF : (d.iec - d.isc + 1 + CF - 3) / (CF - 2);
         mp.nby[m] = y_pass ? d.jec - d.jsc + 2 : d.jec - d.jsc + 1;
         mp.first[m + 1] = mp.first[m] + mp.nbx[m] * mp.nby[m];
         nk_max = d.nk > nk_max ? d.nk : nk_max;
