@@ -533,8 +533,16 @@ __device__ __forceinline__ double col_sum(const double (*col)[CF], int nk, int l
     return lds_fold<L>(col, nk, lane, x0, [](double a_, double x_) { return a_ + x_; });
 }
 
+// the same with each term scaled by w first: x0 + sum_k (w * col[k])
 template <int L>
-__device__ __forceinline__ float col_sum(const float (*col)[CF], int nk, int lane, float x0)
+__device__ __forceinline__ double col_sum(const double (*col)[CF], int nk, int lane, double x0, double w)
+{
+    return lds_fold<L>(col, nk, lane, x0, [w](double a_, double x_) { return a_ + w * x_; });
+}
+
+template <int L>
+__device__ __forceinline__ float col_sum(const float (*col)[CF], int nk, int lane, float x0, float w = 1.0f,
+                                         bool scaled = false)
 {
     constexpr int B = 16;
     float acc = x0, c = 0.0f;
@@ -547,7 +555,7 @@ __device__ __forceinline__ float col_sum(const float (*col)[CF], int nk, int lan
         }
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-            const float y = x[b] - c;
+            const float y = (scaled ? w * x[b] : x[b]) - c;
             const float t = acc + y;
             const float cn = (t - acc) - y;
             const bool in = k0 + b < nk;
@@ -675,15 +683,21 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             col.eval(kk, uk(kk) + du, fk[kk], hm);
             if (valid(kk)) {
                 S.S0[k0 + kk][lane] = fk[kk];
-                S.S1[k0 + kk][lane] = col.dy * hm;
+                S.S1[k0 + kk][lane] = hm;   // scaled by dy in the column sum
             }
         }
     };
-    // column sums in k order by wavefronts 0 and 1 (R[0] = x0 + sum S0, R[1] = x1 + sum S1)
+    // x1 + sum_k dy*hm(k): S1 holds the marginal thickness hm of the pass, so
+    // that the last Newton pass also yields BT_cont's h_u without a pass of its own
+    auto sum_dy = [&](T x1) __attribute__((always_inline)) -> T {
+        if constexpr (sizeof(T) == 8) return col_sum<L>(S.S1, nk, lane, x1, col.dy);
+        else return col_sum<L>(S.S1, nk, lane, x1, col.dy, true);
+    };
+    // column sums in k order by wavefronts 0 and 1 (R[0] = x0 + sum S0, R[1] = x1 + sum dy*S1)
     auto sum_pass = [&](T x0, T x1, bool on) __attribute__((always_inline)) {
         __syncthreads();
         if (w < 2 && on)
-            S.R[w][lane] = (w == 0) ? col_sum<L>(S.S0, nk, lane, x0) : col_sum<L>(S.S1, nk, lane, x1);
+            S.R[w][lane] = (w == 0) ? col_sum<L>(S.S0, nk, lane, x0) : sum_dy(x1);
         __syncthreads();
     };
 
@@ -693,7 +707,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     eval_pass(T(0.0), true);
     __syncthreads();
     if (w < 2) {
-        S.R[w][lane] = (w == 0) ? col_sum<L>(S.S0, nk, lane, T(0.0)) : col_sum<L>(S.S1, nk, lane, T(0.0));
+        S.R[w][lane] = (w == 0) ? col_sum<L>(S.S0, nk, lane, T(0.0)) : sum_dy(T(0.0));
     } else if (w == 2) {
         S.R[2][lane] = lds_fold<L>(S.U, nk, lane, T(2.0) * (CFL_dt * dx_W) * T(1.0), [&](T x, T uu) {
             return (x > dx_W * CFL_dt - uu * mCu) ? (dx_W * CFL_dt - uu) : x;
@@ -813,6 +827,20 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     }
     if (!a.bt_h_u) return;   // block-uniform
 
+    // h_u = marginal thickness at u_cor = u + du (zonal_flux_thickness :602-735):
+    // the last Newton pass's when that was at u + du, else evaluated here
+    if (writes_face) {
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            T hm = S.S1[k0 + kk][lane];
+            if (!last_at_du) {
+                T fc;
+                col.eval(kk, uk(kk) + du, fc, hm);
+            }
+            if (valid(kk)) a.bt_h_u[cu + (long long)(k0 + kk) * s] = hm;
+        }
+    }
+
     // set_zonal_BT_cont (:920-1121).  The zero-transport correction equals du
     // whenever the target is zero (identical Newton iterations).
     T du0 = du, du0_eval = du_eval;
@@ -829,15 +857,6 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         eval_pass(du0, !have0);
         sum_pass(T(0.0), T(0.0), !have0);
         if (!have0) FAmt_0 = S.R[1][lane];
-    }
-    // h_u = marginal thickness at u_cor = u + du (zonal_flux_thickness :602-735)
-    if (writes_face) {
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk) {
-            T fc, hm;
-            col.eval(kk, uk(kk) + du, fc, hm);
-            if (valid(kk)) a.bt_h_u[cu + (long long)(k0 + kk) * s] = hm;
-        }
     }
     // test velocities duR, duL (visc_rem_lim = 1)
     const T Idt = T(1.0) / (T)a.dt;
