@@ -499,29 +499,65 @@ struct ColRegs {
 
 // LDS of one block: U = u of every level (levels past nk repeat level nk; u
 // lives here rather than in registers), S0/S1 = per-level terms of a pass,
-// R = results of the column sums (one row per quantity)
+// R = results of the column sums (one row per quantity).  The column walks
+// read the 16 rows ending at row nk-1 of a column, i.e. up to 15 rows before
+// row 0: `pad` keeps those reads of U inside the block's LDS (S0 and S1 fall
+// back on the array before them); the values are never used.
+constexpr int kWalkBatch = 16;
 template <typename T, int CW, int K>
 struct ContShared {
     static constexpr int L = CW * K;
+    T pad[kWalkBatch - 1][CF];
     T U[L][CF], S0[L][CF], S1[L][CF], R[5][CF];
 };
 
-// Sequential fold of rows 0..nk-1 of one LDS column (k order), the reads
-// issued a batch ahead of the dependent chain.
+// Visits rows 0..nk-1 (nk <= L) of one LDS column of ContShared in k order: step(x_k).
+// Whole batches of B rows first, their reads issued ahead of the dependent
+// chain; then the last nk % B rows, read as the batch that ends at row nk-1
+// and entered part-way through an unrolled sequence (a jump on the uniform
+// remainder).  No step carries a per-row predicate.
+template <int L, typename T, typename Step>
+__device__ __forceinline__ void lds_walk(const T (*col)[CF], int nk, int lane, Step step)
+{
+    constexpr int B = kWalkBatch;
+    int k0 = 0;
+    for (; k0 + B <= nk; k0 += B) {
+        T x[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) x[b] = col[k0 + b][lane];
+#pragma unroll
+        for (int b = 0; b < B; ++b) step(x[b]);
+    }
+    const int skip = B - (nk - k0);   // 1 .. B: rows of the last batch already visited (B: none left)
+    if (skip == B) return;
+    T x[B];
+    const T *last = &col[0][lane] + (nk - B) * CF;   // row nk-B (>= -(B-1): see ContShared::pad)
+#pragma unroll
+    for (int b = 0; b < B; ++b) x[b] = last[b * CF];
+    switch (skip) {
+    case 1: step(x[1]); [[fallthrough]];
+    case 2: step(x[2]); [[fallthrough]];
+    case 3: step(x[3]); [[fallthrough]];
+    case 4: step(x[4]); [[fallthrough]];
+    case 5: step(x[5]); [[fallthrough]];
+    case 6: step(x[6]); [[fallthrough]];
+    case 7: step(x[7]); [[fallthrough]];
+    case 8: step(x[8]); [[fallthrough]];
+    case 9: step(x[9]); [[fallthrough]];
+    case 10: step(x[10]); [[fallthrough]];
+    case 11: step(x[11]); [[fallthrough]];
+    case 12: step(x[12]); [[fallthrough]];
+    case 13: step(x[13]); [[fallthrough]];
+    case 14: step(x[14]); [[fallthrough]];
+    default: step(x[15]);
+    }
+}
+
+// Sequential fold acc = op(acc, x_k) over rows 0..nk-1 of one LDS column
 template <int L, typename T, typename Op>
 __device__ __forceinline__ T lds_fold(const T (*col)[CF], int nk, int lane, T acc, Op op)
 {
-    constexpr int B = 16;
-    for (int k0 = 0; k0 < nk; k0 += B) {
-        T x[B];
-#pragma unroll
-        for (int b = 0; b < B; ++b) {
-            const int k = k0 + b;
-            x[b] = col[k < L ? k : L - 1][lane];
-        }
-#pragma unroll
-        for (int b = 0; b < B; ++b) acc = (k0 + b < nk) ? op(acc, x[b]) : acc;
-    }
+    lds_walk<L>(col, nk, lane, [&](T x) __attribute__((always_inline)) { acc = op(acc, x); });
     return acc;
 }
 
@@ -544,25 +580,13 @@ template <int L>
 __device__ __forceinline__ float col_sum(const float (*col)[CF], int nk, int lane, float x0, float w = 1.0f,
                                          bool scaled = false)
 {
-    constexpr int B = 16;
     float acc = x0, c = 0.0f;
-    for (int k0 = 0; k0 < nk; k0 += B) {
-        float x[B];
-#pragma unroll
-        for (int b = 0; b < B; ++b) {
-            const int k = k0 + b;
-            x[b] = col[k < L ? k : L - 1][lane];
-        }
-#pragma unroll
-        for (int b = 0; b < B; ++b) {
-            const float y = (scaled ? w * x[b] : x[b]) - c;
-            const float t = acc + y;
-            const float cn = (t - acc) - y;
-            const bool in = k0 + b < nk;
-            acc = in ? t : acc;
-            c = in ? cn : c;
-        }
-    }
+    lds_walk<L>(col, nk, lane, [&](float x) __attribute__((always_inline)) {
+        const float y = (scaled ? w * x : x) - c;
+        const float t = acc + y;
+        c = (t - acc) - y;
+        acc = t;
+    });
     return acc;
 }
 
