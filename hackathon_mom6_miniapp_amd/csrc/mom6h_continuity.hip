@@ -753,8 +753,9 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 
     // zonal_flux_adjust (:739-916).  keep: uh_3d present (fluxes of the last
     // evaluation are the result; evaluated also at itt = max_itts).  Returns
-    // du; *du_eval = du of the last evaluation that was summed (NaN if none).
-    auto flux_adjust = [&](T uhbt, bool keep, T *du_eval) __attribute__((always_inline)) -> T {
+    // du; *du_eval = du of the last evaluation that was summed (NaN if none)
+    // and *dyhm_eval = sum_k dy*hm at u + *du_eval (the last duhdu_tot).
+    auto flux_adjust = [&](T uhbt, bool keep, T *du_eval, T *dyhm_eval) __attribute__((always_inline)) -> T {
         const int max_itts = 20;
         T du = T(0.0), du_max = du_max_CFL, du_min = du_min_CFL;
         T uh_err = uh_tot_0 - uhbt, duhdu_tot = duhdu_tot_0;
@@ -813,13 +814,13 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
                 *du_eval = (T)NAN;
             }
         }
+        *dyhm_eval = duhdu_tot;
         return du;
     };
-    T du_eval;
-    const T du = flux_adjust(bt, true, &du_eval);
+    T du_eval, sum_dyhm;
+    const T du = flux_adjust(bt, true, &du_eval, &sum_dyhm);
     // here fk holds the layer fluxes of the last evaluation and, when
-    // du_eval == du, R[1] the sum of dy*hm at u + du
-    const T sum_dyhm = S.R[1][lane];
+    // du_eval == du, sum_dyhm the sum of dy*hm at u + du
     const bool last_at_du = (du_eval == du);
 
     // outputs at the face: uh, u_cor, du_cor; a face shared with the western
@@ -867,16 +868,15 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 
     // set_zonal_BT_cont (:920-1121).  The zero-transport correction equals du
     // whenever the target is zero (identical Newton iterations).
-    T du0 = du, du0_eval = du_eval;
+    T du0 = du, du0_eval = du_eval, FAmt_0 = sum_dyhm;
     if (__syncthreads_or(bt != T(0.0))) {
-        T de;
-        const T d2 = flux_adjust(T(0.0), false, &de);
-        if (bt != T(0.0)) { du0 = d2; du0_eval = de; }
+        T de, s0;
+        const T d2 = flux_adjust(T(0.0), false, &de, &s0);
+        if (bt != T(0.0)) { du0 = d2; du0_eval = de; FAmt_0 = s0; }
     }
-    // FAmt_0 = sum dy*hm(u + du0): the sum of the last evaluation when that
-    // was at du0 == du, else a pass of its own
-    const bool have0 = (bt == T(0.0)) && last_at_du && (du0_eval == du0);
-    T FAmt_0 = sum_dyhm;
+    // FAmt_0 = sum dy*hm(u + du0): the sum of the iteration's last evaluation
+    // when that was at du0 (the same terms in the same order), else a pass of its own
+    const bool have0 = (du0_eval == du0);
     if (__syncthreads_or(!have0)) {
         eval_pass(du0, !have0);
         sum_pass(T(0.0), T(0.0), !have0);
@@ -1325,6 +1325,7 @@ int mom6h_continuity_ppm(mom6h_continuity_cs *cs, const double *u, const double 
 {
     MOM6H_REQUIRE(cs != nullptr, "mom6h_continuity_ppm: NULL cs");
     MOM6H_REQUIRE(u && hin && h && uh, "mom6h_continuity_ppm: NULL array");
+    MOM6H_REQUIRE((const void *)hin != (const void *)h, "mom6h_continuity_ppm: hin and h must be distinct arrays");
     MOM6H_REQUIRE(dt > 0.0, "mom6h_continuity_ppm: dt must be positive");
     MOM6H_REQUIRE(!bt || (bt->FA_u_EE && bt->FA_u_E0 && bt->FA_u_W0 && bt->FA_u_WW && bt->uBT_WW &&
                           bt->uBT_EE && bt->h_u),
@@ -1388,6 +1389,7 @@ int mom6h_continuity_ppm_y(mom6h_continuity_cs *cs, const double *v, const doubl
     MOM6H_REQUIRE(cs != nullptr, "mom6h_continuity_ppm_y: NULL cs");
     MOM6H_REQUIRE(cs->IdyT != nullptr, "mom6h_continuity_ppm_y: call mom6h_continuity_set_meridional first");
     MOM6H_REQUIRE(v && hin && h && vh, "mom6h_continuity_ppm_y: NULL array");
+    MOM6H_REQUIRE((const void *)hin != (const void *)h, "mom6h_continuity_ppm_y: hin and h must be distinct arrays");
     MOM6H_REQUIRE(dt > 0.0, "mom6h_continuity_ppm_y: dt must be positive");
     MOM6H_REQUIRE(!bt || (bt->FA_u_EE && bt->FA_u_E0 && bt->FA_u_W0 && bt->FA_u_WW && bt->uBT_WW &&
                           bt->uBT_EE && bt->h_u),

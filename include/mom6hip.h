@@ -71,7 +71,8 @@ int mom6h_continuity_init(mom6h_continuity_cs **cs, int isd, int ied, int jsd, i
 /* u, hin -> h (compute domain), uh; uhbt (2D, NULL = zero transport target),
  * bt_cont (NULL = skip), u_cor (3D) and du_cor (2D) optional (NULL = not stored).
  * When hin_halo_to_h is non-zero, h outside the compute domain is set to hin
- * (the driver's h reset fused into the call); otherwise it is left untouched. */
+ * (the driver's h reset fused into the call); otherwise it is left untouched.
+ * hin and h are distinct arrays (the stencils read hin while h is written). */
 int mom6h_continuity_ppm(mom6h_continuity_cs *cs, const double *u, const double *hin, double *h,
                          double *uh, double dt, const double *uhbt, const mom6h_bt_cont *bt_cont,
                          double *u_cor, double *du_cor, int hin_halo_to_h, void *stream);
