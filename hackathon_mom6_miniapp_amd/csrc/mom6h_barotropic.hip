@@ -11,6 +11,7 @@
 // surface, time averages, periodic refill) on its tile.  Transports that the
 // reference never writes (uhbt on I = is-1 and ie, vhbt on J = js-1 and je
 // with the narrow ranges) read as zero, as they do there.
+#include <algorithm>
 #include <utility>
 #include <type_traits>
 #include <vector>
@@ -246,6 +247,238 @@ __global__ void __launch_bounds__(256) k_bt_substep(BT p, M m, BTStep st)
     bt_body<M, TY>(p, m, st, tile % gridDim.x, tile / gridDim.x);
 }
 
+// ---------------------------------------------------------------------------
+// Several substeps per launch (temporal blocking) for the substeps between the
+// first and the last of a call, on uniform grids.  The reference refills the
+// periodic halos after every substep but the last, so from the second substep
+// on a halo cell always holds the value of its periodic image: the substeps in
+// between run on a torus.  A block therefore stages the region its output tile
+// needs for ns substeps -- radius 2 to the west/south and 1 to the east/north
+// per substep -- and maps every staged position to its canonical cell (the
+// periodic image inside the compute domain; a position outside the arrays in
+// a wall direction holds zero and is never updated).  Each position is updated
+// as its canonical cell, with the state, Datv and f4 of the neighbouring
+// staged positions (the refilled images; Datv's halo rows equal their images,
+// which the host checks before it uses this kernel).  The region that holds
+// valid values shrinks by the stencil radius each substep and is the output
+// tile after the last.  The output tile is written whole, halo positions
+// included (their images), so the state after the launch is what the reference
+// holds after its refill.  The first substep of a call (which reads the halos
+// as stored) and the last (after which nothing is refilled) stay with the
+// one-substep kernel.
+struct BTMulti {
+    const double *eta_o, *ubt_o, *vbt_o;   // state before substep n0
+    double *eta_n, *ubt_n, *vbt_n;         // state after substep n0 + ns - 1
+    int n0, ns;                            // first substep, number of substeps (1 .. TS)
+};
+
+// Calls f(x0 + t % w, y0 + t / w) for the positions t < w*h of a rectangle of
+// width w >= 8 and at most NT*MAXI positions (<= 4096: t / w as an exact
+// multiply-shift), t spread over the block's NT threads.
+template <int NT, int MAXI, typename F>
+__device__ __forceinline__ void rect_each(int x0, int x1, int y0, int y1, int tid, F &&f)
+{
+    const unsigned w = (unsigned)(x1 - x0 + 1), n = w * (unsigned)(y1 - y0 + 1);
+    const unsigned M = ((1u << 20) + w - 1) / w;
+#pragma unroll
+    for (int r = 0; r < MAXI; ++r) {
+        const unsigned t = (unsigned)tid + (unsigned)NT * r;
+        if (t < n) {
+            const unsigned q = (t * M) >> 20;
+            f(x0 + (int)(t - q * w), y0 + (int)q);
+        }
+    }
+}
+
+template <int NT, int TX, int TY, int TS>
+__device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, const BTMulti &st, int tile_x,
+                                              int tile_y)
+{
+    constexpr int RW = TX + 3 * TS, RH = TY + 3 * TS;   // staged region (LDS)
+    constexpr int NOUT = (TX * TY + NT - 1) / NT;       // output points per thread (t < TX*TY)
+    constexpr int NR = ((TX + 3 * (TS - 1)) * (TY + 3 * (TS - 1)) + NT - 1) / NT;          // valid region
+    constexpr int NP = ((TX + 3 * (TS - 1) + 2) * (TY + 3 * (TS - 1) + 2) + NT - 1) / NT;  // update regions
+    static_assert(RW * RH <= 4096, "staged region too large for the multiply-shift index");
+    __shared__ double s_e[RH][RW], s_u[RH][RW], s_v[RH][RW], s_dv[RH][RW], s_f4[RH][RW];
+    __shared__ int s_cx[RW], s_cy[RH];
+    const Dims &d = p.d;
+    const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
+    const int i0 = d.isd + tile_x * TX, j0 = d.jsd + tile_y * TY;
+    const int bi = i0 - 2 * TS, bj = j0 - 2 * TS;   // LDS origin
+    const int tid = threadIdx.x;
+    const double dtbt = p.dtbt;
+    // canonical index of each staged column and row
+    for (int t = tid; t < RW + RH; t += NT) {
+        if (t < RW) {
+            const int x = bi + t, P = ie - is + 1;
+            s_cx[t] = p.px ? is + ((x - is) % P + P) % P : x;
+        } else {
+            const int y = bj + (t - RW), Q = je - js + 1;
+            s_cy[t - RW] = p.py ? js + ((y - js) % Q + Q) % Q : y;
+        }
+    }
+    __syncthreads();
+    rect_each<NT, (RW * RH + NT - 1) / NT>(0, RW - 1, 0, RH - 1, tid, [&](int li, int lj) {
+        const int i = s_cx[li], j = s_cy[lj];
+        double e = 0.0, uu = 0.0, vv = 0.0, dv = 0.0, f4 = 0.0;
+        if (i >= d.isd && i <= d.ied && j >= d.jsd && j <= d.jed) {
+            const long long c = IX2(d, i, j);
+            e = st.eta_o[c]; uu = st.ubt_o[c]; vv = st.vbt_o[c];
+            dv = m.Datv[c]; f4 = m.f4[c];
+        }
+        s_e[lj][li] = e; s_u[lj][li] = uu; s_v[lj][li] = vv; s_dv[lj][li] = dv; s_f4[lj][li] = f4;
+    });
+    // running averages of this thread's output points (the array positions)
+    double ua_r[NOUT], va_r[NOUT];
+#pragma unroll
+    for (int r = 0; r < NOUT; ++r) {
+        const int t = tid + NT * r;
+        const int i = i0 + t % TX, j = j0 + t / TX;
+        ua_r[r] = 0.0; va_r[r] = 0.0;
+        if (t < TX * TY && i <= d.ied && j <= d.jed) {
+            ua_r[r] = p.ubt_av[IX2(d, i, j)];
+            va_r[r] = p.vbt_av[IX2(d, i, j)];
+        }
+    }
+    __syncthreads();
+    // pressure forces and the velocity updates (:234-260) of the canonical
+    // cell (i, j) at staged position (li, lj); the coefficients are uniform
+    // but for Datv and f4
+    auto new_v = [&](int lj, int li, int i, int j) -> double {
+        double vn = s_v[lj][li];
+        if (i >= is && i <= ie && j >= js && j <= je - 1) {
+            const double f4 = s_f4[lj][li];
+            const double Cor = -1.0 * (((f4 * s_u[lj][li - 1]) + (f4 * s_u[lj + 1][li])) +
+                                       ((f4 * s_u[lj][li]) + (f4 * s_u[lj + 1][li - 1])));
+            const double pf = ((s_e[lj][li] * m.gtot) - (s_e[lj + 1][li] * m.gtot)) * p.dgeo_de * m.IdyCv;
+            vn = m.bt_rem_v * (vn + dtbt * (Cor + pf));
+        }
+        return vn;
+    };
+    auto new_u = [&](int lj, int li, int i, int j) -> double {
+        double un = s_u[lj][li];
+        if (i >= is && i <= ie - 1 && j >= js && j <= je) {
+            const double f4 = s_f4[lj][li];
+            const double Cor = (((f4 * s_v[lj][li + 1]) + (f4 * s_v[lj - 1][li])) +
+                                ((f4 * s_v[lj][li]) + (f4 * s_v[lj - 1][li + 1])));
+            const double pf = ((s_e[lj][li] * m.gtot) - (s_e[lj][li + 1] * m.gtot)) * p.dgeo_de * m.IdxCu;
+            un = m.bt_rem_u * (un + dtbt * (Cor + pf));
+        }
+        return un;
+    };
+    const double trans_wt1 = 1.0 + p.bebt, trans_wt2 = -p.bebt;
+    const double inv_nstep = 1.0 / (double)p.nstep;
+    const int iu0 = p.extended ? is - 1 : is, iu1 = p.extended ? ie : ie - 1;
+    const int jv0 = p.extended ? js - 1 : js, jv1 = p.extended ? je : je - 1;
+    for (int s = 0; s < st.ns; ++s) {
+        const int n = st.n0 + s;
+        const int k = st.ns - 1 - s;   // substeps of this launch after this one
+        // R = [A0, A1] x [B0, B1]: the positions this substep leaves valid
+        const int A0 = 2 * TS - 2 * k, A1 = 2 * TS + TX - 1 + k;
+        const int B0 = 2 * TS - 2 * k, B1 = 2 * TS + TY - 1 + k;
+        const bool v_first = ((n + p.first_direction) % 2) == 1;
+        // this thread's points of R: t = tid + NT r, (A0 + t % rw, B0 + t / rw)
+        const unsigned rw = (unsigned)(A1 - A0 + 1), nR = rw * (unsigned)(B1 - B0 + 1);
+        const unsigned Mr = ((1u << 20) + rw - 1) / rw;
+        auto r_pos = [&](int r, int &li, int &lj) -> bool {
+            const unsigned t = (unsigned)tid + (unsigned)NT * r, q = (t * Mr) >> 20;
+            li = A0 + (int)(t - q * rw);
+            lj = B0 + (int)q;
+            return t < nR;
+        };
+        // old values the transports of these points need
+        double e_old[NR], u_old[NR], uw_old[NR], v_old[NR], vs_old[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            int li, lj;
+            if (r_pos(r, li, lj)) {
+                e_old[r] = s_e[lj][li];
+                u_old[r] = s_u[lj][li]; uw_old[r] = s_u[lj][li - 1];
+                v_old[r] = s_v[lj][li]; vs_old[r] = s_v[lj - 1][li];
+            }
+        }
+        __syncthreads();
+        // eta predictor (:224-231), in place
+        rect_each<NT, NP>(A0 - 1, A1 + 1, B0 - 1, B1 + 1, tid, [&](int li, int lj) {
+            const int i = s_cx[li], j = s_cy[lj];
+            if (i < is || i > ie || j < js || j > je) return;
+            s_e[lj][li] = s_e[lj][li] + (dtbt * m.IareaT) *
+                (((m.Datu * s_u[lj][li - 1]) - (m.Datu * s_u[lj][li])) +
+                 ((s_dv[lj - 1][li] * s_v[lj - 1][li]) - (s_dv[lj][li] * s_v[lj][li])));
+        });
+        __syncthreads();
+        // the first velocity update
+        if (v_first)
+            rect_each<NT, NP>(A0 - 1, A1 + 1, B0 - 1, B1, tid, [&](int li, int lj) {
+                s_v[lj][li] = new_v(lj, li, s_cx[li], s_cy[lj]);
+            });
+        else
+            rect_each<NT, NP>(A0 - 1, A1, B0 - 1, B1 + 1, tid, [&](int li, int lj) {
+                s_u[lj][li] = new_u(lj, li, s_cx[li], s_cy[lj]);
+            });
+        __syncthreads();
+        // the second velocity update
+        if (v_first)
+            rect_each<NT, NP>(A0 - 1, A1, B0, B1, tid, [&](int li, int lj) {
+                s_u[lj][li] = new_u(lj, li, s_cx[li], s_cy[lj]);
+            });
+        else
+            rect_each<NT, NP>(A0, A1, B0 - 1, B1, tid, [&](int li, int lj) {
+                s_v[lj][li] = new_v(lj, li, s_cx[li], s_cy[lj]);
+            });
+        __syncthreads();
+        // time averages (:281-291) of the output points: the array faces
+        // inside the update ranges
+#pragma unroll
+        for (int r = 0; r < NOUT; ++r) {
+            const int t = tid + NT * r;
+            const int i = i0 + t % TX, j = j0 + t / TX;
+            const int li = 2 * TS + t % TX, lj = 2 * TS + t / TX;
+            if (t >= TX * TY) continue;
+            if (i >= is && i <= ie - 1 && j >= js && j <= je) ua_r[r] = ua_r[r] + s_u[lj][li] * inv_nstep;
+            if (i >= is && i <= ie && j >= js && j <= je - 1) va_r[r] = va_r[r] + s_v[lj][li] * inv_nstep;
+        }
+        // transports (:262-272) and the free surface (:274-279) on R, in place
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            int li, lj;
+            if (!r_pos(r, li, lj)) continue;
+            const int i = s_cx[li], j = s_cy[lj];
+            if (i < is || i > ie || j < js || j > je) continue;
+            const double un = s_u[lj][li], uw = s_u[lj][li - 1], vn = s_v[lj][li], vs = s_v[lj - 1][li];
+            const double uh_e = (i >= iu0 && i <= iu1) ? m.Datu * (trans_wt1 * un + trans_wt2 * u_old[r]) : 0.0;
+            const double uh_w = (i - 1 >= iu0 && i - 1 <= iu1) ? m.Datu * (trans_wt1 * uw + trans_wt2 * uw_old[r]) : 0.0;
+            const double vh_n = (j >= jv0 && j <= jv1) ? s_dv[lj][li] * (trans_wt1 * vn + trans_wt2 * v_old[r]) : 0.0;
+            const double vh_s = (j - 1 >= jv0 && j - 1 <= jv1) ? s_dv[lj - 1][li] * (trans_wt1 * vs + trans_wt2 * vs_old[r]) : 0.0;
+            s_e[lj][li] = e_old[r] - dtbt * m.IareaT * ((uh_e - uh_w) + (vh_n - vh_s));
+        }
+        __syncthreads();
+    }
+    // the state after the launch on the output tile, halo positions included
+#pragma unroll
+    for (int r = 0; r < NOUT; ++r) {
+        const int t = tid + NT * r;
+        const int i = i0 + t % TX, j = j0 + t / TX;
+        if (t >= TX * TY || i > d.ied || j > d.jed) continue;
+        const int li = 2 * TS + t % TX, lj = 2 * TS + t / TX;
+        const long long c = IX2(d, i, j);
+        st.eta_n[c] = s_e[lj][li]; st.ubt_n[c] = s_u[lj][li]; st.vbt_n[c] = s_v[lj][li];
+        if (i >= is && i <= ie - 1 && j >= js && j <= je) p.ubt_av[c] = ua_r[r];
+        if (i >= is && i <= ie && j >= js && j <= je - 1) p.vbt_av[c] = va_r[r];
+    }
+}
+
+// Tile of the blocked kernel: 66 x 16 covers a 720-wide grid and its halos in
+// 11 x 46 tiles, one round of two blocks per CU
+constexpr int BTM_TX = 66, BTM_TY = 16, BTM_NT = 512;
+
+template <int TS>
+__global__ void __launch_bounds__(BTM_NT) k_bt_multi(BT p, BTUniform m, BTMulti st)
+{
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    bt_multi_body<BTM_NT, BTM_TX, BTM_TY, TS>(p, m, st, tile % gridDim.x, tile / gridDim.x);
+}
+
 // ---- the ensemble form: every member's substep n in one launch.  Member
 // state ping-pongs between A (after the copy-in: odd substeps read it) and B.
 __global__ void k_bt_copy_in_batch(const BTMember *__restrict__ args, BatchMap mp)
@@ -355,6 +588,7 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
                   "mom6h_barotropic_init: NULL array");
     auto *cs = new mom6h_barotropic_cs();
     cs->tile_rows = 4;
+    cs->substeps = 3;
     cs->d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, 1);
     cs->halo = Halo{periodic_x, periodic_y};
     cs->nstep = nstep;
@@ -391,6 +625,17 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
                 if (f_4_u[4 * q + c] != f[q] || f_4_v[4 * q + c] != f[q]) uni = false;
         }
         if (uni) {
+            // the blocked kernel reads Datv of a periodic halo row at its image
+            // (the row the reference refills it from): it runs only when they agree
+            bool images = true;
+            if (periodic_y) {
+                const int n1 = ied - isd + 1;
+                for (int i = isc; images && i <= iec; ++i)
+                    for (int j = jsd; images && j < jsc; ++j)
+                        images = Datv[(size_t)(j - jsd) * n1 + (i - isd)] ==
+                                 Datv[(size_t)(j + (jec - jsc + 1) - jsd) * n1 + (i - isd)];
+            }
+            cs->blocked_ok = images ? 1 : 0;
             cs->uniform = 1;
             cs->u_Datu = Datu[0]; cs->u_gtot = gtot_E[0]; cs->u_rem_u = bt_rem_u[0]; cs->u_rem_v = bt_rem_v[0];
             cs->u_IareaT = IareaT[0]; cs->u_IdxCu = IdxCu[0]; cs->u_IdyCv = IdyCv[0];
@@ -428,8 +673,28 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
     }
     const int BT_TY = cs->tile_rows;
     dim3 grid((d.n1 + BT_TX - 1) / BT_TX, (d.n2 + BT_TY - 1) / BT_TY), block(256);
+    const dim3 grid_m((d.n1 + BTM_TX - 1) / BTM_TX, (d.n2 + BTM_TY - 1) / BTM_TY);
+    const int per_launch = cs->substeps;
     double *A[3] = {cs->eta, cs->ubt, cs->vbt}, *B[3] = {cs->eta_b, cs->ubt_b, cs->vbt_b};
-    for (int n = 1; n <= cs->nstep; ++n) {
+    for (int n = 1; n <= cs->nstep;) {
+        // substeps 2 .. nstep-1 in blocks of up to per_launch
+        if (per_launch > 1 && cs->blocked_ok && n >= 2 && n <= cs->nstep - 1) {
+            BTMulti st;
+            st.eta_o = A[0]; st.ubt_o = A[1]; st.vbt_o = A[2];
+            st.eta_n = B[0]; st.ubt_n = B[1]; st.vbt_n = B[2];
+            st.n0 = n;
+            st.ns = std::min(per_launch, cs->nstep - n);
+            const BTUniform m{cs->u_Datu, cs->u_gtot, cs->u_rem_u, cs->u_rem_v, cs->u_IareaT, cs->u_IdxCu,
+                              cs->u_IdyCv, cs->Datv, cs->f4};
+            const dim3 block_m(BTM_NT);
+            if (per_launch == 2) k_bt_multi<2><<<grid_m, block_m, 0, strm>>>(p, m, st);
+            else if (per_launch == 3) k_bt_multi<3><<<grid_m, block_m, 0, strm>>>(p, m, st);
+            else k_bt_multi<4><<<grid_m, block_m, 0, strm>>>(p, m, st);
+            MOM6H_LAUNCH_CHECK();
+            for (int q = 0; q < 3; ++q) std::swap(A[q], B[q]);
+            n += st.ns;
+            continue;
+        }
         BTStep st;
         st.eta_o = A[0]; st.ubt_o = A[1]; st.vbt_o = A[2];
         st.eta_n = B[0]; st.ubt_n = B[1]; st.vbt_n = B[2];
@@ -450,6 +715,7 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
         }
         MOM6H_LAUNCH_CHECK();
         for (int q = 0; q < 3; ++q) std::swap(A[q], B[q]);
+        ++n;
     }
     return MOM6H_OK;
 }

@@ -300,7 +300,9 @@ struct mom6h_barotropic_cs {
     int uniform;
     double u_Datu, u_gtot, u_rem_u, u_rem_v, u_IareaT, u_IdxCu, u_IdyCv;
     double *f4;
-    int tile_rows;           // rows of the substep tile (4 or 8)
+    int tile_rows;           // rows of the substep tile (4, 8 or 16)
+    int substeps;            // substeps per launch of the blocked kernel (1: one launch per substep)
+    int blocked_ok;          // uniform coefficients whose periodic Datv halo rows equal their images
     // state (two sets, ping-pong across substeps) and time averages; zero at
     // init, as the reference's never-written cells
     double *eta, *ubt, *vbt, *eta_b, *ubt_b, *vbt_b, *ubt_av, *vbt_av;

@@ -679,6 +679,18 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
             m->graph = nullptr;
         }
     }
+    else if (k == "bt_substeps") {
+        MOM6H_REQUIRE(m->bt_cs, "mom6h_model_set_option: bt_substeps after prepare");
+        MOM6H_REQUIRE(value >= 1.0 && value <= 4.0 && value == (double)(int)value,
+                      "mom6h_model_set_option: bt_substeps must be 1, 2, 3 or 4");
+        m->bt_cs->substeps = (int)value;
+        if (m->gexec) {
+            (void)hipGraphExecDestroy(m->gexec);
+            (void)hipGraphDestroy(m->graph);
+            m->gexec = nullptr;
+            m->graph = nullptr;
+        }
+    }
     else if (k == "momentum_tile" || k == "momentum_splits") {
         const int v = (int)value;
         if (k == "momentum_tile") {

@@ -154,3 +154,27 @@ def test_launch_shapes_give_identical_results(shape):
         assert np.array_equal(ref.get(f), alt.get(f)), f
     ref.close()
     alt.close()
+
+
+@pytest.mark.parametrize("name", ["doubly_periodic_fplane", "zonal_periodic_walls_beta", "static_seeded_ragged"])
+@pytest.mark.parametrize("per_launch", [2, 3])
+def test_blocked_barotropic_substeps_match_one_per_launch(name, per_launch):
+    """Several barotropic substeps per launch (the blocked kernel, periodic
+    halos as images) give the values of one launch per substep."""
+    from hackathon_mom6_miniapp_amd.model import Model
+
+    print(f"[gpu bt blocks] case={name} per_launch={per_launch}", flush=True)
+    cfg, init, _ = CASES[name]
+    case = make_case(cfg, init=init, seed=21)
+    one = Model(cfg, case)
+    blk = Model(cfg, case)
+    one.set_option("bt_substeps", 1)
+    blk.set_option("bt_substeps", per_launch)
+    one.step(2)
+    blk.step(2)
+    one.synchronize()
+    blk.synchronize()
+    for f in ("eta", "ubt", "vbt", "ubt_av", "vbt_av", "eta_av", "u", "v", "h"):
+        assert np.array_equal(one.get(f), blk.get(f)), f
+    one.close()
+    blk.close()
