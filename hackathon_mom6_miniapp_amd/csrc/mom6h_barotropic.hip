@@ -479,8 +479,9 @@ __global__ void __launch_bounds__(BTM_NT) k_bt_multi(BT p, BTUniform m, BTMulti 
     bt_multi_body<BTM_NT, BTM_TX, BTM_TY, TS>(p, m, st, tile % gridDim.x, tile / gridDim.x);
 }
 
-// ---- the ensemble form: every member's substep n in one launch.  Member
-// state ping-pongs between A (after the copy-in: odd substeps read it) and B.
+// ---- the ensemble form: every member's substep n (or substeps n0 ..
+// n0+ns-1, blocked) in one launch.  Member state ping-pongs between A (where
+// the copy-in puts it) and B; from_a says which set a launch reads.
 __global__ void k_bt_copy_in_batch(const BTMember *__restrict__ args, BatchMap mp)
 {
     int m, local;
@@ -496,29 +497,48 @@ __global__ void k_bt_copy_in_batch(const BTMember *__restrict__ args, BatchMap m
 }
 
 template <int TY>
-__global__ void __launch_bounds__(256) k_bt_substep_batch(const BTMember *__restrict__ args, BatchMap mp, int n)
+__global__ void __launch_bounds__(256) k_bt_substep_batch(const BTMember *__restrict__ args, BatchMap mp, int n,
+                                                          int from_a)
 {
     int m, local;
     batch_locate(mp, xcd_tile(blockIdx.x, gridDim.x), m, local);
     const BTMember a = args[m];
-    const bool odd = (n % 2) == 1;
     BTStep st;
-    st.eta_o = odd ? a.A[0] : a.B[0]; st.ubt_o = odd ? a.A[1] : a.B[1]; st.vbt_o = odd ? a.A[2] : a.B[2];
-    st.eta_n = odd ? a.B[0] : a.A[0]; st.ubt_n = odd ? a.B[1] : a.A[1]; st.vbt_n = odd ? a.B[2] : a.A[2];
+    st.eta_o = from_a ? a.A[0] : a.B[0]; st.ubt_o = from_a ? a.A[1] : a.B[1]; st.vbt_o = from_a ? a.A[2] : a.B[2];
+    st.eta_n = from_a ? a.B[0] : a.A[0]; st.ubt_n = from_a ? a.B[1] : a.A[1]; st.vbt_n = from_a ? a.B[2] : a.A[2];
     st.out_u = a.out[0]; st.out_v = a.out[1]; st.out_eta = a.out[2];
     st.n = n;
     bt_body<BTUniform, TY>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
 }
 
-// btstep of every member (all with nstep substeps and uniform coefficients)
-int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s, int ty)
+template <int TS>
+__global__ void __launch_bounds__(BTM_NT) k_bt_multi_batch(const BTMember *__restrict__ args, BatchMap mp, int n0,
+                                                            int ns, int from_a)
+{
+    int m, local;
+    batch_locate(mp, xcd_tile(blockIdx.x, gridDim.x), m, local);
+    const BTMember a = args[m];
+    BTMulti st;
+    st.eta_o = from_a ? a.A[0] : a.B[0]; st.ubt_o = from_a ? a.A[1] : a.B[1]; st.vbt_o = from_a ? a.A[2] : a.B[2];
+    st.eta_n = from_a ? a.B[0] : a.A[0]; st.ubt_n = from_a ? a.B[1] : a.A[1]; st.vbt_n = from_a ? a.B[2] : a.A[2];
+    st.n0 = n0;
+    st.ns = ns;
+    bt_multi_body<BTM_NT, BTM_TX, BTM_TY, TS>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
+}
+
+// btstep of every member (all with nstep substeps and uniform coefficients);
+// substeps 2 .. nstep-1 in blocked launches of up to per_launch when every
+// member allows it
+int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s, int ty, int per_launch)
 {
     MOM6H_REQUIRE(ty == 4 || ty == 8 || ty == 16, "btstep batch: tile rows must be 4, 8 or 16");
+    MOM6H_REQUIRE(per_launch >= 1 && per_launch <= 4, "btstep batch: 1..4 substeps per launch");
     const int BT_TY = ty;
     MOM6H_REQUIRE(n >= 1 && n <= kMaxMembers, "btstep batch: 1..%d members", kMaxMembers);
     const int nstep = h_args[0].p.nstep;
-    BatchMap mc{}, mt{};
-    mc.n = mt.n = n;
+    BatchMap mc{}, mt{}, mm{};
+    mc.n = mt.n = mm.n = n;
+    bool blocked = per_launch > 1;
     for (int m = 0; m < n; ++m) {
         const Dims &d = h_args[m].p.d;
         MOM6H_REQUIRE(h_args[m].p.nstep == nstep, "btstep batch: members differ in bt_nsteps");
@@ -527,14 +547,30 @@ int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStrea
         mt.nbx[m] = (d.n1 + BT_TX - 1) / BT_TX;
         mt.nby[m] = (d.n2 + BT_TY - 1) / BT_TY;
         mt.first[m + 1] = mt.first[m] + mt.nbx[m] * mt.nby[m];
+        mm.nbx[m] = (d.n1 + BTM_TX - 1) / BTM_TX;
+        mm.nby[m] = (d.n2 + BTM_TY - 1) / BTM_TY;
+        mm.first[m + 1] = mm.first[m] + mm.nbx[m] * mm.nby[m];
+        blocked = blocked && h_args[m].blocked_ok;
     }
     k_bt_copy_in_batch<<<(unsigned)mc.first[n], 256, 0, s>>>(d_args, mc);
     MOM6H_LAUNCH_CHECK();
-    for (int q = 1; q <= nstep; ++q) {
-        if (ty == 4) k_bt_substep_batch<4><<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q);
-        else if (ty == 16) k_bt_substep_batch<16><<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q);
-        else k_bt_substep_batch<8><<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q);
-        MOM6H_LAUNCH_CHECK();
+    int from_a = 1;
+    for (int q = 1; q <= nstep;) {
+        if (blocked && q >= 2 && q <= nstep - 1) {
+            const int ns = std::min(per_launch, nstep - q);
+            if (per_launch == 2) k_bt_multi_batch<2><<<(unsigned)mm.first[n], BTM_NT, 0, s>>>(d_args, mm, q, ns, from_a);
+            else if (per_launch == 3) k_bt_multi_batch<3><<<(unsigned)mm.first[n], BTM_NT, 0, s>>>(d_args, mm, q, ns, from_a);
+            else k_bt_multi_batch<4><<<(unsigned)mm.first[n], BTM_NT, 0, s>>>(d_args, mm, q, ns, from_a);
+            MOM6H_LAUNCH_CHECK();
+            q += ns;
+        } else {
+            if (ty == 4) k_bt_substep_batch<4><<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q, from_a);
+            else if (ty == 16) k_bt_substep_batch<16><<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q, from_a);
+            else k_bt_substep_batch<8><<<(unsigned)mt.first[n], 256, 0, s>>>(d_args, mt, q, from_a);
+            MOM6H_LAUNCH_CHECK();
+            ++q;
+        }
+        from_a = !from_a;
     }
     return MOM6H_OK;
 }
@@ -553,6 +589,7 @@ int bt_member(const mom6h_barotropic_cs *cs, const double *eta_in, const double 
     a.p.ubt_av = cs->ubt_av; a.p.vbt_av = cs->vbt_av;
     a.m = BTUniform{cs->u_Datu, cs->u_gtot, cs->u_rem_u, cs->u_rem_v, cs->u_IareaT, cs->u_IdxCu, cs->u_IdyCv,
                     cs->Datv, cs->f4};
+    a.blocked_ok = cs->blocked_ok;
     a.A[0] = cs->eta; a.A[1] = cs->ubt; a.A[2] = cs->vbt;
     a.B[0] = cs->eta_b; a.B[1] = cs->ubt_b; a.B[2] = cs->vbt_b;
     a.in[0] = eta_in; a.in[1] = ubt_in; a.in[2] = vbt_in;

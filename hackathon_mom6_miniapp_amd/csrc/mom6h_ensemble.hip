@@ -85,6 +85,7 @@ struct mom6h_ensemble {
     hipGraphExec_t gexec = nullptr;
     int momentum_tile = 8;
     int bt_tile = 4;         // rows of the barotropic substep tile (4, 8 or 16)
+    int bt_substeps = 3;     // barotropic substeps per blocked launch (1 = one launch per substep)
     long long max_lds = 0;
     StepTables<float> tf;
     StepTables<double> td;
@@ -196,8 +197,8 @@ int enqueue_3d(mom6h_ensemble *e, StepTables<T> &t)
 int enqueue_bt(mom6h_ensemble *e)
 {
     const int n = (int)e->members.size();
-    RC(btstep_batch(e->bt_pred.d, e->bt_pred.h.data(), n, e->s1, e->bt_tile));
-    RC(btstep_batch(e->bt_corr.d, e->bt_corr.h.data(), n, e->s1, e->bt_tile));
+    RC(btstep_batch(e->bt_pred.d, e->bt_pred.h.data(), n, e->s1, e->bt_tile, e->bt_substeps));
+    RC(btstep_batch(e->bt_corr.d, e->bt_corr.h.data(), n, e->s1, e->bt_tile, e->bt_substeps));
     return halo(e->h2d, e->s1);
 }
 
@@ -377,6 +378,11 @@ int mom6h_ensemble_set_option(mom6h_ensemble *e, const char *key, double value)
     } else if (k == "bt_tile") {
         MOM6H_REQUIRE(value == 4.0 || value == 8.0 || value == 16.0, "mom6h_ensemble_set_option: bt_tile must be 4, 8 or 16");
         e->bt_tile = (int)value;
+        drop_graph(e);
+    } else if (k == "bt_substeps") {
+        MOM6H_REQUIRE(value >= 1.0 && value <= 4.0 && value == (double)(int)value,
+                      "mom6h_ensemble_set_option: bt_substeps must be 1, 2, 3 or 4");
+        e->bt_substeps = (int)value;
         drop_graph(e);
     } else if (k == "momentum_splits") {
         MOM6H_REQUIRE(e->prepared, "mom6h_ensemble_set_option: momentum_splits after prepare");

@@ -224,11 +224,12 @@ struct BTStep {
 struct BTMember {
     BT p;
     BTUniform m;
+    int blocked_ok;             // the blocked kernel may run its interior substeps
     double *A[3], *B[3];        // eta, ubt, vbt: A after the copy-in, B the other set
     const double *in[3];        // eta_in, ubt_in, vbt_in
     double *out[3];             // u_av, v_av, eta_av
 };
-int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s, int ty);
+int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s, int ty, int per_launch);
 
 template <typename T>
 int vert_visc_batch(const VVArgs<T> *d_args, const VVArgs<T> *h_args, int n, long long max_lds, hipStream_t s);

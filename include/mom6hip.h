@@ -299,7 +299,8 @@ int mom6h_ensemble_step(mom6h_ensemble *e, int niter);
 int mom6h_ensemble_synchronize(mom6h_ensemble *e);
 /* Options: "graph" (1 = capture one step, replay it per step), "momentum_tile"
  * (rows of the fused momentum tile: 4, 8 or 16; default 8), "momentum_splits"
- * (blocks per tile sharing its layers; after prepare; default 1), "bt_tile" (4, 8 or 16). */
+ * (blocks per tile sharing its layers; after prepare; default 1), "bt_tile" (4, 8 or 16),
+ * "bt_substeps" (barotropic substeps per blocked launch, 1-4; default 3). */
 int mom6h_ensemble_set_option(mom6h_ensemble *e, const char *key, double value);
 /* Times the fused momentum launch of the ensemble (reps launches) for every
  * launch shape (tile rows x layer splits), keeps the fastest and writes the

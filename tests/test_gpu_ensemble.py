@@ -27,10 +27,12 @@ def _members(precision="fp64", meridional=False, biharmonic=False):
     return cfgs, cases
 
 
-def _check_against_single(cfgs, cases, steps, fields, graph=False):
+def _check_against_single(cfgs, cases, steps, fields, graph=False, options=None):
     from hackathon_mom6_miniapp_amd.model import Ensemble, Model
 
     ens = Ensemble(cfgs, cases, graph=graph)
+    for k, v in (options or {}).items():
+        ens.set_option(k, v)
     ens.step(steps)
     ens.synchronize()
     bad = []
@@ -51,6 +53,16 @@ def test_ensemble_matches_single_runs(precision):
     print(f"[gpu ensemble] precision={precision}", flush=True)
     cfgs, cases = _members(precision)
     _check_against_single(cfgs, cases, 3, FIELDS)
+
+
+@pytest.mark.parametrize("per_launch", [1, 2, 4])
+def test_ensemble_barotropic_substeps_per_launch(per_launch):
+    """The ensemble's barotropic chain with 1, 2 or 4 substeps per launch (the
+    single models run the default) gives the same members."""
+    print(f"[gpu ensemble] bt_substeps={per_launch}", flush=True)
+    cfgs, cases = _members("fp64")
+    _check_against_single(cfgs, cases, 2, ("eta", "ubt", "vbt", "ubt_av", "vbt_av", "eta_av", "u", "h"),
+                          options={"bt_substeps": per_launch})
 
 
 def test_ensemble_meridional_biharmonic_mixed():
