@@ -500,14 +500,15 @@ struct ColRegs {
 // LDS of one block: U = u of every level (levels past nk repeat level nk; u
 // lives here rather than in registers), S0/S1 = per-level terms of a pass,
 // R = results of the column sums (one row per quantity).  The column walks
-// read the 16 rows ending at row nk-1 of a column, i.e. up to 15 rows before
-// row 0: `pad` keeps those reads of U inside the block's LDS (S0 and S1 fall
-// back on the array before them); the values are never used.
-constexpr int kWalkBatch = 16;
+// read the B rows ending at row nk-1 of a column (B = walk_batch<T>), i.e. up
+// to B-1 rows before row 0: `pad` keeps those reads of U inside the block's
+// LDS (S0 and S1 fall back on the array before them); the values are never used.
+template <typename T>
+constexpr int walk_batch = sizeof(T) == 8 ? 8 : 16;   // rows read ahead of a column walk
 template <typename T, int CW, int K>
 struct ContShared {
     static constexpr int L = CW * K;
-    T pad[kWalkBatch - 1][CF];
+    T pad[walk_batch<T> - 1][CF];
     T U[L][CF], S0[L][CF], S1[L][CF], R[5][CF];
 };
 
@@ -519,7 +520,7 @@ struct ContShared {
 template <int L, typename T, typename Step>
 __device__ __forceinline__ void lds_walk(const T (*col)[CF], int nk, int lane, Step step)
 {
-    constexpr int B = kWalkBatch;
+    constexpr int B = walk_batch<T>;
     int k0 = 0;
     for (; k0 + B <= nk; k0 += B) {
         T x[B];
@@ -534,22 +535,35 @@ __device__ __forceinline__ void lds_walk(const T (*col)[CF], int nk, int lane, S
     const T *last = &col[0][lane] + (nk - B) * CF;   // row nk-B (>= -(B-1): see ContShared::pad)
 #pragma unroll
     for (int b = 0; b < B; ++b) x[b] = last[b * CF];
-    switch (skip) {
-    case 1: step(x[1]); [[fallthrough]];
-    case 2: step(x[2]); [[fallthrough]];
-    case 3: step(x[3]); [[fallthrough]];
-    case 4: step(x[4]); [[fallthrough]];
-    case 5: step(x[5]); [[fallthrough]];
-    case 6: step(x[6]); [[fallthrough]];
-    case 7: step(x[7]); [[fallthrough]];
-    case 8: step(x[8]); [[fallthrough]];
-    case 9: step(x[9]); [[fallthrough]];
-    case 10: step(x[10]); [[fallthrough]];
-    case 11: step(x[11]); [[fallthrough]];
-    case 12: step(x[12]); [[fallthrough]];
-    case 13: step(x[13]); [[fallthrough]];
-    case 14: step(x[14]); [[fallthrough]];
-    default: step(x[15]);
+    if constexpr (B == 16) {
+        switch (skip) {
+        case 1: step(x[1]); [[fallthrough]];
+        case 2: step(x[2]); [[fallthrough]];
+        case 3: step(x[3]); [[fallthrough]];
+        case 4: step(x[4]); [[fallthrough]];
+        case 5: step(x[5]); [[fallthrough]];
+        case 6: step(x[6]); [[fallthrough]];
+        case 7: step(x[7]); [[fallthrough]];
+        case 8: step(x[8]); [[fallthrough]];
+        case 9: step(x[9]); [[fallthrough]];
+        case 10: step(x[10]); [[fallthrough]];
+        case 11: step(x[11]); [[fallthrough]];
+        case 12: step(x[12]); [[fallthrough]];
+        case 13: step(x[13]); [[fallthrough]];
+        case 14: step(x[14]); [[fallthrough]];
+        default: step(x[15]);
+        }
+    } else {
+        static_assert(B == 8, "walk batch of 8 or 16 rows");
+        switch (skip) {
+        case 1: step(x[1]); [[fallthrough]];
+        case 2: step(x[2]); [[fallthrough]];
+        case 3: step(x[3]); [[fallthrough]];
+        case 4: step(x[4]); [[fallthrough]];
+        case 5: step(x[5]); [[fallthrough]];
+        case 6: step(x[6]); [[fallthrough]];
+        default: step(x[7]);
+        }
     }
 }
 
@@ -928,7 +942,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 }
 
 template <typename T, bool Y, int CW, int K>
-__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, (CW >= 16 || sizeof(T) == 4) ? 4 : 2)))
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : (CW + 3) / 4, (CW >= 12 || sizeof(T) == 4) ? 4 : 2)))
 k_continuity_blk(ContArgsT<T> a, int monotonic)
 {
     continuity_body<T, Y, CW, K>(a, monotonic, blockIdx.x, blockIdx.y);
@@ -1055,6 +1069,9 @@ int launch_blk(const ContArgsT<T> &a, int mono, dim3 g, hipStream_t s)
     if (nk <= 8) k_continuity_blk<T, Y, 8, 1><<<g, b, sizeof(ContShared<T, 8, 1>), s>>>(a, mono);
     else if (nk <= 16) k_continuity_blk<T, Y, 8, 2><<<g, b, sizeof(ContShared<T, 8, 2>), s>>>(a, mono);
     else if (nk <= 32) k_continuity_blk<T, Y, 8, 4><<<g, b, sizeof(ContShared<T, 8, 4>), s>>>(a, mono);
+    // FP64 beyond 32 levels: 12 waves of 7 levels (three waves per SIMD; the
+    // FP32 form keeps 8 x 10, two blocks per CU)
+    else if (sizeof(T) == 8) k_continuity_blk<T, Y, 12, 7><<<g, dim3(64 * 12), sizeof(ContShared<T, 12, 7>), s>>>(a, mono);
     else k_continuity_blk<T, Y, 8, 10><<<g, b, sizeof(ContShared<T, 8, 10>), s>>>(a, mono);
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
@@ -1075,6 +1092,12 @@ int set_blk_lds(long long max_lds)
                         (const void *)k_continuity_batch<T, false, 8, 4>, (const void *)k_continuity_batch<T, false, 8, 10>,
                         (const void *)k_continuity_batch<T, true, 8, 1>, (const void *)k_continuity_batch<T, true, 8, 2>,
                         (const void *)k_continuity_batch<T, true, 8, 4>, (const void *)k_continuity_batch<T, true, 8, 10>};
+    if ((int)sizeof(ContShared<T, 12, 7>) <= max_lds) {
+        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, false, 12, 7>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
+        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, true, 12, 7>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
+    }
     for (int q = 0; q < 8; ++q)
         if (bytes[q % 4] <= max_lds) {
             MOM6H_CHECK(hipFuncSetAttribute(kern[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q % 4]));
@@ -1092,8 +1115,11 @@ void launch_halo_copy(const Dims &d, const T *hin, T *h, hipStream_t s)
 
 bool blk_fits(const mom6h_continuity_cs *cs)
 {
-    return cs->d.nk <= 80 && !cs->upwind_1st && !cs->simple_2nd &&
-           (long long)sizeof(ContShared<double, 8, 10>) <= cs->max_lds;
+    // the largest FP64 blocks: 12 x 7 levels (one grid), 8 x 10 (ensembles)
+    constexpr long long bytes = sizeof(ContShared<double, 12, 7>) > sizeof(ContShared<double, 8, 10>)
+                                    ? sizeof(ContShared<double, 12, 7>)
+                                    : sizeof(ContShared<double, 8, 10>);
+    return cs->d.nk <= 80 && !cs->upwind_1st && !cs->simple_2nd && bytes <= cs->max_lds;
 }
 
 // The zonal pass of the block kernel for either type
