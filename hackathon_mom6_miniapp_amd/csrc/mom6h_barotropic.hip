@@ -468,15 +468,34 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
     }
 }
 
-// Tile of the blocked kernel: 66 x 16 covers a 720-wide grid and its halos in
-// 11 x 46 tiles, one round of two blocks per CU
-constexpr int BTM_TX = 66, BTM_TY = 16, BTM_NT = 512;
+// Tile and block of the blocked kernel for TS substeps per launch.  66 x 16
+// (512 threads) covers a 720-wide grid and its halos in 11 x 46 tiles, one
+// round of two blocks per CU; 4 substeps take 66 x 32 tiles of 1024 threads
+// (one block per CU, 11 x 23 tiles), which recompute less of their halo.
+template <int TS>
+struct BTMShape {
+    static constexpr int TX = 66, TY = 16, NT = 512;
+};
+template <>
+struct BTMShape<4> {
+    static constexpr int TX = 66, TY = 32, NT = 1024;
+};
+template <>
+struct BTMShape<5> : BTMShape<4> {};
+
+inline dim3 bt_multi_grid(const Dims &d, int ts)
+{
+    const int tx = ts >= 4 ? BTMShape<4>::TX : BTMShape<3>::TX, ty = ts >= 4 ? BTMShape<4>::TY : BTMShape<3>::TY;
+    return dim3((d.n1 + tx - 1) / tx, (d.n2 + ty - 1) / ty);
+}
+inline int bt_multi_threads(int ts) { return ts >= 4 ? BTMShape<4>::NT : BTMShape<3>::NT; }
 
 template <int TS>
-__global__ void __launch_bounds__(BTM_NT) k_bt_multi(BT p, BTUniform m, BTMulti st)
+__global__ void __launch_bounds__(BTMShape<TS>::NT) k_bt_multi(BT p, BTUniform m, BTMulti st)
 {
+    using S = BTMShape<TS>;
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    bt_multi_body<BTM_NT, BTM_TX, BTM_TY, TS>(p, m, st, tile % gridDim.x, tile / gridDim.x);
+    bt_multi_body<S::NT, S::TX, S::TY, TS>(p, m, st, tile % gridDim.x, tile / gridDim.x);
 }
 
 // ---- the ensemble form: every member's substep n (or substeps n0 ..
@@ -512,7 +531,7 @@ __global__ void __launch_bounds__(256) k_bt_substep_batch(const BTMember *__rest
 }
 
 template <int TS>
-__global__ void __launch_bounds__(BTM_NT) k_bt_multi_batch(const BTMember *__restrict__ args, BatchMap mp, int n0,
+__global__ void __launch_bounds__(BTMShape<TS>::NT) k_bt_multi_batch(const BTMember *__restrict__ args, BatchMap mp, int n0,
                                                             int ns, int from_a)
 {
     int m, local;
@@ -523,7 +542,8 @@ __global__ void __launch_bounds__(BTM_NT) k_bt_multi_batch(const BTMember *__res
     st.eta_n = from_a ? a.B[0] : a.A[0]; st.ubt_n = from_a ? a.B[1] : a.A[1]; st.vbt_n = from_a ? a.B[2] : a.A[2];
     st.n0 = n0;
     st.ns = ns;
-    bt_multi_body<BTM_NT, BTM_TX, BTM_TY, TS>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
+    using S = BTMShape<TS>;
+    bt_multi_body<S::NT, S::TX, S::TY, TS>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
 }
 
 // btstep of every member (all with nstep substeps and uniform coefficients);
@@ -532,7 +552,7 @@ __global__ void __launch_bounds__(BTM_NT) k_bt_multi_batch(const BTMember *__res
 int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStream_t s, int ty, int per_launch)
 {
     MOM6H_REQUIRE(ty == 4 || ty == 8 || ty == 16, "btstep batch: tile rows must be 4, 8 or 16");
-    MOM6H_REQUIRE(per_launch >= 1 && per_launch <= 4, "btstep batch: 1..4 substeps per launch");
+    MOM6H_REQUIRE(per_launch >= 1 && per_launch <= 5, "btstep batch: 1..5 substeps per launch");
     const int BT_TY = ty;
     MOM6H_REQUIRE(n >= 1 && n <= kMaxMembers, "btstep batch: 1..%d members", kMaxMembers);
     const int nstep = h_args[0].p.nstep;
@@ -547,8 +567,9 @@ int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStrea
         mt.nbx[m] = (d.n1 + BT_TX - 1) / BT_TX;
         mt.nby[m] = (d.n2 + BT_TY - 1) / BT_TY;
         mt.first[m + 1] = mt.first[m] + mt.nbx[m] * mt.nby[m];
-        mm.nbx[m] = (d.n1 + BTM_TX - 1) / BTM_TX;
-        mm.nby[m] = (d.n2 + BTM_TY - 1) / BTM_TY;
+        const dim3 gm = bt_multi_grid(d, per_launch);
+        mm.nbx[m] = (int)gm.x;
+        mm.nby[m] = (int)gm.y;
         mm.first[m + 1] = mm.first[m] + mm.nbx[m] * mm.nby[m];
         blocked = blocked && h_args[m].blocked_ok;
     }
@@ -558,9 +579,11 @@ int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStrea
     for (int q = 1; q <= nstep;) {
         if (blocked && q >= 2 && q <= nstep - 1) {
             const int ns = std::min(per_launch, nstep - q);
-            if (per_launch == 2) k_bt_multi_batch<2><<<(unsigned)mm.first[n], BTM_NT, 0, s>>>(d_args, mm, q, ns, from_a);
-            else if (per_launch == 3) k_bt_multi_batch<3><<<(unsigned)mm.first[n], BTM_NT, 0, s>>>(d_args, mm, q, ns, from_a);
-            else k_bt_multi_batch<4><<<(unsigned)mm.first[n], BTM_NT, 0, s>>>(d_args, mm, q, ns, from_a);
+            const unsigned nb = (unsigned)mm.first[n], nt = (unsigned)bt_multi_threads(per_launch);
+            if (per_launch == 2) k_bt_multi_batch<2><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+            else if (per_launch == 3) k_bt_multi_batch<3><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+            else if (per_launch == 4) k_bt_multi_batch<4><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+            else k_bt_multi_batch<5><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
             MOM6H_LAUNCH_CHECK();
             q += ns;
         } else {
@@ -625,7 +648,7 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
                   "mom6h_barotropic_init: NULL array");
     auto *cs = new mom6h_barotropic_cs();
     cs->tile_rows = 4;
-    cs->substeps = 3;
+    cs->substeps = 4;
     cs->d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, 1);
     cs->halo = Halo{periodic_x, periodic_y};
     cs->nstep = nstep;
@@ -710,7 +733,7 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
     }
     const int BT_TY = cs->tile_rows;
     dim3 grid((d.n1 + BT_TX - 1) / BT_TX, (d.n2 + BT_TY - 1) / BT_TY), block(256);
-    const dim3 grid_m((d.n1 + BTM_TX - 1) / BTM_TX, (d.n2 + BTM_TY - 1) / BTM_TY);
+    const dim3 grid_m = bt_multi_grid(d, cs->substeps);
     const int per_launch = cs->substeps;
     double *A[3] = {cs->eta, cs->ubt, cs->vbt}, *B[3] = {cs->eta_b, cs->ubt_b, cs->vbt_b};
     for (int n = 1; n <= cs->nstep;) {
@@ -723,10 +746,11 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
             st.ns = std::min(per_launch, cs->nstep - n);
             const BTUniform m{cs->u_Datu, cs->u_gtot, cs->u_rem_u, cs->u_rem_v, cs->u_IareaT, cs->u_IdxCu,
                               cs->u_IdyCv, cs->Datv, cs->f4};
-            const dim3 block_m(BTM_NT);
+            const dim3 block_m(bt_multi_threads(per_launch));
             if (per_launch == 2) k_bt_multi<2><<<grid_m, block_m, 0, strm>>>(p, m, st);
             else if (per_launch == 3) k_bt_multi<3><<<grid_m, block_m, 0, strm>>>(p, m, st);
-            else k_bt_multi<4><<<grid_m, block_m, 0, strm>>>(p, m, st);
+            else if (per_launch == 4) k_bt_multi<4><<<grid_m, block_m, 0, strm>>>(p, m, st);
+            else k_bt_multi<5><<<grid_m, block_m, 0, strm>>>(p, m, st);
             MOM6H_LAUNCH_CHECK();
             for (int q = 0; q < 3; ++q) std::swap(A[q], B[q]);
             n += st.ns;

@@ -85,7 +85,7 @@ struct mom6h_ensemble {
     hipGraphExec_t gexec = nullptr;
     int momentum_tile = 8;
     int bt_tile = 4;         // rows of the barotropic substep tile (4, 8 or 16)
-    int bt_substeps = 3;     // barotropic substeps per blocked launch (1 = one launch per substep)
+    int bt_substeps = 4;     // barotropic substeps per blocked launch (1 = one launch per substep)
     long long max_lds = 0;
     StepTables<float> tf;
     StepTables<double> td;
@@ -380,8 +380,8 @@ int mom6h_ensemble_set_option(mom6h_ensemble *e, const char *key, double value)
         e->bt_tile = (int)value;
         drop_graph(e);
     } else if (k == "bt_substeps") {
-        MOM6H_REQUIRE(value >= 1.0 && value <= 4.0 && value == (double)(int)value,
-                      "mom6h_ensemble_set_option: bt_substeps must be 1, 2, 3 or 4");
+        MOM6H_REQUIRE(value >= 1.0 && value <= 5.0 && value == (double)(int)value,
+                      "mom6h_ensemble_set_option: bt_substeps must be 1..5");
         e->bt_substeps = (int)value;
         drop_graph(e);
     } else if (k == "momentum_splits") {

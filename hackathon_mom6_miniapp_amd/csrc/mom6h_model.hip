@@ -681,8 +681,8 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
     }
     else if (k == "bt_substeps") {
         MOM6H_REQUIRE(m->bt_cs, "mom6h_model_set_option: bt_substeps after prepare");
-        MOM6H_REQUIRE(value >= 1.0 && value <= 4.0 && value == (double)(int)value,
-                      "mom6h_model_set_option: bt_substeps must be 1, 2, 3 or 4");
+        MOM6H_REQUIRE(value >= 1.0 && value <= 5.0 && value == (double)(int)value,
+                      "mom6h_model_set_option: bt_substeps must be 1..5");
         m->bt_cs->substeps = (int)value;
         if (m->gexec) {
             (void)hipGraphExecDestroy(m->gexec);

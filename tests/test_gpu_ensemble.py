@@ -55,7 +55,7 @@ def test_ensemble_matches_single_runs(precision):
     _check_against_single(cfgs, cases, 3, FIELDS)
 
 
-@pytest.mark.parametrize("per_launch", [1, 2, 4])
+@pytest.mark.parametrize("per_launch", [1, 2, 3, 5])
 def test_ensemble_barotropic_substeps_per_launch(per_launch):
     """The ensemble's barotropic chain with 1, 2 or 4 substeps per launch (the
     single models run the default) gives the same members."""

@@ -157,7 +157,7 @@ def test_launch_shapes_give_identical_results(shape):
 
 
 @pytest.mark.parametrize("name", ["doubly_periodic_fplane", "zonal_periodic_walls_beta", "static_seeded_ragged"])
-@pytest.mark.parametrize("per_launch", [2, 3])
+@pytest.mark.parametrize("per_launch", [2, 3, 4, 5])
 def test_blocked_barotropic_substeps_match_one_per_launch(name, per_launch):
     """Several barotropic substeps per launch (the blocked kernel, periodic
     halos as images) give the values of one launch per substep."""
