@@ -504,7 +504,7 @@ struct ColRegs {
 // to B-1 rows before row 0: `pad` keeps those reads of U inside the block's
 // LDS (S0 and S1 fall back on the array before them); the values are never used.
 template <typename T>
-constexpr int walk_batch = sizeof(T) == 8 ? 8 : 16;   // rows read ahead of a column walk
+constexpr int walk_batch = 8;   // rows read ahead of a column walk
 template <typename T, int CW, int K>
 struct ContShared {
     static constexpr int L = CW * K;
@@ -942,7 +942,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 }
 
 template <typename T, bool Y, int CW, int K>
-__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : (CW + 3) / 4, (CW >= 12 || sizeof(T) == 4) ? 4 : 2)))
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, (CW >= 16 || sizeof(T) == 4) ? 4 : 2)))
 k_continuity_blk(ContArgsT<T> a, int monotonic)
 {
     continuity_body<T, Y, CW, K>(a, monotonic, blockIdx.x, blockIdx.y);
@@ -1069,9 +1069,6 @@ int launch_blk(const ContArgsT<T> &a, int mono, dim3 g, hipStream_t s)
     if (nk <= 8) k_continuity_blk<T, Y, 8, 1><<<g, b, sizeof(ContShared<T, 8, 1>), s>>>(a, mono);
     else if (nk <= 16) k_continuity_blk<T, Y, 8, 2><<<g, b, sizeof(ContShared<T, 8, 2>), s>>>(a, mono);
     else if (nk <= 32) k_continuity_blk<T, Y, 8, 4><<<g, b, sizeof(ContShared<T, 8, 4>), s>>>(a, mono);
-    // FP64 beyond 32 levels: 12 waves of 7 levels (three waves per SIMD; the
-    // FP32 form keeps 8 x 10, two blocks per CU)
-    else if (sizeof(T) == 8) k_continuity_blk<T, Y, 12, 7><<<g, dim3(64 * 12), sizeof(ContShared<T, 12, 7>), s>>>(a, mono);
     else k_continuity_blk<T, Y, 8, 10><<<g, b, sizeof(ContShared<T, 8, 10>), s>>>(a, mono);
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
@@ -1092,12 +1089,6 @@ int set_blk_lds(long long max_lds)
                         (const void *)k_continuity_batch<T, false, 8, 4>, (const void *)k_continuity_batch<T, false, 8, 10>,
                         (const void *)k_continuity_batch<T, true, 8, 1>, (const void *)k_continuity_batch<T, true, 8, 2>,
                         (const void *)k_continuity_batch<T, true, 8, 4>, (const void *)k_continuity_batch<T, true, 8, 10>};
-    if ((int)sizeof(ContShared<T, 12, 7>) <= max_lds) {
-        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, false, 12, 7>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
-        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, true, 12, 7>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
-    }
     for (int q = 0; q < 8; ++q)
         if (bytes[q % 4] <= max_lds) {
             MOM6H_CHECK(hipFuncSetAttribute(kern[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q % 4]));
@@ -1115,11 +1106,8 @@ void launch_halo_copy(const Dims &d, const T *hin, T *h, hipStream_t s)
 
 bool blk_fits(const mom6h_continuity_cs *cs)
 {
-    // the largest FP64 blocks: 12 x 7 levels (one grid), 8 x 10 (ensembles)
-    constexpr long long bytes = sizeof(ContShared<double, 12, 7>) > sizeof(ContShared<double, 8, 10>)
-                                    ? sizeof(ContShared<double, 12, 7>)
-                                    : sizeof(ContShared<double, 8, 10>);
-    return cs->d.nk <= 80 && !cs->upwind_1st && !cs->simple_2nd && bytes <= cs->max_lds;
+    return cs->d.nk <= 80 && !cs->upwind_1st && !cs->simple_2nd &&
+           (long long)sizeof(ContShared<double, 8, 10>) <= cs->max_lds;
 }
 
 // The zonal pass of the block kernel for either type
