@@ -710,17 +710,18 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     }
 
     auto uk = [&](int kk) __attribute__((always_inline)) -> T { return S.U[k0 + kk][lane]; };
-    T fk[K];   // layer transports of the last evaluation
-    // one pass: evaluate every level at u + du; terms (fk, dy*hm) to LDS
+    // one pass: evaluate every level at u + du; terms (layer transport, hm) to
+    // LDS.  S0 keeps each lane's transports of its last evaluation, the
+    // result the face outputs and the convergence read.
     auto eval_pass = [&](T du, bool on) __attribute__((always_inline)) {
         if (!on) return;
         col.fence();
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            T hm;
-            col.eval(kk, uk(kk) + du, fk[kk], hm);
+            T f, hm;
+            col.eval(kk, uk(kk) + du, f, hm);
             if (valid(kk)) {
-                S.S0[k0 + kk][lane] = fk[kk];
+                S.S0[k0 + kk][lane] = f;
                 S.S1[k0 + kk][lane] = hm;   // scaled by dy in the column sum
             }
         }
@@ -833,7 +834,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     };
     T du_eval, sum_dyhm;
     const T du = flux_adjust(bt, true, &du_eval, &sum_dyhm);
-    // here fk holds the layer fluxes of the last evaluation and, when
+    // here S0 holds the layer fluxes of the last evaluation and, when
     // du_eval == du, sum_dyhm the sum of dy*hm at u + du
     const bool last_at_du = (du_eval == du);
 
@@ -845,7 +846,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         for (int kk = 0; kk < K; ++kk) {
             if (valid(kk)) {
                 const long long o = cu + (long long)(k0 + kk) * s;
-                a.uh[o] = fk[kk];
+                a.uh[o] = S.S0[k0 + kk][lane];
                 if (a.u_cor) a.u_cor[o] = uk(kk) + du;
             }
         }
@@ -857,9 +858,9 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         const bool writes_cell = live && lane > 0 && lane < CF - 1 && i >= d.isc;
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            const T fw = __shfl_up(fk[kk], 1, 64);   // face I-1, same level
             if (writes_cell && valid(kk)) {
-                const T v = col.w0[kk] - dtIareaT * (fk[kk] - fw);
+                const T fw = S.S0[k0 + kk][lane - 1];   // face I-1 (the lane to the west), same level
+                const T v = col.w0[kk] - dtIareaT * (S.S0[k0 + kk][lane] - fw);
                 a.h[IX3(d, i, j, k0 + kk + 1)] = vmax(v, T(0.0));
             }
         }
