@@ -499,7 +499,8 @@ struct ColRegs {
 
 // LDS of one block: U = u of every level (levels past nk repeat level nk; u
 // lives here rather than in registers), S0/S1 = per-level terms of a pass,
-// R = results of the column sums (one row per quantity).  The column walks
+// R = results of the column sums (one row per quantity; rows 5..8 keep the
+// Newton iteration's starting point for a second iteration).  The column walks
 // read the B rows ending at row nk-1 of a column (B = walk_batch<T>), i.e. up
 // to B-1 rows before row 0: `pad` keeps those reads of U inside the block's
 // LDS (S0 and S1 fall back on the array before them); the values are never used.
@@ -509,7 +510,7 @@ template <typename T, int CW, int K>
 struct ContShared {
     static constexpr int L = CW * K;
     T pad[walk_batch<T> - 1][CF];
-    T U[L][CF], S0[L][CF], S1[L][CF], R[5][CF];
+    T U[L][CF], S0[L][CF], S1[L][CF], R[9][CF];
 };
 
 // Visits rows 0..nk-1 (nk <= L) of one LDS column of ContShared in k order: step(x_k).
@@ -759,8 +760,19 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         S.R[4][lane] = lds_fold<L>(S.S0, nk, lane, T(0.0), [](T x, T f) { return x + vabs(f); });
     }
     __syncthreads();
-    const T uh_tot_0 = S.R[0][lane], duhdu_tot_0 = S.R[1][lane];
-    const T du_max_CFL = vmax(S.R[2][lane], T(0.0)), du_min_CFL = vmin(S.R[3][lane], T(0.0));
+    // the Newton iteration's starting point: the first pass's sums and the CFL
+    // bracket; LDS rows 5..8 keep it for the zero-target iteration of BT_cont
+    // (so that no register holds it across the first iteration)
+    struct Start {
+        T uh_tot_0, duhdu_tot_0, du_max_CFL, du_min_CFL;
+    };
+    const Start start{S.R[0][lane], S.R[1][lane], vmax(S.R[2][lane], T(0.0)), vmin(S.R[3][lane], T(0.0))};
+    if (w == 0) {
+        S.R[5][lane] = start.uh_tot_0;
+        S.R[6][lane] = start.duhdu_tot_0;
+        S.R[7][lane] = start.du_max_CFL;
+        S.R[8][lane] = start.du_min_CFL;
+    }
     const T dtIarea = (T)(a.dt * dmin(a.IareaT[cell(2)], a.IareaT[cell(3)]));
     const T bt = a.uhbt ? (T)a.uhbt[c2] : T(0.0);
     const T tol_floor = F32 ? dtIarea * (T(4.0f * 1.1920929e-7f) * S.R[4][lane]) : T(0.0);
@@ -770,10 +782,11 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // evaluation are the result; evaluated also at itt = max_itts).  Returns
     // du; *du_eval = du of the last evaluation that was summed (NaN if none)
     // and *dyhm_eval = sum_k dy*hm at u + *du_eval (the last duhdu_tot).
-    auto flux_adjust = [&](T uhbt, bool keep, T *du_eval, T *dyhm_eval) __attribute__((always_inline)) -> T {
+    auto flux_adjust = [&](const Start &st, T uhbt, bool keep, T *du_eval, T *dyhm_eval)
+                           __attribute__((always_inline)) -> T {
         const int max_itts = 20;
-        T du = T(0.0), du_max = du_max_CFL, du_min = du_min_CFL;
-        T uh_err = uh_tot_0 - uhbt, duhdu_tot = duhdu_tot_0;
+        T du = T(0.0), du_max = st.du_max_CFL, du_min = st.du_min_CFL;
+        T uh_err = st.uh_tot_0 - uhbt, duhdu_tot = st.duhdu_tot_0;
         T uh_err_best = vabs(uh_err);
         // x: lanes past the last face and the block's last lane (no cell 1 of
         // its own) take no Newton passes
@@ -833,7 +846,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         return du;
     };
     T du_eval, sum_dyhm;
-    const T du = flux_adjust(bt, true, &du_eval, &sum_dyhm);
+    const T du = flux_adjust(start, bt, true, &du_eval, &sum_dyhm);
     // here S0 holds the layer fluxes of the last evaluation and, when
     // du_eval == du, sum_dyhm the sum of dy*hm at u + du
     const bool last_at_du = (du_eval == du);
@@ -886,7 +899,8 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     T du0 = du, du0_eval = du_eval, FAmt_0 = sum_dyhm;
     if (__syncthreads_or(bt != T(0.0))) {
         T de, s0;
-        const T d2 = flux_adjust(T(0.0), false, &de, &s0);
+        const Start again{S.R[5][lane], S.R[6][lane], S.R[7][lane], S.R[8][lane]};
+        const T d2 = flux_adjust(again, T(0.0), false, &de, &s0);
         if (bt != T(0.0)) { du0 = d2; du0_eval = de; FAmt_0 = s0; }
     }
     // FAmt_0 = sum dy*hm(u + du0): the sum of the iteration's last evaluation
@@ -943,7 +957,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 }
 
 template <typename T, bool Y, int CW, int K>
-__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, (CW >= 16 || sizeof(T) == 4) ? 4 : 2)))
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : (CW + 3) / 4, (CW >= 12 || sizeof(T) == 4) ? 4 : 2)))
 k_continuity_blk(ContArgsT<T> a, int monotonic)
 {
     continuity_body<T, Y, CW, K>(a, monotonic, blockIdx.x, blockIdx.y);
@@ -1070,6 +1084,9 @@ int launch_blk(const ContArgsT<T> &a, int mono, dim3 g, hipStream_t s)
     if (nk <= 8) k_continuity_blk<T, Y, 8, 1><<<g, b, sizeof(ContShared<T, 8, 1>), s>>>(a, mono);
     else if (nk <= 16) k_continuity_blk<T, Y, 8, 2><<<g, b, sizeof(ContShared<T, 8, 2>), s>>>(a, mono);
     else if (nk <= 32) k_continuity_blk<T, Y, 8, 4><<<g, b, sizeof(ContShared<T, 8, 4>), s>>>(a, mono);
+    // FP64 beyond 32 levels: 12 waves of 7 levels (three waves per SIMD; the
+    // FP32 form keeps 8 x 10, two blocks per CU)
+    else if (sizeof(T) == 8) k_continuity_blk<T, Y, 12, 7><<<g, dim3(64 * 12), sizeof(ContShared<T, 12, 7>), s>>>(a, mono);
     else k_continuity_blk<T, Y, 8, 10><<<g, b, sizeof(ContShared<T, 8, 10>), s>>>(a, mono);
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
@@ -1090,6 +1107,12 @@ int set_blk_lds(long long max_lds)
                         (const void *)k_continuity_batch<T, false, 8, 4>, (const void *)k_continuity_batch<T, false, 8, 10>,
                         (const void *)k_continuity_batch<T, true, 8, 1>, (const void *)k_continuity_batch<T, true, 8, 2>,
                         (const void *)k_continuity_batch<T, true, 8, 4>, (const void *)k_continuity_batch<T, true, 8, 10>};
+    if ((int)sizeof(ContShared<T, 12, 7>) <= max_lds) {
+        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, false, 12, 7>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
+        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, true, 12, 7>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
+    }
     for (int q = 0; q < 8; ++q)
         if (bytes[q % 4] <= max_lds) {
             MOM6H_CHECK(hipFuncSetAttribute(kern[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q % 4]));
@@ -1107,8 +1130,11 @@ void launch_halo_copy(const Dims &d, const T *hin, T *h, hipStream_t s)
 
 bool blk_fits(const mom6h_continuity_cs *cs)
 {
-    return cs->d.nk <= 80 && !cs->upwind_1st && !cs->simple_2nd &&
-           (long long)sizeof(ContShared<double, 8, 10>) <= cs->max_lds;
+    // the largest FP64 blocks: 12 x 7 levels (one grid), 8 x 10 (ensembles)
+    constexpr long long bytes = sizeof(ContShared<double, 12, 7>) > sizeof(ContShared<double, 8, 10>)
+                                    ? sizeof(ContShared<double, 12, 7>)
+                                    : sizeof(ContShared<double, 8, 10>);
+    return cs->d.nk <= 80 && !cs->upwind_1st && !cs->simple_2nd && bytes <= cs->max_lds;
 }
 
 // The zonal pass of the block kernel for either type
