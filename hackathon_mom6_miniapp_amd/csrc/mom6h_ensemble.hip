@@ -257,13 +257,13 @@ int time_momentum(mom6h_ensemble *e, StepTables<T> &t, int ty, int reps, float *
 
 extern "C" {
 
-int mom6h_ensemble_create(mom6h_ensemble **out, const mom6h_config *configs, int n)
+int mom6h_ensemble_create(mom6h_ensemble **out, const mom6h_config *member_cfg, int n)
 {
-    MOM6H_REQUIRE(out && configs, "mom6h_ensemble_create: NULL argument");
+    MOM6H_REQUIRE(out && member_cfg, "mom6h_ensemble_create: NULL argument");
     MOM6H_REQUIRE(n >= 1 && n <= 21, "mom6h_ensemble_create: 1..21 members (3 halo jobs each in one launch)");
-    const mom6h_config &c0 = configs[0];
+    const mom6h_config &c0 = member_cfg[0];
     for (int q = 1; q < n; ++q) {
-        const mom6h_config &c = configs[q];
+        const mom6h_config &c = member_cfg[q];
         MOM6H_REQUIRE(c.bt_nsteps == c0.bt_nsteps && c.precision == c0.precision && c.meridional == c0.meridional &&
                           c.biharmonic == c0.biharmonic,
                       "mom6h_ensemble_create: members must share bt_nsteps, precision, meridional and biharmonic");
@@ -274,7 +274,7 @@ int mom6h_ensemble_create(mom6h_ensemble **out, const mom6h_config *configs, int
     int rc = 0;
     for (int q = 0; q < n && !rc; ++q) {
         mom6h_model *m = nullptr;
-        rc = mom6h_model_create(&m, &configs[q]);
+        rc = mom6h_model_create(&m, &member_cfg[q]);
         if (!rc) e->members.push_back(m);
     }
     if (!rc && hipStreamCreateWithFlags(&e->s0, hipStreamNonBlocking) != hipSuccess)

@@ -292,7 +292,7 @@ int mom6h_model_destroy(mom6h_model *m);
  * equal those of the same model stepped alone.  No counterpart in the
  * reference (its drivers run one grid). */
 typedef struct mom6h_ensemble mom6h_ensemble;
-int mom6h_ensemble_create(mom6h_ensemble **e, const mom6h_config *configs, int n_members);
+int mom6h_ensemble_create(mom6h_ensemble **e, const mom6h_config *member_cfg, int n_members);
 int mom6h_ensemble_member(mom6h_ensemble *e, int index, mom6h_model **m);
 int mom6h_ensemble_prepare(mom6h_ensemble *e);
 int mom6h_ensemble_step(mom6h_ensemble *e, int niter);

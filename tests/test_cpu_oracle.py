@@ -136,8 +136,8 @@ def test_volume_conserved_to_round_off(bc):
     assert o.verify()[2] < 1e-13
 
 
-def test_btstep_closed_box_conserves_eta_volume():
-    """The serial btstep's narrow transport ranges close the box: sum(eta*area) is
+def test_btstep_closed_domain_conserves_eta_volume():
+    """The serial btstep's narrow transport ranges close the domain: sum(eta*area) is
     conserved to round-off (mom6_barotropic_serial.F90:263-279)."""
     cfg = Config(ni=24, nj=20, nk=3, bt_nsteps=25)
     case = make_case(cfg, init="reference")
