@@ -89,6 +89,8 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
     __shared__ T s_u[MH][MW], s_v[MH][MW], s_h[MH][MW];
     __shared__ T s_q[MH][MW], s_KE[MH][MW];
     __shared__ T s_d2u[BIH ? MH : 1][BIH ? MW : 1], s_d2v[BIH ? MH : 1][BIH ? MW : 1];
+    // the 2D masks and Coriolis of the staged region, the same for every layer
+    __shared__ T s_mT[MH][MW], s_mBu[MH][MW], s_cor[MH][MW];
     const Dims &d = a.d;
     const UMetT<T> g(a.g);
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
@@ -127,6 +129,17 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             }
         }
     };
+    for (int t = tid; t < MW * MH; t += blockDim.x) {
+        const int li = t % MW, lj = t / MW, i = bi + li, j = bj + lj;
+        T mt = T(0.0), mb = T(0.0), cor = T(0.0);
+        if (i <= d.ied && j <= d.jed) {
+            const long long c2 = IX2(d, i, j);
+            mt = (T)a.mask2dT[c2];
+            mb = (T)a.mask2dBu[c2];
+            cor = (T)a.CoriolisBu[c2];
+        }
+        s_mT[lj][li] = mt; s_mBu[lj][li] = mb; s_cor[lj][li] = cor;
+    }
     load_layer(k_first);
     for (int k = k_first; k <= k_last; ++k) {
 #pragma unroll
@@ -148,15 +161,14 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
         for (int t = tid; t < MW * MH; t += blockDim.x) {
             const int li = t % MW, lj = t / MW, i = bi + li, j = bj + lj;
             if (li < MW - 1 && lj < MH - 1 && i <= ie && j <= je) {
-                const long long c2 = IX2(d, i, j);
                 const T dvdx = (s_v[lj][li + 1] * g.dyCv) - (s_v[lj][li] * g.dyCv);
                 const T dudy = (s_u[lj + 1][li] * g.dxCu) - (s_u[lj][li] * g.dxCu);
                 const T hAu0 = (j < js) ? T(0.0) : T(0.5) * ((g.areaT * s_h[lj][li]) + (g.areaT * s_h[lj][li + 1]));
                 const T hAu1 = T(0.5) * ((g.areaT * s_h[lj + 1][li]) + (g.areaT * s_h[lj + 1][li + 1]));
                 const T hAv0 = (i < is) ? T(0.0) : T(0.5) * ((g.areaT * s_h[lj][li]) + (g.areaT * s_h[lj + 1][li]));
                 const T hAv1 = T(0.5) * ((g.areaT * s_h[lj][li + 1]) + (g.areaT * s_h[lj + 1][li + 1]));
-                const T rel_vort = (T)a.mask2dBu[c2] * (dvdx - dudy) * g.IareaBu;
-                const T abs_vort = (T)a.CoriolisBu[c2] + rel_vort;
+                const T rel_vort = s_mBu[lj][li] * (dvdx - dudy) * g.IareaBu;
+                const T abs_vort = s_cor[lj][li] + rel_vort;
                 const T hArea_q = (hAu0 + hAu1) + (hAv0 + hAv1);
                 s_q[lj][li] = abs_vort * (g.Area_q / (hArea_q + T(1.0e-20)));
             }
@@ -186,10 +198,10 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
                 }
                 return str * (s_h[pj][pi] * T(1.0));
             };
-            auto str_xy = [&](int pi, int pj, int I, int J) -> T {   // q point
+            auto str_xy = [&](int pi, int pj) -> T {   // q point at LDS (pi, pj)
                 const T dvdx = g.DY_dxBu * ((s_v[pj][pi + 1] * g.IdyCv) - (s_v[pj][pi] * g.IdyCv));
                 const T dudy = g.DX_dyBu * ((s_u[pj + 1][pi] * g.IdxCu) - (s_u[pj][pi] * g.IdxCu));
-                const T m = (T)a.mask2dBu[IX2(d, I, J)];
+                const T m = s_mBu[pj][pi];
                 const T sh = (a.no_slip ? (T(2.0) - m) : m) * (dvdx + dudy);
                 const T hq = T(0.25) * ((s_h[pj][pi] + s_h[pj + 1][pi + 1]) + (s_h[pj][pi + 1] + s_h[pj + 1][pi]));
                 T str = -(a.Kh * sh);
@@ -202,8 +214,8 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             };
             if (i <= ie - 1) {
                 // diffu (:1059-1067)
-                const T h_u = T(0.5) * ((T)a.mask2dT[IX2(d, i, j)] * s_h[lj][li] + (T)a.mask2dT[IX2(d, i + 1, j)] * s_h[lj][li + 1]);
-                const T diffu = ((g.IdxCu * ((g.dx2q * str_xy(li, lj - 1, i, j - 1)) - (g.dx2q * str_xy(li, lj, i, j))) +
+                const T h_u = T(0.5) * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj][li + 1] * s_h[lj][li + 1]);
+                const T diffu = ((g.IdxCu * ((g.dx2q * str_xy(li, lj - 1)) - (g.dx2q * str_xy(li, lj))) +
                                        g.IdyCu * ((g.dy2h * str_xx(li, lj)) - (g.dy2h * str_xx(li + 1, lj)))) *
                                       g.IareaCu) / (h_u + a.h_neglect);
                 // CAu, Sadourny (:208-216); vh from compute_transports
@@ -215,8 +227,8 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             }
             if (j <= je - 1) {
                 // diffv (:1070-1078)
-                const T h_v = T(0.5) * ((T)a.mask2dT[IX2(d, i, j)] * s_h[lj][li] + (T)a.mask2dT[IX2(d, i, j + 1)] * s_h[lj + 1][li]);
-                const T diffv = ((g.IdyCv * ((g.dy2q * str_xy(li - 1, lj, i - 1, j)) - (g.dy2q * str_xy(li, lj, i, j))) -
+                const T h_v = T(0.5) * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj + 1][li] * s_h[lj + 1][li]);
+                const T diffv = ((g.IdyCv * ((g.dy2q * str_xy(li - 1, lj)) - (g.dy2q * str_xy(li, lj))) -
                                        g.IdxCv * ((g.dx2h * str_xx(li, lj)) - (g.dx2h * str_xx(li, lj + 1)))) *
                                       g.IareaCv) / (h_v + a.h_neglect);
                 // CAv, Sadourny (:218-226); uh from compute_transports
