@@ -234,18 +234,25 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
     // C: the arithmetic type of the coefficients (z_i, bottom taper, hvel, the
     // coupling a); the tridiagonal solve is FP64 whatever C is
     using C = T;
+    // division of the coefficients: exact in FP64; in the FP32 form a product
+    // with the hardware reciprocal (about 1 ulp, the rounding scale of the
+    // FP32 coefficients themselves; the tridiagonal solve stays FP64)
+    auto cdiv = [](C num, C den) __attribute__((always_inline)) -> C {
+        if constexpr (sizeof(C) == 4) return num * __builtin_amdgcn_rcpf(den);
+        else return num / den;
+    };
     const C hn = (C)h_neglect;
     const C Kv = (C)p.Kv, Kv_ml = (C)p.Kv_ml, Kv_extra_bbl = (C)p.Kv_extra_bbl, Hmix = (C)p.Hmix;
     const double dt = p.dt;
     const C I_Hbbl = C(1.0) / ((C)p.Hbbl + hn);
-    auto botfn = [&](C z2) -> C { return C(1.0) / (C(1.0) + C(0.09) * z2 * z2 * z2 * z2 * z2 * z2); };
+    auto botfn = [&](C z2) -> C { return cdiv(C(1.0), C(1.0) + C(0.09) * z2 * z2 * z2 * z2 * z2 * z2); };
     auto ZC = [&](int k) -> C { return (C)Z(k); };
     Z(nk + 1) = 0.0;
 
     // ---- sweep A, bottom-up (:160-179): z_i(k) ----
     C zi = C(0.0);
     auto A_level = [&](int k, C ha, C hb) {
-        const C h_harm = C(2.0) * ha * hb / (ha + hb + hn);
+        const C h_harm = cdiv(C(2.0) * ha * hb, ha + hb + hn);
         zi = zi + h_harm * I_Hbbl;
         Z(k) = (double)zi;
     };
@@ -273,7 +280,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
 
     // hvel (:165-176) given botfn(z_i(k+1)); the upwind blend is selected, not branched
     auto hvel = [&](C ha, C hb, C vk, C bf) -> C {
-        const C h_harm = C(2.0) * ha * hb / (ha + hb + hn);
+        const C h_harm = cdiv(C(2.0) * ha * hb, ha + hb + hn);
         const C h_arith = C(0.5) * (hb + ha);
         const C h_delta = hb - ha;
         const C blend = (C(1.0) - bf) * h_harm + bf * h_arith;
@@ -285,7 +292,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
         const C topfn = C(1.0) - z_top / Hmix;
         C Kv_tot = (z_top < Hmix) ? Kv + (Kv_ml - Kv) * topfn : Kv;
         Kv_tot = Kv_tot + Kv_extra_bbl * bfK;
-        return Kv_tot / (C(0.5) * (hv_0 + hv_m1 + hn));
+        return cdiv(Kv_tot, C(0.5) * (hv_0 + hv_m1 + hn));
     };
 
     // ---- sweep B, top-down (:430-444) ----
