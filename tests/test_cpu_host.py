@@ -101,3 +101,18 @@ def test_bt_cfl_matches_closed_form(dt, nsteps, status):
     assert cfl2.value == pytest.approx(cfl.value * math.sqrt(2.0), rel=1e-15)
     assert nmin.value == math.ceil(dt * c / 1.0e4 * 2.5) == 15
     assert st.value == status
+
+
+def test_traffic_summary_holds_every_roofline_kernel():
+    """bench.py takes each roofline kernel's HBM bytes per launch from the
+    committed counter summary; every one of them is there and plausible
+    (between the kernel's algorithmic bytes / 2 and 4x them)."""
+    import bench
+
+    n2 = 723 * 723   # the 720x720 grid with its halos
+    for stage, kernel in bench.ROOFLINE_KERNELS.items():
+        t = bench.measured_traffic(kernel)
+        assert t is not None, kernel
+        a3, a2 = bench.STAGE_ARRAYS[stage]
+        alg = 8.0 * (a3 * n2 * 75 + a2 * n2)
+        assert 0.5 * alg < t < 4.0 * alg, (kernel, t, alg)
