@@ -79,6 +79,16 @@ __global__ void __launch_bounds__(256) k_del2_batch(const MomArgsT<T> *__restric
     del2_body(a, local % mp.nbx[m], r % mp.nby[m], r / mp.nby[m]);
 }
 
+// Division in the stencil's type: exact in FP64 (the reference's arithmetic);
+// in the FP32 form a product with the hardware reciprocal (about 1 ulp, the
+// rounding scale of the FP32 stencil itself)
+template <typename T>
+__device__ __forceinline__ T sdiv(T num, T den)
+{
+    if constexpr (sizeof(T) == 4) return num * __builtin_amdgcn_rcpf(den);
+    else return num / den;
+}
+
 // The tile (tile_x, tile_y) of one grid.  BIH: the biharmonic stress of the
 // Del2 fields staged like u and v, added to the Laplacian stress
 // (:824-886, :979-1036).
@@ -170,12 +180,12 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
                 const T rel_vort = s_mBu[lj][li] * (dvdx - dudy) * g.IareaBu;
                 const T abs_vort = s_cor[lj][li] + rel_vort;
                 const T hArea_q = (hAu0 + hAu1) + (hAv0 + hAv1);
-                s_q[lj][li] = abs_vort * (g.Area_q / (hArea_q + T(1.0e-20)));
+                s_q[lj][li] = abs_vort * sdiv(g.Area_q, hArea_q + T(1.0e-20));
             }
             if (li >= 1 && lj >= 1 && i <= ie && j <= je) {
                 const T uc = s_u[lj][li], uw = s_u[lj][li - 1], vc = s_v[lj][li], vs = s_v[lj - 1][li];
-                s_KE[lj][li] = T(0.25) * ((g.dyCu * (uc * uc) + g.dyCu * (uw * uw)) +
-                                       (g.dxCv * (vc * vc) + g.dxCv * (vs * vs))) / g.areaT;
+                s_KE[lj][li] = sdiv(T(0.25) * ((g.dyCu * (uc * uc) + g.dyCu * (uw * uw)) +
+                                            (g.dxCv * (vc * vc) + g.dxCv * (vs * vs))), g.areaT);
             }
         }
         __syncthreads();
@@ -215,9 +225,9 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             if (i <= ie - 1) {
                 // diffu (:1059-1067)
                 const T h_u = T(0.5) * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj][li + 1] * s_h[lj][li + 1]);
-                const T diffu = ((g.IdxCu * ((g.dx2q * str_xy(li, lj - 1)) - (g.dx2q * str_xy(li, lj))) +
-                                       g.IdyCu * ((g.dy2h * str_xx(li, lj)) - (g.dy2h * str_xx(li + 1, lj)))) *
-                                      g.IareaCu) / (h_u + a.h_neglect);
+                const T diffu = sdiv((g.IdxCu * ((g.dx2q * str_xy(li, lj - 1)) - (g.dx2q * str_xy(li, lj))) +
+                                      g.IdyCu * ((g.dy2h * str_xx(li, lj)) - (g.dy2h * str_xx(li + 1, lj)))) *
+                                         g.IareaCu, h_u + a.h_neglect);
                 // CAu, Sadourny (:208-216); vh from compute_transports
                 auto vh = [&](int pi, int pj) { return s_v[pj][pi] * T(0.5) * (s_h[pj][pi] + s_h[pj + 1][pi]) * g.dxCv; };
                 const T KEx = (s_KE[lj][li + 1] - s_KE[lj][li]) * g.IdxCu;
@@ -228,9 +238,9 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             if (j <= je - 1) {
                 // diffv (:1070-1078)
                 const T h_v = T(0.5) * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj + 1][li] * s_h[lj + 1][li]);
-                const T diffv = ((g.IdyCv * ((g.dy2q * str_xy(li - 1, lj)) - (g.dy2q * str_xy(li, lj))) -
-                                       g.IdxCv * ((g.dx2h * str_xx(li, lj)) - (g.dx2h * str_xx(li, lj + 1)))) *
-                                      g.IareaCv) / (h_v + a.h_neglect);
+                const T diffv = sdiv((g.IdyCv * ((g.dy2q * str_xy(li - 1, lj)) - (g.dy2q * str_xy(li, lj))) -
+                                      g.IdxCv * ((g.dx2h * str_xx(li, lj)) - (g.dx2h * str_xx(li, lj + 1)))) *
+                                         g.IareaCv, h_v + a.h_neglect);
                 // CAv, Sadourny (:218-226); uh from compute_transports
                 auto uh = [&](int pi, int pj) { return s_u[pj][pi] * T(0.5) * (s_h[pj][pi] + s_h[pj][pi + 1]) * g.dyCu; };
                 const T KEy = (s_KE[lj + 1][li] - s_KE[lj][li]) * g.IdyCv;
