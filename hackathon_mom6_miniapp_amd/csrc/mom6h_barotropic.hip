@@ -290,7 +290,11 @@ __device__ __forceinline__ void rect_each(int x0, int x1, int y0, int y1, int ti
     }
 }
 
-template <int NT, int TX, int TY, int TS>
+// ROW: Datv and f4 constant along each row of the compute domain (walls close
+// whole rows, f depends on latitude only; the host checks it): one staged value
+// per row, and the predicted eta goes to an array of its own (no barrier
+// between reading the old values and the predictor).
+template <int NT, int TX, int TY, int TS, bool ROW>
 __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, const BTMulti &st, int tile_x,
                                               int tile_y)
 {
@@ -299,8 +303,13 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
     constexpr int NR = ((TX + 3 * (TS - 1)) * (TY + 3 * (TS - 1)) + NT - 1) / NT;          // valid region
     constexpr int NP = ((TX + 3 * (TS - 1) + 2) * (TY + 3 * (TS - 1) + 2) + NT - 1) / NT;  // update regions
     static_assert(RW * RH <= 4096, "staged region too large for the multiply-shift index");
-    __shared__ double s_e[RH][RW], s_u[RH][RW], s_v[RH][RW], s_dv[RH][RW], s_f4[RH][RW];
+    __shared__ double s_e[RH][RW], s_u[RH][RW], s_v[RH][RW];
+    __shared__ double s_dv[RH][ROW ? 1 : RW], s_f4[RH][ROW ? 1 : RW];
+    __shared__ double s_ep[ROW ? RH : 1][ROW ? RW : 1];   // ROW: the predicted eta
     __shared__ int s_cx[RW], s_cy[RH];
+    auto DV = [&](int lj, int li) -> double { return s_dv[lj][ROW ? 0 : li]; };
+    auto F4 = [&](int lj, int li) -> double { return s_f4[lj][ROW ? 0 : li]; };
+    auto EP = [&](int lj, int li) -> double { return ROW ? s_ep[ROW ? lj : 0][ROW ? li : 0] : s_e[lj][li]; };
     const Dims &d = p.d;
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
     const int i0 = d.isd + tile_x * TX, j0 = d.jsd + tile_y * TY;
@@ -324,10 +333,19 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
         if (i >= d.isd && i <= d.ied && j >= d.jsd && j <= d.jed) {
             const long long c = IX2(d, i, j);
             e = st.eta_o[c]; uu = st.ubt_o[c]; vv = st.vbt_o[c];
-            dv = m.Datv[c]; f4 = m.f4[c];
+            if (!ROW) { dv = m.Datv[c]; f4 = m.f4[c]; }
         }
-        s_e[lj][li] = e; s_u[lj][li] = uu; s_v[lj][li] = vv; s_dv[lj][li] = dv; s_f4[lj][li] = f4;
+        s_e[lj][li] = e; s_u[lj][li] = uu; s_v[lj][li] = vv;
+        if (!ROW) { s_dv[lj][ROW ? 0 : li] = dv; s_f4[lj][ROW ? 0 : li] = f4; }
     });
+    if (ROW) {
+        for (int t = tid; t < RH; t += NT) {
+            const int j = s_cy[t];
+            const bool in = j >= d.jsd && j <= d.jed;
+            s_dv[t][0] = in ? m.Datv_row[j - d.jsd] : 0.0;
+            s_f4[t][0] = in ? m.f4_row[j - d.jsd] : 0.0;
+        }
+    }
     // running averages of this thread's output points (the array positions)
     double ua_r[NOUT], va_r[NOUT];
 #pragma unroll
@@ -347,10 +365,10 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
     auto new_v = [&](int lj, int li, int i, int j) -> double {
         double vn = s_v[lj][li];
         if (i >= is && i <= ie && j >= js && j <= je - 1) {
-            const double f4 = s_f4[lj][li];
+            const double f4 = F4(lj, li);
             const double Cor = -1.0 * (((f4 * s_u[lj][li - 1]) + (f4 * s_u[lj + 1][li])) +
                                        ((f4 * s_u[lj][li]) + (f4 * s_u[lj + 1][li - 1])));
-            const double pf = ((s_e[lj][li] * m.gtot) - (s_e[lj + 1][li] * m.gtot)) * p.dgeo_de * m.IdyCv;
+            const double pf = ((EP(lj, li) * m.gtot) - (EP(lj + 1, li) * m.gtot)) * p.dgeo_de * m.IdyCv;
             vn = m.bt_rem_v * (vn + dtbt * (Cor + pf));
         }
         return vn;
@@ -358,10 +376,10 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
     auto new_u = [&](int lj, int li, int i, int j) -> double {
         double un = s_u[lj][li];
         if (i >= is && i <= ie - 1 && j >= js && j <= je) {
-            const double f4 = s_f4[lj][li];
+            const double f4 = F4(lj, li);
             const double Cor = (((f4 * s_v[lj][li + 1]) + (f4 * s_v[lj - 1][li])) +
                                 ((f4 * s_v[lj][li]) + (f4 * s_v[lj - 1][li + 1])));
-            const double pf = ((s_e[lj][li] * m.gtot) - (s_e[lj][li + 1] * m.gtot)) * p.dgeo_de * m.IdxCu;
+            const double pf = ((EP(lj, li) * m.gtot) - (EP(lj, li + 1) * m.gtot)) * p.dgeo_de * m.IdxCu;
             un = m.bt_rem_u * (un + dtbt * (Cor + pf));
         }
         return un;
@@ -397,14 +415,20 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
                 v_old[r] = s_v[lj][li]; vs_old[r] = s_v[lj - 1][li];
             }
         }
-        __syncthreads();
-        // eta predictor (:224-231), in place
+        if (!ROW) __syncthreads();   // the predictor below overwrites s_e in place
+        // eta predictor (:224-231): in place, or (ROW) into s_ep, which then
+        // holds eta itself where no prediction applies
         rect_each<NT, NP>(A0 - 1, A1 + 1, B0 - 1, B1 + 1, tid, [&](int li, int lj) {
             const int i = s_cx[li], j = s_cy[lj];
-            if (i < is || i > ie || j < js || j > je) return;
-            s_e[lj][li] = s_e[lj][li] + (dtbt * m.IareaT) *
-                (((m.Datu * s_u[lj][li - 1]) - (m.Datu * s_u[lj][li])) +
-                 ((s_dv[lj - 1][li] * s_v[lj - 1][li]) - (s_dv[lj][li] * s_v[lj][li])));
+            const bool in = !(i < is || i > ie || j < js || j > je);
+            if (!ROW && !in) return;
+            const double e0 = s_e[lj][li];
+            const double ep = in ? e0 + (dtbt * m.IareaT) *
+                                            (((m.Datu * s_u[lj][li - 1]) - (m.Datu * s_u[lj][li])) +
+                                             ((DV(lj - 1, li) * s_v[lj - 1][li]) - (DV(lj, li) * s_v[lj][li])))
+                                 : e0;
+            if (ROW) s_ep[ROW ? lj : 0][ROW ? li : 0] = ep;
+            else s_e[lj][li] = ep;
         });
         __syncthreads();
         // the first velocity update
@@ -448,8 +472,8 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
             const double un = s_u[lj][li], uw = s_u[lj][li - 1], vn = s_v[lj][li], vs = s_v[lj - 1][li];
             const double uh_e = (i >= iu0 && i <= iu1) ? m.Datu * (trans_wt1 * un + trans_wt2 * u_old[r]) : 0.0;
             const double uh_w = (i - 1 >= iu0 && i - 1 <= iu1) ? m.Datu * (trans_wt1 * uw + trans_wt2 * uw_old[r]) : 0.0;
-            const double vh_n = (j >= jv0 && j <= jv1) ? s_dv[lj][li] * (trans_wt1 * vn + trans_wt2 * v_old[r]) : 0.0;
-            const double vh_s = (j - 1 >= jv0 && j - 1 <= jv1) ? s_dv[lj - 1][li] * (trans_wt1 * vs + trans_wt2 * vs_old[r]) : 0.0;
+            const double vh_n = (j >= jv0 && j <= jv1) ? DV(lj, li) * (trans_wt1 * vn + trans_wt2 * v_old[r]) : 0.0;
+            const double vh_s = (j - 1 >= jv0 && j - 1 <= jv1) ? DV(lj - 1, li) * (trans_wt1 * vs + trans_wt2 * vs_old[r]) : 0.0;
             s_e[lj][li] = e_old[r] - dtbt * m.IareaT * ((uh_e - uh_w) + (vh_n - vh_s));
         }
         __syncthreads();
@@ -490,12 +514,22 @@ inline dim3 bt_multi_grid(const Dims &d, int ts)
 }
 inline int bt_multi_threads(int ts) { return ts >= 4 ? BTMShape<4>::NT : BTMShape<3>::NT; }
 
-template <int TS>
+template <int TS, bool ROW>
 __global__ void __launch_bounds__(BTMShape<TS>::NT) k_bt_multi(BT p, BTUniform m, BTMulti st)
 {
     using S = BTMShape<TS>;
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    bt_multi_body<S::NT, S::TX, S::TY, TS>(p, m, st, tile % gridDim.x, tile / gridDim.x);
+    bt_multi_body<S::NT, S::TX, S::TY, TS, ROW>(p, m, st, tile % gridDim.x, tile / gridDim.x);
+}
+
+template <bool ROW>
+void launch_bt_multi(int ts, dim3 g, const BT &p, const BTUniform &m, const BTMulti &st, hipStream_t s)
+{
+    const dim3 b(bt_multi_threads(ts));
+    if (ts == 2) k_bt_multi<2, ROW><<<g, b, 0, s>>>(p, m, st);
+    else if (ts == 3) k_bt_multi<3, ROW><<<g, b, 0, s>>>(p, m, st);
+    else if (ts == 4) k_bt_multi<4, ROW><<<g, b, 0, s>>>(p, m, st);
+    else k_bt_multi<5, ROW><<<g, b, 0, s>>>(p, m, st);
 }
 
 // ---- the ensemble form: every member's substep n (or substeps n0 ..
@@ -530,7 +564,7 @@ __global__ void __launch_bounds__(256) k_bt_substep_batch(const BTMember *__rest
     bt_body<BTUniform, TY>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
 }
 
-template <int TS>
+template <int TS, bool ROW>
 __global__ void __launch_bounds__(BTMShape<TS>::NT) k_bt_multi_batch(const BTMember *__restrict__ args, BatchMap mp, int n0,
                                                             int ns, int from_a)
 {
@@ -543,7 +577,7 @@ __global__ void __launch_bounds__(BTMShape<TS>::NT) k_bt_multi_batch(const BTMem
     st.n0 = n0;
     st.ns = ns;
     using S = BTMShape<TS>;
-    bt_multi_body<S::NT, S::TX, S::TY, TS>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
+    bt_multi_body<S::NT, S::TX, S::TY, TS, ROW>(a.p, a.m, st, local % mp.nbx[m], local / mp.nbx[m]);
 }
 
 // btstep of every member (all with nstep substeps and uniform coefficients);
@@ -558,7 +592,7 @@ int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStrea
     const int nstep = h_args[0].p.nstep;
     BatchMap mc{}, mt{}, mm{};
     mc.n = mt.n = mm.n = n;
-    bool blocked = per_launch > 1;
+    bool blocked = per_launch > 1, rowwise = true;
     for (int m = 0; m < n; ++m) {
         const Dims &d = h_args[m].p.d;
         MOM6H_REQUIRE(h_args[m].p.nstep == nstep, "btstep batch: members differ in bt_nsteps");
@@ -572,6 +606,7 @@ int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStrea
         mm.nby[m] = (int)gm.y;
         mm.first[m + 1] = mm.first[m] + mm.nbx[m] * mm.nby[m];
         blocked = blocked && h_args[m].blocked_ok;
+        rowwise = rowwise && h_args[m].m.Datv_row != nullptr;
     }
     k_bt_copy_in_batch<<<(unsigned)mc.first[n], 256, 0, s>>>(d_args, mc);
     MOM6H_LAUNCH_CHECK();
@@ -580,10 +615,17 @@ int btstep_batch(const BTMember *d_args, const BTMember *h_args, int n, hipStrea
         if (blocked && q >= 2 && q <= nstep - 1) {
             const int ns = std::min(per_launch, nstep - q);
             const unsigned nb = (unsigned)mm.first[n], nt = (unsigned)bt_multi_threads(per_launch);
-            if (per_launch == 2) k_bt_multi_batch<2><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
-            else if (per_launch == 3) k_bt_multi_batch<3><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
-            else if (per_launch == 4) k_bt_multi_batch<4><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
-            else k_bt_multi_batch<5><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+            if (rowwise) {
+                if (per_launch == 2) k_bt_multi_batch<2, true><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+                else if (per_launch == 3) k_bt_multi_batch<3, true><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+                else if (per_launch == 4) k_bt_multi_batch<4, true><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+                else k_bt_multi_batch<5, true><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+            } else {
+                if (per_launch == 2) k_bt_multi_batch<2, false><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+                else if (per_launch == 3) k_bt_multi_batch<3, false><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+                else if (per_launch == 4) k_bt_multi_batch<4, false><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+                else k_bt_multi_batch<5, false><<<nb, nt, 0, s>>>(d_args, mm, q, ns, from_a);
+            }
             MOM6H_LAUNCH_CHECK();
             q += ns;
         } else {
@@ -612,6 +654,8 @@ int bt_member(const mom6h_barotropic_cs *cs, const double *eta_in, const double 
     a.p.ubt_av = cs->ubt_av; a.p.vbt_av = cs->vbt_av;
     a.m = BTUniform{cs->u_Datu, cs->u_gtot, cs->u_rem_u, cs->u_rem_v, cs->u_IareaT, cs->u_IdxCu, cs->u_IdyCv,
                     cs->Datv, cs->f4};
+    a.m.Datv_row = cs->use_rows ? cs->Datv_row : nullptr;
+    a.m.f4_row = cs->use_rows ? cs->f4_row : nullptr;
     a.blocked_ok = cs->blocked_ok;
     a.A[0] = cs->eta; a.A[1] = cs->ubt; a.A[2] = cs->vbt;
     a.B[0] = cs->eta_b; a.B[1] = cs->ubt_b; a.B[2] = cs->vbt_b;
@@ -649,6 +693,7 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
     auto *cs = new mom6h_barotropic_cs();
     cs->tile_rows = 4;
     cs->substeps = 4;
+    cs->use_rows = 1;
     cs->d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, 1);
     cs->halo = Halo{periodic_x, periodic_y};
     cs->nstep = nstep;
@@ -696,10 +741,28 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
                                  Datv[(size_t)(j + (jec - jsc + 1) - jsd) * n1 + (i - isd)];
             }
             cs->blocked_ok = images ? 1 : 0;
+            // Datv and f4 constant along every row of the compute domain: the
+            // blocked kernel stages one value per row
+            {
+                const int n1 = ied - isd + 1, n2r = jed - jsd + 1;
+                bool rows = true;
+                std::vector<double> dvr(n2r), f4r(n2r);
+                for (int j = 0; j < n2r && rows; ++j) {
+                    const size_t r0 = (size_t)j * n1;
+                    dvr[j] = Datv[r0 + (isc - isd)];
+                    f4r[j] = f[r0 + (isc - isd)];
+                    for (int i = isc; i <= iec && rows; ++i)
+                        rows = Datv[r0 + (i - isd)] == dvr[j] && f[r0 + (i - isd)] == f4r[j];
+                }
+                if (rows) {
+                    rc = dev_upload(&cs->Datv_row, dvr.data(), (size_t)n2r);
+                    rc = rc ? rc : dev_upload(&cs->f4_row, f4r.data(), (size_t)n2r);
+                }
+            }
             cs->uniform = 1;
             cs->u_Datu = Datu[0]; cs->u_gtot = gtot_E[0]; cs->u_rem_u = bt_rem_u[0]; cs->u_rem_v = bt_rem_v[0];
             cs->u_IareaT = IareaT[0]; cs->u_IdxCu = IdxCu[0]; cs->u_IdyCv = IdyCv[0];
-            rc = dev_upload(&cs->f4, f.data(), n2);
+            rc = rc ? rc : dev_upload(&cs->f4, f.data(), n2);
         }
     }
     if (rc) {
@@ -744,13 +807,15 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
             st.eta_n = B[0]; st.ubt_n = B[1]; st.vbt_n = B[2];
             st.n0 = n;
             st.ns = std::min(per_launch, cs->nstep - n);
-            const BTUniform m{cs->u_Datu, cs->u_gtot, cs->u_rem_u, cs->u_rem_v, cs->u_IareaT, cs->u_IdxCu,
-                              cs->u_IdyCv, cs->Datv, cs->f4};
-            const dim3 block_m(bt_multi_threads(per_launch));
-            if (per_launch == 2) k_bt_multi<2><<<grid_m, block_m, 0, strm>>>(p, m, st);
-            else if (per_launch == 3) k_bt_multi<3><<<grid_m, block_m, 0, strm>>>(p, m, st);
-            else if (per_launch == 4) k_bt_multi<4><<<grid_m, block_m, 0, strm>>>(p, m, st);
-            else k_bt_multi<5><<<grid_m, block_m, 0, strm>>>(p, m, st);
+            BTUniform m{cs->u_Datu, cs->u_gtot, cs->u_rem_u, cs->u_rem_v, cs->u_IareaT, cs->u_IdxCu,
+                        cs->u_IdyCv, cs->Datv, cs->f4};
+            if (cs->Datv_row && cs->use_rows) {
+                m.Datv_row = cs->Datv_row;
+                m.f4_row = cs->f4_row;
+                launch_bt_multi<true>(per_launch, grid_m, p, m, st, strm);
+            } else {
+                launch_bt_multi<false>(per_launch, grid_m, p, m, st, strm);
+            }
             MOM6H_LAUNCH_CHECK();
             for (int q = 0; q < 3; ++q) std::swap(A[q], B[q]);
             n += st.ns;
@@ -786,6 +851,7 @@ int mom6h_barotropic_end(mom6h_barotropic_cs *cs)
     if (!cs) return MOM6H_OK;
     for (double *p : {cs->Datu, cs->Datv, cs->gtot_E, cs->gtot_W, cs->gtot_N, cs->gtot_S, cs->f_4_u,
                       cs->f_4_v, cs->bt_rem_u, cs->bt_rem_v, cs->IareaT, cs->IdxCu, cs->IdyCv, cs->f4,
+                      cs->Datv_row, cs->f4_row,
                       cs->eta, cs->ubt, cs->vbt, cs->eta_b, cs->ubt_b, cs->vbt_b, cs->ubt_av, cs->vbt_av})
         dev_free(p);
     delete cs;

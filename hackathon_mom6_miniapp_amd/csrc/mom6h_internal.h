@@ -196,6 +196,9 @@ struct BTArrays {
 struct BTUniform {
     double Datu, gtot, bt_rem_u, bt_rem_v, IareaT, IdxCu, IdyCv;
     const double *Datv, *f4;
+    // Datv and f4 per row (jsd..jed) when each is constant along the rows of
+    // the compute domain, else NULL
+    const double *Datv_row = nullptr, *f4_row = nullptr;
     __device__ double datu(long long) const { return Datu; }
     __device__ double datv(long long c) const { return Datv[c]; }
     __device__ double gE(long long) const { return gtot; }
@@ -304,6 +307,8 @@ struct mom6h_barotropic_cs {
     int tile_rows;           // rows of the substep tile (4, 8 or 16)
     int substeps;            // substeps per launch of the blocked kernel (1: one launch per substep)
     int blocked_ok;          // uniform coefficients whose periodic Datv halo rows equal their images
+    double *Datv_row, *f4_row;   // Datv and f4 per row when constant along the compute rows (else NULL)
+    int use_rows;                // stage Datv and f4 per row when the grid allows (1, the default) or as 2D fields
     // state (two sets, ping-pong across substeps) and time averages; zero at
     // init, as the reference's never-written cells
     double *eta, *ubt, *vbt, *eta_b, *ubt_b, *vbt_b, *ubt_av, *vbt_av;

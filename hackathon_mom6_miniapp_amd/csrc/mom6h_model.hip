@@ -679,6 +679,16 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
             m->graph = nullptr;
         }
     }
+    else if (k == "bt_rows") {
+        MOM6H_REQUIRE(m->bt_cs, "mom6h_model_set_option: bt_rows after prepare");
+        m->bt_cs->use_rows = value != 0.0;
+        if (m->gexec) {
+            (void)hipGraphExecDestroy(m->gexec);
+            (void)hipGraphDestroy(m->graph);
+            m->gexec = nullptr;
+            m->graph = nullptr;
+        }
+    }
     else if (k == "bt_substeps") {
         MOM6H_REQUIRE(m->bt_cs, "mom6h_model_set_option: bt_substeps after prepare");
         MOM6H_REQUIRE(value >= 1.0 && value <= 5.0 && value == (double)(int)value,

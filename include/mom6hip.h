@@ -263,7 +263,10 @@ int mom6h_model_synchronize(mom6h_model *m);
  * layers, 1..nk; default 1), "bt_tile" (rows of the barotropic substep tile,
  * 4, 8 or 16; after prepare; default 4), "bt_substeps" (barotropic substeps per
  * launch between the first and the last of a call: 1 = one launch per substep,
- * up to 5 = the blocked kernel; after prepare; default 4), "fused" (1 = fused momentum stencil and
+ * up to 5 = the blocked kernel; after prepare; default 4), "bt_rows" (1 = the
+ * blocked kernel stages Datv and the Coriolis factor per row when both are
+ * constant along the compute rows, 0 = always as 2D fields; after prepare;
+ * default 1), "fused" (1 = fused momentum stencil and
  * on-chip vert_visc coefficients where the grid allows, default 1; 0 = the
  * module-by-module path), "profile" (1 = record per-stage HIP events; the
  * graph is not used while profiling), "serialize" (1 = run the barotropic

@@ -157,19 +157,21 @@ def test_launch_shapes_give_identical_results(shape):
 
 
 @pytest.mark.parametrize("name", ["doubly_periodic_fplane", "zonal_periodic_walls_beta", "static_seeded_ragged"])
-@pytest.mark.parametrize("per_launch", [2, 3, 4, 5])
-def test_blocked_barotropic_substeps_match_one_per_launch(name, per_launch):
+@pytest.mark.parametrize("per_launch,rows", [(2, 1), (3, 0), (4, 1), (4, 0), (5, 1)])
+def test_blocked_barotropic_substeps_match_one_per_launch(name, per_launch, rows):
     """Several barotropic substeps per launch (the blocked kernel, periodic
-    halos as images) give the values of one launch per substep."""
+    halos as images; coefficients staged per row or as 2D fields) give the
+    values of one launch per substep."""
     from hackathon_mom6_miniapp_amd.model import Model
 
-    print(f"[gpu bt blocks] case={name} per_launch={per_launch}", flush=True)
+    print(f"[gpu bt blocks] case={name} per_launch={per_launch} rows={rows}", flush=True)
     cfg, init, _ = CASES[name]
     case = make_case(cfg, init=init, seed=21)
     one = Model(cfg, case)
     blk = Model(cfg, case)
     one.set_option("bt_substeps", 1)
     blk.set_option("bt_substeps", per_launch)
+    blk.set_option("bt_rows", rows)
     one.step(2)
     blk.step(2)
     one.synchronize()
