@@ -178,10 +178,16 @@ __global__ void k_vert_visc(Dims d, double *__restrict__ u, double *__restrict__
 // botfn, the mixed-layer taper, the coupling coefficient) is independent across
 // levels, so the compiler interleaves them.  Only the Thomas recurrence itself
 // (one division per level) is serial.  Levels that do not fill a chunk take a
-// one-level loop with the same arithmetic.
+// one-level loop with the same arithmetic (a partial chunk guarded level by
+// level measured slower: the guards split the body and stop the interleaving).
+// The LDS column holds the kernel at one wave per SIMD, so the registers of
+// the prefetch are free: the FP64 bodies use AGPRs beyond 256 VGPRs rather
+// than scratch.  At nk = 75 these chunk sizes leave a tail of 1-3 levels per
+// sweep; 8 / 16 measured 7-10% slower (an 11-level unprefetched tail in
+// sweeps A and C), 16 / 32 and larger slower again.
 
-constexpr int VV_C = 8;    // levels per chunk of sweep B (heavy bodies)
-constexpr int VV_A = 16;   // levels per chunk of sweeps A and C (light bodies, latency-bound loads)
+constexpr int VV_C = 12;   // levels per chunk of sweep B (heavy bodies)
+constexpr int VV_A = 24;   // levels per chunk of sweeps A and C (light bodies, latency-bound loads)
 
 // Block (bx, row) of one grid: 64 columns of one row; Zs = (nk+1)*64 doubles of LDS.
 // Rows are interleaved v, u, v, u, ... (v row J = js-1+r/2, u row j = js+(r-1)/2)
@@ -434,7 +440,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
 
 // one grid, tiles renumbered per XCD
 template <typename T, bool KEEP>
-__global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs<T> p)
+__global__ void __launch_bounds__(64, 1) k_vert_visc_chunked(VVArgs<T> p)
 {
     extern __shared__ double Zs[];
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -443,7 +449,7 @@ __global__ void __launch_bounds__(64, 2) k_vert_visc_chunked(VVArgs<T> p)
 
 // every member of an ensemble in one launch (LDS sized for the deepest member)
 template <typename T>
-__global__ void __launch_bounds__(64, 2) k_vert_visc_batch(const VVArgs<T> *__restrict__ args, BatchMap mp)
+__global__ void __launch_bounds__(64, 1) k_vert_visc_batch(const VVArgs<T> *__restrict__ args, BatchMap mp)
 {
     extern __shared__ double Zs[];
     int m, local;
