@@ -358,7 +358,10 @@ template <typename T>
 __device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
 
 // zonal_flux_layer (:516-598) with the upwind cell selected instead of
-// branched; the same operations in the same order as zflux().
+// branched; the same operations in the same order as zflux().  CFL = |c|:
+// dt > 0 and IdxT > 0, so c has the sign of u and |c| is the reference's
+// (u > 0 ? c : -c) (the u = 0 case is replaced below).  Both marginal
+// thicknesses are formed and one is selected (no divergent branch per level).
 template <typename T>
 __device__ __forceinline__ void zflux_sel(T u, T dt, T dy, T IdxT0, T IdxT1, T hW0, T hE0, T h0, T hW1, T hE1,
                                           T h1, T &uh, T &hmarg)
@@ -368,14 +371,16 @@ __device__ __forceinline__ void zflux_sel(T u, T dt, T dy, T IdxT0, T IdxT1, T h
     const T B = pos ? hW0 : hE1;       // upwind far edge
     const T hc = pos ? h0 : h1;
     const T c = u * dt * (pos ? IdxT0 : IdxT1);
-    const T CFL = pos ? c : -c;
+    const T CFL = vabs(c);
     const T curv_3 = (B + A) - T(2.0) * hc;
     const T D = B - A;
     const T f = (dy * u) * (A + CFL * (T(0.5) * D + curv_3 * (CFL - T(1.5))));
-    const T hm = A + CFL * (D + T(3.0) * curv_3 * (CFL - T(1.0)));
+    T hm = A + CFL * (D + T(3.0) * curv_3 * (CFL - T(1.0)));
+    T h0m = T(0.5) * (hW1 + hE0);
+    asm volatile("" : "+v"(hm), "+v"(h0m));   // both computed: a select, not a branch
     const bool zero = !(u > T(0.0)) && !(u < T(0.0));
     uh = zero ? T(0.0) : f;
-    hmarg = zero ? T(0.5) * (hW1 + hE0) : hm;
+    hmarg = zero ? h0m : hm;
 }
 
 // PPM_reconstruction_x (:1124-1247) slope from the three cells, branch-free
@@ -714,17 +719,21 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // one pass: evaluate every level at u + du; terms (layer transport, hm) to
     // LDS.  S0 keeps each lane's transports of its last evaluation, the
     // result the face outputs and the convergence read.
+    // Each level's u is read before the stores of the level above it, so the
+    // LDS read overlaps that level's arithmetic; rows past nk are written too
+    // (they lie inside the arrays and no sum or output reads them).
     auto eval_pass = [&](T du, bool on) __attribute__((always_inline)) {
         if (!on) return;
         col.fence();
+        T u_next = uk(0);
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
+            const T u_k = u_next;
+            if (kk + 1 < K) u_next = uk(kk + 1);
             T f, hm;
-            col.eval(kk, uk(kk) + du, f, hm);
-            if (valid(kk)) {
-                S.S0[k0 + kk][lane] = f;
-                S.S1[k0 + kk][lane] = hm;   // scaled by dy in the column sum
-            }
+            col.eval(kk, u_k + du, f, hm);
+            S.S0[k0 + kk][lane] = f;
+            S.S1[k0 + kk][lane] = hm;   // scaled by dy in the column sum
         }
     };
     // x1 + sum_k dy*hm(k): S1 holds the marginal thickness hm of the pass, so
