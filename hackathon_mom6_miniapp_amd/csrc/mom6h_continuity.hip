@@ -505,72 +505,57 @@ struct ColRegs {
 // LDS of one block: U = u of every level (levels past nk repeat level nk; u
 // lives here rather than in registers), S0/S1 = per-level terms of a pass,
 // R = results of the column sums (one row per quantity; rows 5..8 keep the
-// Newton iteration's starting point for a second iteration).  The column walks
-// read the B rows ending at row nk-1 of a column (B = walk_batch<T>), i.e. up
-// to B-1 rows before row 0: `pad` keeps those reads of U inside the block's
-// LDS (S0 and S1 fall back on the array before them); the values are never used.
+// Newton iteration's starting point for a second iteration).
 template <typename T>
-constexpr int walk_batch = 8;   // rows read ahead of a column walk
+constexpr int walk_batch = 4;   // rows per batch of a column walk (two batches in flight)
 template <typename T, int CW, int K>
 struct ContShared {
     static constexpr int L = CW * K;
-    T pad[walk_batch<T> - 1][CF];
     T U[L][CF], S0[L][CF], S1[L][CF], R[9][CF];
 };
 
 // Visits rows 0..nk-1 (nk <= L) of one LDS column of ContShared in k order: step(x_k).
-// Whole batches of B rows first, their reads issued ahead of the dependent
-// chain; then the last nk % B rows, read as the batch that ends at row nk-1
-// and entered part-way through an unrolled sequence (a jump on the uniform
-// remainder).  No step carries a per-row predicate.
+// Whole batches of B rows first, each batch's reads issued before the batch
+// ahead of it is folded (two batches in flight, so the LDS latency overlaps
+// the dependent chain; the two buffers alternate, so no copies wait on the
+// reads); then the last nk % B (< B) rows one by one.  A remainder entered
+// part-way through an unrolled sequence (a switch that falls through) was
+// miscompiled for one entry point by the compiler of this toolchain: a
+// hoisted operand of the step was left undefined on that path.
 template <int L, typename T, typename Step>
 __device__ __forceinline__ void lds_walk(const T (*col)[CF], int nk, int lane, Step step)
 {
     constexpr int B = walk_batch<T>;
     int k0 = 0;
-    for (; k0 + B <= nk; k0 += B) {
-        T x[B];
+    if (nk >= B) {
+        T x[B], y[B];
+        auto load = [&](T(&v)[B], int r) __attribute__((always_inline)) {
 #pragma unroll
-        for (int b = 0; b < B; ++b) x[b] = col[k0 + b][lane];
+            for (int b = 0; b < B; ++b) v[b] = col[r + b][lane];
+        };
+        auto fold = [&](const T(&v)[B]) __attribute__((always_inline)) {
 #pragma unroll
-        for (int b = 0; b < B; ++b) step(x[b]);
-    }
-    const int skip = B - (nk - k0);   // 1 .. B: rows of the last batch already visited (B: none left)
-    if (skip == B) return;
-    T x[B];
-    const T *last = &col[0][lane] + (nk - B) * CF;   // row nk-B (>= -(B-1): see ContShared::pad)
-#pragma unroll
-    for (int b = 0; b < B; ++b) x[b] = last[b * CF];
-    if constexpr (B == 16) {
-        switch (skip) {
-        case 1: step(x[1]); [[fallthrough]];
-        case 2: step(x[2]); [[fallthrough]];
-        case 3: step(x[3]); [[fallthrough]];
-        case 4: step(x[4]); [[fallthrough]];
-        case 5: step(x[5]); [[fallthrough]];
-        case 6: step(x[6]); [[fallthrough]];
-        case 7: step(x[7]); [[fallthrough]];
-        case 8: step(x[8]); [[fallthrough]];
-        case 9: step(x[9]); [[fallthrough]];
-        case 10: step(x[10]); [[fallthrough]];
-        case 11: step(x[11]); [[fallthrough]];
-        case 12: step(x[12]); [[fallthrough]];
-        case 13: step(x[13]); [[fallthrough]];
-        case 14: step(x[14]); [[fallthrough]];
-        default: step(x[15]);
+            for (int b = 0; b < B; ++b) step(v[b]);
+        };
+        load(x, 0);   // x: rows k0 .. k0+B-1
+        for (; k0 + 3 * B <= nk; k0 += 2 * B) {
+            load(y, k0 + B);
+            fold(x);
+            load(x, k0 + 2 * B);
+            fold(y);
         }
-    } else {
-        static_assert(B == 8, "walk batch of 8 or 16 rows");
-        switch (skip) {
-        case 1: step(x[1]); [[fallthrough]];
-        case 2: step(x[2]); [[fallthrough]];
-        case 3: step(x[3]); [[fallthrough]];
-        case 4: step(x[4]); [[fallthrough]];
-        case 5: step(x[5]); [[fallthrough]];
-        case 6: step(x[6]); [[fallthrough]];
-        default: step(x[7]);
+        if (k0 + 2 * B <= nk) {
+            load(y, k0 + B);
+            fold(x);
+            fold(y);
+            k0 += 2 * B;
+        } else {
+            fold(x);
+            k0 += B;
         }
     }
+#pragma nounroll
+    for (; k0 < nk; ++k0) step(col[k0][lane]);
 }
 
 // Sequential fold acc = op(acc, x_k) over rows 0..nk-1 of one LDS column

@@ -109,9 +109,12 @@ class Oracle:
         return np.ctypeslib.as_array(p, shape=(n.value,)).reshape(shape, order="F")
 
     def load(self, case: dict):
+        # (at nk = 1 a 3D field has the size of a 2D one and field() returns it 2D)
         for name, arr in case.items():
-            self.field(name)[...] = arr
-        self.field("h")[...] = case["h0"]
+            f = self.field(name)
+            f[...] = np.asarray(arr).reshape(f.shape, order="F")
+        f = self.field("h")
+        f[...] = np.asarray(case["h0"]).reshape(f.shape, order="F")
         lib().om_prepare(self.m)
 
     def step(self, niter: int = 1):
