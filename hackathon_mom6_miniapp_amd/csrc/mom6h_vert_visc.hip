@@ -13,6 +13,8 @@
 // zero above the bottom layer (rk2_serial_driver.F90:524-540), so the same
 // expressions are evaluated with Ray = 0 there and the bottom-stress sum keeps
 // only its bottom term.
+#include <type_traits>
+
 #include "mom6h_internal.h"
 
 namespace mom6h {
@@ -189,11 +191,18 @@ __global__ void k_vert_visc(Dims d, double *__restrict__ u, double *__restrict__
 constexpr int VV_C = 12;   // levels per chunk of sweep B (heavy bodies)
 constexpr int VV_A = 24;   // levels per chunk of sweeps A and C (light bodies, latency-bound loads)
 
-// Block (bx, row) of one grid: 64 columns of one row; Zs = (nk+1)*64 doubles of LDS.
+// LDS slot type of the column (z_i, then c1): FP64 for FP64 state; FP32 in the
+// mixed-precision form, where z_i is already FP32 and c1 is rounded once to
+// FP32 (the same rounding the forward-eliminated velocity gets when stored as
+// FP32), which halves the column and gives two waves per SIMD.
+template <typename T>
+using VVSlot = std::conditional_t<sizeof(T) == 4, float, double>;
+
+// Block (bx, row) of one grid: 64 columns of one row; Zs = (nk+1)*64 slots of LDS.
 // Rows are interleaved v, u, v, u, ... (v row J = js-1+r/2, u row j = js+(r-1)/2)
 // so that consecutive rows share their h rows.
 template <typename T, bool KEEP>
-__device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, double *Zs)
+__device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVSlot<T> *Zs)
 {
     const Dims &d = p.d;
     const int lane = threadIdx.x;
@@ -236,7 +245,8 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
         }
         return;
     }
-    auto Z = [&](int k) -> double & { return Zs[(k - 1) * 64 + lane]; };   // level k = 1..nk+1
+    using ZT = VVSlot<T>;
+    auto Z = [&](int k) -> ZT & { return Zs[(k - 1) * 64 + lane]; };   // level k = 1..nk+1
     // C: the arithmetic type of the coefficients (z_i, bottom taper, hvel, the
     // coupling a); the tridiagonal solve is FP64 whatever C is
     using C = T;
@@ -260,7 +270,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
     auto A_level = [&](int k, C ha, C hb) {
         const C h_harm = cdiv(C(2.0) * ha * hb, ha + hb + hn);
         zi = zi + h_harm * I_Hbbl;
-        Z(k) = (double)zi;
+        Z(k) = (ZT)zi;
     };
     int k0 = nk;
     if (k0 >= VV_A) {
@@ -336,7 +346,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
         const C hv_k = hv_n;
         const double hu_k = (double)hv_k + h_neglect;
         if (KEEP) hu_out[(k - 1) * s] = hu_k;
-        Z(k) = dt * a_k * b1;                              // c1(k); z_i(k) is no longer read
+        Z(k) = (ZT)(dt * a_k * b1);                        // c1(k); z_i(k) is no longer read
         const double b_denom_1 = hu_k + dt * (0.0 + a_k * d1);
         b1 = 1.0 / (b_denom_1 + dt * a_next);
         d1 = b_denom_1 * b1;
@@ -396,7 +406,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
         a_n = (double)((Kv + Kv_extra_bbl) / (C(0.5) * hv_k + hn));
         const double hu_k = (double)hv_k + h_neglect;
         if (KEEP) hu_out[(k - 1) * s] = hu_k;
-        Z(k) = dt * a_k * b1;
+        Z(k) = (ZT)(dt * a_k * b1);
         const double b_denom_1 = hu_k + dt * (Ray_bot + a_k * d1);
         b1 = 1.0 / (b_denom_1 + dt * a_n);
         d1 = b_denom_1 * b1;
@@ -442,7 +452,8 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, dou
 template <typename T, bool KEEP>
 __global__ void __launch_bounds__(64, 1) k_vert_visc_chunked(VVArgs<T> p)
 {
-    extern __shared__ double Zs[];
+    extern __shared__ double Zs_raw[];
+    VVSlot<T> *Zs = reinterpret_cast<VVSlot<T> *>(Zs_raw);
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
     vv_body<T, KEEP>(p, tile % gridDim.x, tile / gridDim.x, Zs);
 }
@@ -451,7 +462,8 @@ __global__ void __launch_bounds__(64, 1) k_vert_visc_chunked(VVArgs<T> p)
 template <typename T>
 __global__ void __launch_bounds__(64, 1) k_vert_visc_batch(const VVArgs<T> *__restrict__ args, BatchMap mp)
 {
-    extern __shared__ double Zs[];
+    extern __shared__ double Zs_raw[];
+    VVSlot<T> *Zs = reinterpret_cast<VVSlot<T> *>(Zs_raw);
     int m, local;
     batch_locate(mp, xcd_tile(blockIdx.x, gridDim.x), m, local);
     const VVArgs<T> p = args[m];
@@ -466,7 +478,7 @@ int vert_visc_f32(mom6h_vert_visc_cs *cs, float *u, float *v, const float *h, do
     const Dims &d = cs->d;
     const int ni = d.iec - d.isc + 2;
     const int rows = (d.jec - d.jsc + 1) + (d.jec - d.jsc + 2);
-    const size_t lds = (size_t)(d.nk + 1) * 64 * sizeof(double);
+    const size_t lds = (size_t)(d.nk + 1) * 64 * sizeof(VVSlot<float>);
     MOM6H_REQUIRE((long long)lds <= cs->max_lds, "vert_visc (FP32 state): nk = %d exceeds the LDS column (%lld B)",
                   d.nk, cs->max_lds);
     VVArgs<float> p{d, u, v, h, dt, taux, tauy, ray_u_bot, ray_v_bot, cs->mask2dCu, cs->mask2dCv,
@@ -491,7 +503,7 @@ int vert_visc_batch(const VVArgs<T> *d_args, const VVArgs<T> *h_args, int n, lon
         mp.first[m + 1] = mp.first[m] + mp.nbx[m] * mp.nby[m];
         nk_max = d.nk > nk_max ? d.nk : nk_max;
     }
-    const size_t lds = (size_t)(nk_max + 1) * 64 * sizeof(double);
+    const size_t lds = (size_t)(nk_max + 1) * 64 * sizeof(VVSlot<T>);
     MOM6H_REQUIRE((long long)lds <= max_lds, "vert_visc batch: nk = %d exceeds the LDS column", nk_max);
     k_vert_visc_batch<T><<<(unsigned)mp.first[n], 64, lds, s>>>(d_args, mp);
     MOM6H_LAUNCH_CHECK();
