@@ -190,6 +190,7 @@ __global__ void k_vert_visc(Dims d, double *__restrict__ u, double *__restrict__
 
 constexpr int VV_C = 12;   // levels per chunk of sweep B (heavy bodies)
 constexpr int VV_A = 24;   // levels per chunk of sweeps A and C (light bodies, latency-bound loads)
+constexpr int VV_T = 4;    // levels after the last whole chunk of sweeps A and C loaded with that chunk
 
 // LDS slot type of the column (z_i, then c1): FP64 for FP64 state; FP32 in the
 // mixed-precision form, where z_i is already FP32 and c1 is rounded once to
@@ -273,6 +274,8 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         Z(k) = (ZT)zi;
     };
     int k0 = nk;
+    C ta[VV_T], tb[VV_T];   // the first VV_T levels after the last whole chunk (clamped to level 1)
+    bool tail_ready = false;
     if (k0 >= VV_A) {
         C na[VV_A], nb[VV_A];
 #pragma unroll
@@ -287,10 +290,38 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
                     na[c] = hA[(k0 - VV_A - 1 - c) * s];
                     nb[c] = hB[(k0 - VV_A - 1 - c) * s];
                 }
+            } else if (k0 - VV_A >= 1) {
+#pragma unroll
+                for (int c = 0; c < VV_T; ++c) {
+                    const int k = k0 - VV_A - c >= 1 ? k0 - VV_A - c : 1;
+                    ta[c] = hA[(k - 1) * s];
+                    tb[c] = hB[(k - 1) * s];
+                }
+                tail_ready = true;
             }
 #pragma unroll
             for (int c = 0; c < VV_A; ++c) A_level(k0 - c, ha[c], hb[c]);
         }
+    }
+    // sweep B's first loads (levels 1 and 2, the first chunk's level k+1
+    // terms), issued before the last levels of sweep A are folded; sweep A
+    // writes no velocity and sweep B stores none of these levels before it reads them
+    const C p_v1 = vel[0], p_a1 = hA[0], p_b1 = hB[0];
+    C p_v2 = C(0.0), p_a2 = C(0.0), p_b2 = C(0.0);
+    if (nk >= 2) { p_v2 = vel[s]; p_a2 = hA[s]; p_b2 = hB[s]; }
+    C na[VV_C], nb[VV_C], nv[VV_C];
+    const bool b_chunks = 2 + VV_C - 1 <= nk - 1;
+    if (b_chunks) {
+#pragma unroll
+        for (int c = 0; c < VV_C; ++c) {   // h and u at k+1 for the chunk's levels k = 2 .. VV_C+1
+            na[c] = hA[(2 + c) * s]; nb[c] = hB[(2 + c) * s]; nv[c] = vel[(2 + c) * s];
+        }
+    }
+    if (tail_ready) {
+#pragma unroll
+        for (int c = 0; c < VV_T; ++c)
+            if (k0 - c >= 1) A_level(k0 - c, ta[c], tb[c]);
+        k0 -= VV_T;
     }
     for (; k0 >= 1; --k0) A_level(k0, hA[(k0 - 1) * s], hB[(k0 - 1) * s]);
 
@@ -315,13 +346,13 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
     C z_top, hv_n;
     double a_n, vk_cur, b1, d1, vprev;
     {   // level 1
-        const C v1 = vel[0];
-        const C hv1 = hvel(hA[0], hB[0], v1, botfn(ZC(2)));
+        const C v1 = p_v1;
+        const C hv1 = hvel(p_a1, p_b1, v1, botfn(ZC(2)));
         hv_n = C(0.0);
         z_top = C(0.0);
         if (nk >= 2) {
-            const C v2 = vel[s];
-            hv_n = hvel(hA[s], hB[s], v2, botfn(ZC(3)));
+            const C v2 = p_v2;
+            hv_n = hvel(p_a2, p_b2, v2, botfn(ZC(3)));
             vk_cur = (double)v2;
             z_top = z_top + hv1;
             a_n = (double)coefA(z_top, botfn(ZC(2)), hv1, hv_n);
@@ -357,12 +388,9 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         vk_cur = vk_next;
     };
     k0 = 2;
-    if (k0 + VV_C - 1 <= nk - 1) {
-        C na[VV_C], nb[VV_C], nv[VV_C];
-#pragma unroll
-        for (int c = 0; c < VV_C; ++c) {   // h and u at k+1 for the chunk's levels
-            na[c] = hA[(k0 + c) * s]; nb[c] = hB[(k0 + c) * s]; nv[c] = vel[(k0 + c) * s];
-        }
+    C tb_a = C(0.0), tb_b = C(0.0), tb_v = C(0.0);   // the first level after the last whole chunk
+    bool tb_ready = false;
+    if (b_chunks) {
         for (; k0 + VV_C - 1 <= nk - 1; k0 += VV_C) {   // levels k0 .. k0+VV_C-1
             C ha[VV_C], hb[VV_C], vv[VV_C];
 #pragma unroll
@@ -374,6 +402,11 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
                     nb[c] = hB[(k0 + VV_C + c) * s];
                     nv[c] = vel[(k0 + VV_C + c) * s];
                 }
+            } else if (k0 + VV_C <= nk - 1) {   // its velocity is stored after this load
+                tb_a = hA[(k0 + VV_C) * s];
+                tb_b = hB[(k0 + VV_C) * s];
+                tb_v = vel[(k0 + VV_C) * s];
+                tb_ready = true;
             }
             // off-recurrence terms: bf[c] = botfn(z_i(k0+1+c)), hvel(k+1), z_top, a(k+1)
             C bf[VV_C + 1], hvx[VV_C], ax[VV_C];
@@ -392,8 +425,14 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         }
     }
     for (; k0 <= nk - 1; ++k0) {
-        const C vnx = vel[k0 * s];
-        const C hvx = hvel(hA[k0 * s], hB[k0 * s], vnx, botfn(ZC(k0 + 2)));
+        C vnx, ha_, hb_;
+        if (tb_ready) {
+            vnx = tb_v; ha_ = tb_a; hb_ = tb_b;
+            tb_ready = false;
+        } else {
+            vnx = vel[k0 * s]; ha_ = hA[k0 * s]; hb_ = hB[k0 * s];
+        }
+        const C hvx = hvel(ha_, hb_, vnx, botfn(ZC(k0 + 2)));
         z_top = z_top + hv_n;
         const C ax = coefA(z_top, botfn(ZC(k0 + 1)), hv_n, hvx);
         B_serial(k0, (double)ax, hvx, (double)vnx);
@@ -418,6 +457,8 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
     // ---- sweep C, bottom-up back substitution (:447-449) ----
     double vnext = vprev;
     k0 = nk - 1;
+    double tf[VV_T];   // the first VV_T levels after the last whole chunk (clamped to level 1)
+    bool tc_ready = false;
     if (k0 >= VV_A) {
         double nf[VV_A];
 #pragma unroll
@@ -429,6 +470,13 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
             if (k0 - VV_A >= VV_A) {
 #pragma unroll
                 for (int c = 0; c < VV_A; ++c) nf[c] = vel[(k0 - VV_A - 1 - c) * s];
+            } else if (k0 - VV_A >= 1) {   // below this chunk's stores
+#pragma unroll
+                for (int c = 0; c < VV_T; ++c) {
+                    const int k = k0 - VV_A - c >= 1 ? k0 - VV_A - c : 1;
+                    tf[c] = vel[(k - 1) * s];
+                }
+                tc_ready = true;
             }
 #pragma unroll
             for (int c = 0; c < VV_A; ++c) {
@@ -436,6 +484,16 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
                 vel[(k0 - c - 1) * s] = (T)vnext;
             }
         }
+    }
+    if (tc_ready) {
+#pragma unroll
+        for (int c = 0; c < VV_T; ++c) {
+            if (k0 - c >= 1) {
+                vnext = tf[c] + Z(k0 - c + 1) * vnext;
+                vel[(k0 - c - 1) * s] = (T)vnext;
+            }
+        }
+        k0 -= VV_T;
     }
     for (; k0 >= 1; --k0) {
         vnext = vel[(k0 - 1) * s] + Z(k0 + 1) * vnext;
@@ -450,7 +508,10 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
 
 // one grid, tiles renumbered per XCD
 template <typename T, bool KEEP>
-__global__ void __launch_bounds__(64, 1) k_vert_visc_chunked(VVArgs<T> p)
+// FP64: one wave per SIMD (the LDS column), registers up to 512 with AGPRs;
+// FP32 state: the half-size column allows two, and the register budget keeps them
+__global__ void __launch_bounds__(64, 1) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+k_vert_visc_chunked(VVArgs<T> p)
 {
     extern __shared__ double Zs_raw[];
     VVSlot<T> *Zs = reinterpret_cast<VVSlot<T> *>(Zs_raw);
@@ -460,7 +521,8 @@ __global__ void __launch_bounds__(64, 1) k_vert_visc_chunked(VVArgs<T> p)
 
 // every member of an ensemble in one launch (LDS sized for the deepest member)
 template <typename T>
-__global__ void __launch_bounds__(64, 1) k_vert_visc_batch(const VVArgs<T> *__restrict__ args, BatchMap mp)
+__global__ void __launch_bounds__(64, 1) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
+k_vert_visc_batch(const VVArgs<T> *__restrict__ args, BatchMap mp)
 {
     extern __shared__ double Zs_raw[];
     VVSlot<T> *Zs = reinterpret_cast<VVSlot<T> *>(Zs_raw);
