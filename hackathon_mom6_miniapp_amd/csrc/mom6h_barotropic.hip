@@ -692,7 +692,7 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
                   "mom6h_barotropic_init: NULL array");
     auto *cs = new mom6h_barotropic_cs();
     cs->tile_rows = 4;
-    cs->substeps = 4;
+    cs->substeps = 5;
     cs->use_rows = 1;
     cs->d = make_dims(isd, ied, jsd, jed, isc, iec, jsc, jec, 1);
     cs->halo = Halo{periodic_x, periodic_y};

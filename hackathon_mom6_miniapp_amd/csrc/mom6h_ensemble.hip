@@ -85,7 +85,7 @@ struct mom6h_ensemble {
     hipGraphExec_t gexec = nullptr;
     int momentum_tile = 8;
     int bt_tile = 4;         // rows of the barotropic substep tile (4, 8 or 16)
-    int bt_substeps = 4;     // barotropic substeps per blocked launch (1 = one launch per substep)
+    int bt_substeps = 5;     // barotropic substeps per blocked launch (1 = one launch per substep)
     long long max_lds = 0;
     StepTables<float> tf;
     StepTables<double> td;

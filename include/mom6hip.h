@@ -263,7 +263,7 @@ int mom6h_model_synchronize(mom6h_model *m);
  * layers, 1..nk; default 1), "bt_tile" (rows of the barotropic substep tile,
  * 4, 8 or 16; after prepare; default 4), "bt_substeps" (barotropic substeps per
  * launch between the first and the last of a call: 1 = one launch per substep,
- * up to 5 = the blocked kernel; after prepare; default 4), "bt_rows" (1 = the
+ * up to 5 = the blocked kernel; after prepare; default 5), "bt_rows" (1 = the
  * blocked kernel stages Datv and the Coriolis factor per row when both are
  * constant along the compute rows, 0 = always as 2D fields; after prepare;
  * default 1), "fused" (1 = fused momentum stencil and
@@ -303,7 +303,7 @@ int mom6h_ensemble_synchronize(mom6h_ensemble *e);
 /* Options: "graph" (1 = capture one step, replay it per step), "momentum_tile"
  * (rows of the fused momentum tile: 4, 8 or 16; default 8), "momentum_splits"
  * (blocks per tile sharing its layers; after prepare; default 1), "bt_tile" (4, 8 or 16),
- * "bt_substeps" (barotropic substeps per blocked launch, 1-5; default 4). */
+ * "bt_substeps" (barotropic substeps per blocked launch, 1-5; default 5). */
 int mom6h_ensemble_set_option(mom6h_ensemble *e, const char *key, double value);
 /* Times the fused momentum launch of the ensemble (reps launches) for every
  * launch shape (tile rows x layer splits), keeps the fastest and writes the
