@@ -21,6 +21,10 @@ MIXED_CASES = {
                                coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3),
     "doubly_periodic_fplane_merid": Config(ni=40, nj=44, nk=20, bt_nsteps=20, bc_x=BC_PERIODIC,
                                            bc_y=BC_PERIODIC, coriolis="fplane", f0=1e-4, meridional=True),
+    # more than 32 levels: the deep FP32 continuity block (8 waves x 10 levels)
+    # and the FP32 vertical-viscosity column at full chunks
+    "walls_beta_deep_75": Config(ni=40, nj=32, nk=75, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                 coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3),
     # Laplacian + biharmonic viscosity (Kh4 = 1e11 m^4/s)
     "walls_beta_biharmonic": Config(ni=48, nj=40, nk=10, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
                                     coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, biharmonic=True,
