@@ -505,13 +505,17 @@ struct ColRegs {
 // LDS of one block: U = u of every level (levels past nk repeat level nk; u
 // lives here rather than in registers), S0/S1 = per-level terms of a pass,
 // R = results of the column sums (one row per quantity; rows 5..8 keep the
-// Newton iteration's starting point for a second iteration).
+// Newton iteration's starting point for a second iteration), F = the votes of
+// block_any.
 template <typename T>
 constexpr int walk_batch = 4;   // rows per batch of a column walk (two batches in flight)
 template <typename T, int CW, int K>
 struct ContShared {
     static constexpr int L = CW * K;
     T U[L][CF], S0[L][CF], S1[L][CF], R[9][CF];
+    // per-wave votes of the block-wide "any" tests: 0, 1 alternate over the
+    // Newton iterations, 2 = the barotropic-transport test, 3 = the du0 pass test
+    int F[4][16];
 };
 
 // Visits rows 0..nk-1 (nk <= L) of one LDS column of ContShared in k order: step(x_k).
@@ -556,6 +560,23 @@ __device__ __forceinline__ void lds_walk(const T (*col)[CF], int nk, int lane, S
     }
 #pragma nounroll
     for (; k0 < nk; ++k0) step(col[k0][lane]);
+}
+
+// Block-wide "any lane has pred" with one barrier (the runtime's
+// __syncthreads_or takes three): each wave stores its vote in its own word of
+// F, then every thread reads the CW words.  The caller passes a buffer that no
+// other wave can still be reading: consecutive uses of one buffer must be
+// separated by another barrier.
+template <int CW>
+__device__ __forceinline__ bool block_any(bool pred, int *F, int w, int lane)
+{
+    const bool vote = __any(pred);
+    if (lane == 0) F[w] = vote ? 1 : 0;
+    __syncthreads();
+    int r = 0;
+#pragma unroll
+    for (int q = 0; q < CW; ++q) r |= F[q];
+    return r != 0;
 }
 
 // Sequential fold acc = op(acc, x_k) over rows 0..nk-1 of one LDS column
@@ -823,7 +844,8 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             }
             // a finished column keeps its du for the rest of the reference's row loop
             const bool ev = do_I && ((itt < max_itts) || keep);
-            if (!__syncthreads_or(ev)) break;
+            // buffers alternate by iteration: the sum's two barriers lie between two uses
+            if (!block_any<CW>(ev, S.F[itt & 1], w, lane)) break;
             eval_pass(du, ev);
             const bool summed = ev && itt < max_itts;
             sum_pass(-uhbt, T(0.0), summed);
@@ -891,7 +913,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // set_zonal_BT_cont (:920-1121).  The zero-transport correction equals du
     // whenever the target is zero (identical Newton iterations).
     T du0 = du, du0_eval = du_eval, FAmt_0 = sum_dyhm;
-    if (__syncthreads_or(bt != T(0.0))) {
+    if (a.uhbt && block_any<CW>(bt != T(0.0), S.F[2], w, lane)) {   // without uhbt, bt = 0 everywhere
         T de, s0;
         const Start again{S.R[5][lane], S.R[6][lane], S.R[7][lane], S.R[8][lane]};
         const T d2 = flux_adjust(again, T(0.0), false, &de, &s0);
@@ -900,7 +922,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // FAmt_0 = sum dy*hm(u + du0): the sum of the iteration's last evaluation
     // when that was at du0 (the same terms in the same order), else a pass of its own
     const bool have0 = (du0_eval == du0);
-    if (__syncthreads_or(!have0)) {
+    if (block_any<CW>(!have0, S.F[3], w, lane)) {
         eval_pass(du0, !have0);
         sum_pass(T(0.0), T(0.0), !have0);
         if (!have0) FAmt_0 = S.R[1][lane];
