@@ -69,9 +69,11 @@ struct mom6h_model {
     mom6h_barotropic_cs *bt_cs = nullptr;
     mom6h_vert_visc_cs *vv = nullptr;
     mom6h_hor_visc_cs *hv = nullptr;
-    // one step captured as a graph (both streams, fork and join inside)
-    hipGraph_t graph = nullptr;
+    // one step captured as two graphs, one per stream (replayed with one fork and one join per call)
+    hipGraph_t graph = nullptr;          // one 3D step (stream s0)
     hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph_bt = nullptr;       // one barotropic step (stream s1)
+    hipGraphExec_t gexec_bt = nullptr;
     int graph_fused = -1;    // fused_ok() at capture time
     // profiling
     std::vector<hipEvent_t> ev;

@@ -584,6 +584,37 @@ int mom6h_model_prepare(mom6h_model *m)
     return MOM6H_OK;
 }
 
+// Destroys the captured step graphs (a launch shape or option they hold changed)
+static void drop_graphs(mom6h_model *m)
+{
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    if (m->gexec_bt) (void)hipGraphExecDestroy(m->gexec_bt);
+    if (m->graph_bt) (void)hipGraphDestroy(m->graph_bt);
+    m->gexec = nullptr;
+    m->graph = nullptr;
+    m->gexec_bt = nullptr;
+    m->graph_bt = nullptr;
+}
+
+// Captures what enq enqueues on stream s as a graph and instantiates it
+static int capture_on(mom6h_model *m, hipStream_t s, int (*enq)(mom6h_model *), hipGraph_t *graph,
+                      hipGraphExec_t *exec)
+{
+    MOM6H_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enq(m);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    MOM6H_CHECK(ec);
+    *graph = g;
+    MOM6H_CHECK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return MOM6H_OK;
+}
+
 int mom6h_model_step(mom6h_model *m, int niter)
 {
     MOM6H_REQUIRE(m != nullptr, "mom6h_model_step: NULL model");
@@ -593,36 +624,27 @@ int mom6h_model_step(mom6h_model *m, int niter)
     // module path keeps them in the control structure, as the reference does.
     RC(mom6h_vert_visc_keep_coefficients(m->vv, fused_ok(m) ? 0 : 1));
     if (m->use_graph && !m->profile && !m->serialize) {
-        // One step (barotropic chain forked onto s1 and joined back) replayed
-        // as a graph; steps then no longer overlap across the join.
+        // One 3D step (s0) and one barotropic step (s1), each captured as a
+        // graph on its own stream and replayed as the eager path enqueues:
+        // one fork, every barotropic step on s1, every 3D step on s0, one join
+        // (so a barotropic step may run beside any 3D step of the call).
         const int key = fused_ok(m) ? 1 : 0;
-        if (m->gexec && m->graph_fused != key) {
-            (void)hipGraphExecDestroy(m->gexec);
-            (void)hipGraphDestroy(m->graph);
-            m->gexec = nullptr;
-            m->graph = nullptr;
-        }
+        if (m->gexec && m->graph_fused != key) drop_graphs(m);
         if (!m->gexec) {
-            MOM6H_CHECK(hipStreamBeginCapture(m->s0, hipStreamCaptureModeThreadLocal));
-            int rc = MOM6H_OK;
-            if (hipEventRecord(m->e_start, m->s0) != hipSuccess || hipStreamWaitEvent(m->s1, m->e_start, 0) != hipSuccess)
-                rc = set_error(MOM6H_ESTATE, "mom6h_model_step: fork into the captured step failed");
-            if (!rc) rc = enqueue_barotropic(m);
-            if (!rc) rc = enqueue_step(m);
-            if (!rc && (hipEventRecord(m->e_bt, m->s1) != hipSuccess || hipStreamWaitEvent(m->s0, m->e_bt, 0) != hipSuccess))
-                rc = set_error(MOM6H_ESTATE, "mom6h_model_step: join of the captured step failed");
-            hipGraph_t g = nullptr;
-            const hipError_t ec = hipStreamEndCapture(m->s0, &g);
+            RC(capture_on(m, m->s0, enqueue_step, &m->graph, &m->gexec));
+            const int rc = capture_on(m, m->s1, enqueue_barotropic, &m->graph_bt, &m->gexec_bt);
             if (rc) {
-                if (g) (void)hipGraphDestroy(g);
+                drop_graphs(m);
                 return rc;
             }
-            MOM6H_CHECK(ec);
-            m->graph = g;
-            MOM6H_CHECK(hipGraphInstantiate(&m->gexec, m->graph, nullptr, nullptr, 0));
             m->graph_fused = key;
         }
+        MOM6H_CHECK(hipEventRecord(m->e_start, m->s0));
+        MOM6H_CHECK(hipStreamWaitEvent(m->s1, m->e_start, 0));
+        for (int it = 0; it < niter; ++it) MOM6H_CHECK(hipGraphLaunch(m->gexec_bt, m->s1));
         for (int it = 0; it < niter; ++it) MOM6H_CHECK(hipGraphLaunch(m->gexec, m->s0));
+        MOM6H_CHECK(hipEventRecord(m->e_bt, m->s1));
+        MOM6H_CHECK(hipStreamWaitEvent(m->s0, m->e_bt, 0));
         return MOM6H_OK;
     }
     if (m->serialize) {
@@ -672,34 +694,19 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
         MOM6H_REQUIRE(m->bt_cs, "mom6h_model_set_option: bt_tile after prepare");
         MOM6H_REQUIRE(value == 4.0 || value == 8.0 || value == 16.0, "mom6h_model_set_option: bt_tile must be 4, 8 or 16");
         m->bt_cs->tile_rows = (int)value;
-        if (m->gexec) {
-            (void)hipGraphExecDestroy(m->gexec);
-            (void)hipGraphDestroy(m->graph);
-            m->gexec = nullptr;
-            m->graph = nullptr;
-        }
+        drop_graphs(m);
     }
     else if (k == "bt_rows") {
         MOM6H_REQUIRE(m->bt_cs, "mom6h_model_set_option: bt_rows after prepare");
         m->bt_cs->use_rows = value != 0.0;
-        if (m->gexec) {
-            (void)hipGraphExecDestroy(m->gexec);
-            (void)hipGraphDestroy(m->graph);
-            m->gexec = nullptr;
-            m->graph = nullptr;
-        }
+        drop_graphs(m);
     }
     else if (k == "bt_substeps") {
         MOM6H_REQUIRE(m->bt_cs, "mom6h_model_set_option: bt_substeps after prepare");
         MOM6H_REQUIRE(value >= 1.0 && value <= 5.0 && value == (double)(int)value,
                       "mom6h_model_set_option: bt_substeps must be 1..5");
         m->bt_cs->substeps = (int)value;
-        if (m->gexec) {
-            (void)hipGraphExecDestroy(m->gexec);
-            (void)hipGraphDestroy(m->graph);
-            m->gexec = nullptr;
-            m->graph = nullptr;
-        }
+        drop_graphs(m);
     }
     else if (k == "momentum_tile" || k == "momentum_splits") {
         const int v = (int)value;
@@ -710,12 +717,7 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
             MOM6H_REQUIRE(v >= 1 && v <= m->c.nk, "mom6h_model_set_option: momentum_splits must be 1..nk");
             m->momentum_splits = v;
         }
-        if (m->gexec) {   // the captured step holds the old launch shape
-            (void)hipGraphExecDestroy(m->gexec);
-            (void)hipGraphDestroy(m->graph);
-            m->gexec = nullptr;
-            m->graph = nullptr;
-        }
+        drop_graphs(m);   // the captured step holds the old launch shape
     }
     else if (k == "fused") {
         MOM6H_REQUIRE(!m->mixed || value != 0.0, "mom6h_model_set_option: the mixed-precision form is fused only");
@@ -796,12 +798,7 @@ int mom6h_model_autotune(mom6h_model *m, int reps, char *report, int nbytes)
     m->profile = saved_profile;
     m->momentum_tile = pick;
     m->momentum_splits = pick_s;
-    if (m->gexec) {
-        (void)hipGraphExecDestroy(m->gexec);
-        (void)hipGraphDestroy(m->graph);
-        m->gexec = nullptr;
-        m->graph = nullptr;
-    }
+    drop_graphs(m);
     if (rc) return rc;
     rep += " chosen " + std::to_string(pick) + "x" + std::to_string(pick_s);
     if (report && nbytes > 0) snprintf(report, (size_t)nbytes, "%s", rep.c_str());
@@ -822,8 +819,7 @@ int mom6h_model_destroy(mom6h_model *m)
         if (f.owned && *f.p) (void)hipFree(*f.p);
     mom6h_bt_cont_free(&m->bt);
     mom6h_bt_cont_free(&m->bt_v);
-    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
-    if (m->graph) (void)hipGraphDestroy(m->graph);
+    drop_graphs(m);
     if (m->e_start) (void)hipEventDestroy(m->e_start);
     if (m->e_bt) (void)hipEventDestroy(m->e_bt);
     if (m->s0) (void)hipStreamDestroy(m->s0);

@@ -257,8 +257,9 @@ int mom6h_model_fill(mom6h_model *m, const char *name, const double *d_src, long
 int mom6h_model_prepare(mom6h_model *m);
 int mom6h_model_step(mom6h_model *m, int niter);
 int mom6h_model_synchronize(mom6h_model *m);
-/* Options: "graph" (1 = capture one step, both streams, as a hipGraph and
- * replay it per step; default 0), "momentum_tile" (rows of the fused momentum
+/* Options: "graph" (1 = capture one step as two hipGraphs, the 3D chain on
+ * the model's first stream and the barotropic chain on its second, and replay
+ * them per step, joined once per call as the eager path is; default 0), "momentum_tile" (rows of the fused momentum
  * tile: 4, 8 or 16; default 8), "momentum_splits" (blocks per tile sharing its
  * layers, 1..nk; default 1), "bt_tile" (rows of the barotropic substep tile,
  * 4, 8 or 16; after prepare; default 4), "bt_substeps" (barotropic substeps per
