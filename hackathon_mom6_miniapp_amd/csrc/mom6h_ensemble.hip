@@ -82,7 +82,9 @@ struct mom6h_ensemble {
     hipEvent_t e_start = nullptr, e_bt = nullptr;
     int use_graph = 0;
     hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t gexec = nullptr;      // one 3D step of every member (stream s0)
+    hipGraph_t graph_bt = nullptr;       // one barotropic step of every member (stream s1)
+    hipGraphExec_t gexec_bt = nullptr;
     int momentum_tile = 8;
     int bt_tile = 4;         // rows of the barotropic substep tile (4, 8 or 16)
     int bt_substeps = 5;     // barotropic substeps per blocked launch (1 = one launch per substep)
@@ -202,14 +204,27 @@ int enqueue_bt(mom6h_ensemble *e)
     return halo(e->h2d, e->s1);
 }
 
-int enqueue_step(mom6h_ensemble *e)
+// one 3D step of every member on s0
+int enqueue_3d_step(mom6h_ensemble *e)
 {
-    MOM6H_CHECK(hipEventRecord(e->e_start, e->s0));
-    MOM6H_CHECK(hipStreamWaitEvent(e->s1, e->e_start, 0));
-    RC(enqueue_bt(e));
-    RC(e->mixed ? enqueue_3d<float>(e, e->tf) : enqueue_3d<double>(e, e->td));
-    MOM6H_CHECK(hipEventRecord(e->e_bt, e->s1));
-    MOM6H_CHECK(hipStreamWaitEvent(e->s0, e->e_bt, 0));
+    return e->mixed ? enqueue_3d<float>(e, e->tf) : enqueue_3d<double>(e, e->td);
+}
+
+// Captures what enq enqueues on stream s as a graph and instantiates it
+int capture_on(mom6h_ensemble *e, hipStream_t s, int (*enq)(mom6h_ensemble *), hipGraph_t *graph,
+               hipGraphExec_t *exec)
+{
+    MOM6H_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enq(e);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    MOM6H_CHECK(ec);
+    *graph = g;
+    MOM6H_CHECK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
     return MOM6H_OK;
 }
 
@@ -228,8 +243,12 @@ void drop_graph(mom6h_ensemble *e)
 {
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
+    if (e->gexec_bt) (void)hipGraphExecDestroy(e->gexec_bt);
+    if (e->graph_bt) (void)hipGraphDestroy(e->graph_bt);
     e->gexec = nullptr;
     e->graph = nullptr;
+    e->gexec_bt = nullptr;
+    e->graph_bt = nullptr;
 }
 
 // mean time (ms) of reps predictor momentum launches with tile height ty
@@ -335,24 +354,29 @@ int mom6h_ensemble_step(mom6h_ensemble *e, int niter)
     MOM6H_REQUIRE(e != nullptr, "mom6h_ensemble_step: NULL ensemble");
     MOM6H_REQUIRE(e->prepared, "mom6h_ensemble_step: call mom6h_ensemble_prepare first");
     MOM6H_REQUIRE(niter >= 0, "mom6h_ensemble_step: niter >= 0 required");
-    if (!e->use_graph) {
-        for (int it = 0; it < niter; ++it) RC(enqueue_step(e));
-        return MOM6H_OK;
-    }
-    if (!e->gexec) {
-        MOM6H_CHECK(hipStreamBeginCapture(e->s0, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_step(e);
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(e->s0, &g);
+    // The barotropic chain and the 3D chain share no array: one fork, every
+    // barotropic step on s1, every 3D step on s0, one join (as one graph per
+    // stream and step with the graph option)
+    if (e->use_graph && !e->gexec) {
+        RC(capture_on(e, e->s0, enqueue_3d_step, &e->graph, &e->gexec));
+        const int rc = capture_on(e, e->s1, enqueue_bt, &e->graph_bt, &e->gexec_bt);
         if (rc) {
-            if (g) (void)hipGraphDestroy(g);
+            drop_graph(e);
             return rc;
         }
-        MOM6H_CHECK(ec);
-        e->graph = g;
-        MOM6H_CHECK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
     }
-    for (int it = 0; it < niter; ++it) MOM6H_CHECK(hipGraphLaunch(e->gexec, e->s0));
+    MOM6H_CHECK(hipEventRecord(e->e_start, e->s0));
+    MOM6H_CHECK(hipStreamWaitEvent(e->s1, e->e_start, 0));
+    for (int it = 0; it < niter; ++it) {
+        if (e->use_graph) MOM6H_CHECK(hipGraphLaunch(e->gexec_bt, e->s1));
+        else RC(enqueue_bt(e));
+    }
+    for (int it = 0; it < niter; ++it) {
+        if (e->use_graph) MOM6H_CHECK(hipGraphLaunch(e->gexec, e->s0));
+        else RC(enqueue_3d_step(e));
+    }
+    MOM6H_CHECK(hipEventRecord(e->e_bt, e->s1));
+    MOM6H_CHECK(hipStreamWaitEvent(e->s0, e->e_bt, 0));
     return MOM6H_OK;
 }
 
