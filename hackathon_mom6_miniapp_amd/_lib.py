@@ -2,8 +2,10 @@
 is missing, or a call fails, an exception says so."""
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmom6hip.so")
@@ -27,6 +29,11 @@ class MConfig(C.Structure):
 
 
 _lib = None
+# models and ensembles not yet closed: closed by an exit hook registered when
+# the library loads, so that no device call is made while the interpreter
+# tears its modules down (in no defined order) or after the HIP runtime and a
+# profiler's tool library have run their own exit handlers
+_LIVE: "weakref.WeakSet" = weakref.WeakSet()
 P = C.c_void_p
 D = C.POINTER(C.c_double)
 
@@ -86,7 +93,25 @@ def lib() -> C.CDLL:
             f.argtypes = args
             f.restype = res
         _lib = L
+        atexit.register(close_all)
     return _lib
+
+
+def track(obj) -> None:
+    """Register a live model or ensemble (it has a close() method)."""
+    _LIVE.add(obj)
+
+
+def close_all() -> None:
+    """Close every model and ensemble still open (ensembles first: they own
+    their members' device state).  Runs at interpreter exit."""
+    live = list(_LIVE)
+    for obj in sorted(live, key=lambda o: 0 if hasattr(o, "members") else 1):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    _LIVE.clear()
 
 
 def check(rc: int, what: str = ""):

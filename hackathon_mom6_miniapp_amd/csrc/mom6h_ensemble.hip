@@ -392,6 +392,10 @@ int mom6h_ensemble_set_option(mom6h_ensemble *e, const char *key, double value)
 {
     MOM6H_REQUIRE(e && key, "mom6h_ensemble_set_option: NULL argument");
     const std::string k(key);
+    // Steps already enqueued read the argument tables rewritten below and may
+    // be replaying a graph about to be destroyed: let them finish first.
+    if (e->s0) MOM6H_CHECK(hipStreamSynchronize(e->s0));
+    if (e->s1) MOM6H_CHECK(hipStreamSynchronize(e->s1));
     if (k == "graph") {
         e->use_graph = value != 0.0;
     } else if (k == "momentum_tile") {

@@ -688,6 +688,10 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
 {
     MOM6H_REQUIRE(m && key, "mom6h_model_set_option: NULL argument");
     const std::string k(key);
+    // Steps already enqueued read the launch shape and may be replaying a
+    // graph about to be destroyed: let them finish before anything changes.
+    if (m->s0) MOM6H_CHECK(hipStreamSynchronize(m->s0));
+    if (m->s1) MOM6H_CHECK(hipStreamSynchronize(m->s1));
     if (k == "graph") m->use_graph = value != 0.0;
     else if (k == "serialize") m->serialize = value != 0.0;
     else if (k == "bt_tile") {

@@ -4,10 +4,11 @@ advances it with mom6h_model_step and reads fields back."""
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 
-from ._lib import MConfig, check, lib
+from ._lib import MConfig, check, lib, track
 from .state import BC_PERIODIC, BC_WALL, Config, Seeded
 
 # inputs uploaded before mom6h_model_prepare (2D first: masks feed the 3D recipes)
@@ -46,6 +47,7 @@ class Model:
             return
         mc = to_mconfig(cfg)
         check(self.L.mom6h_model_create(C.byref(self.m), C.byref(mc)), "mom6h_model_create")
+        track(self)
         self.fill_inputs(case)
         check(self.L.mom6h_model_set_option(self.m, b"graph", float(graph)), "set_option")
         check(self.L.mom6h_model_prepare(self.m), "mom6h_model_prepare")
@@ -72,6 +74,10 @@ class Model:
         self.m = C.c_void_p()
 
     def __del__(self):
+        # at interpreter exit the exit hook has closed it already; make no
+        # device call while modules are being torn down
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -160,6 +166,7 @@ class Ensemble:
         self.e = C.c_void_p()
         arr = (MConfig * len(cfgs))(*[to_mconfig(c) for c in cfgs])
         check(self.L.mom6h_ensemble_create(C.byref(self.e), arr, len(cfgs)), "mom6h_ensemble_create")
+        track(self)
         self.members = []
         for q, (cfg, case) in enumerate(zip(cfgs, cases)):
             h = C.c_void_p()
@@ -192,6 +199,8 @@ class Ensemble:
             self.e = C.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

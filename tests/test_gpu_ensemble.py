@@ -90,9 +90,9 @@ def test_ensemble_graph_and_autotune():
     assert "chosen" in report
     for q, m in enumerate(graph.members):
         assert np.array_equal(m.get("u"), before[q])
-    for rep in range(2):
-        eager.step(1)
-        graph.step(1)
+    for rep, n in enumerate((1, 3)):
+        eager.step(n)
+        graph.step(n)
         eager.synchronize()
         graph.synchronize()
         for q in range(len(cfgs)):

@@ -88,11 +88,11 @@ def test_mixed_graph_replay_matches_eager():
     case = make_case(cfg, init="seeded", seed=8)
     eager = Model(cfg, case)
     graph = Model(cfg, case, graph=True)
-    for rep in range(3):
-        print(f"[gpu mixed graph] replay {rep}", flush=True)
-        eager.step(1)
+    for rep, n in enumerate((1, 3, 2)):
+        print(f"[gpu mixed graph] replay {rep}: {n} steps", flush=True)
+        eager.step(n)
         eager.synchronize()
-        graph.step(1)
+        graph.step(n)
         graph.synchronize()
         for f in ("u", "v", "h", "eta", "u_cor", "BT_h_u"):
             assert np.array_equal(eager.get(f), graph.get(f)), (rep, f)

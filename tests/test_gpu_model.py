@@ -66,18 +66,20 @@ def test_seeded_state_generated_on_device_matches_host():
 
 def test_graph_replay_matches_eager():
     """A step captured as a hipGraph and replayed gives the eager result after
-    every replay, with eager work and synchronisations in between."""
+    every call, with eager work and synchronisations in between."""
     from hackathon_mom6_miniapp_amd.model import Model
 
     cfg, init, _ = CASES["zonal_periodic_walls_beta"]
     case = make_case(cfg, init=init, seed=5)
     eager = Model(cfg, case)
     graph = Model(cfg, case, graph=True)
-    for rep in range(3):
-        print(f"[gpu graph] replay {rep}", flush=True)
-        eager.step(1)
+    # calls of 1, 3 and 2 steps: a call of n steps replays n barotropic graphs
+    # on the second stream beside n 3D graphs on the first, joined once
+    for rep, n in enumerate((1, 3, 2)):
+        print(f"[gpu graph] replay {rep}: {n} steps", flush=True)
+        eager.step(n)
         eager.synchronize()
-        graph.step(1)
+        graph.step(n)
         graph.synchronize()
         for f in ("u", "v", "h", "eta", "ubt_av", "vbt_av", "uh", "up"):
             assert np.array_equal(eager.get(f), graph.get(f)), (rep, f)
