@@ -55,6 +55,11 @@ _SIGS = {
                              P, C.c_long, P], C.c_int),
     "mom6h_bt_cfl": ([C.c_double, C.c_int, C.c_double, C.c_double, D, D, C.POINTER(C.c_int),
                       C.POINTER(C.c_int)], C.c_int),
+    "mom6h_barotropic_init": ([C.POINTER(P)] + [C.c_int] * 8 + [C.c_int, C.c_double, C.c_double, C.c_int]
+                              + [D] * 13 + [C.c_int, C.c_int], C.c_int),
+    "mom6h_btstep": ([P] + [P] * 6 + [P], C.c_int),
+    "mom6h_btstep_noalloc": ([P] + [P] * 6 + [P], C.c_int),
+    "mom6h_barotropic_end": ([P], C.c_int),
     "mom6h_model_create": ([C.POINTER(P), C.POINTER(MConfig)], C.c_int),
     "mom6h_model_field": ([P, C.c_char_p, C.POINTER(P), C.POINTER(C.c_long)], C.c_int),
     "mom6h_model_field_bytes": ([P, C.c_char_p, C.POINTER(C.c_int)], C.c_int),

@@ -12,6 +12,7 @@
 // reference never writes (uhbt on I = is-1 and ie, vhbt on J = js-1 and je
 // with the narrow ranges) read as zero, as they do there.
 #include <algorithm>
+#include <mutex>
 #include <utility>
 #include <type_traits>
 #include <vector>
@@ -31,7 +32,8 @@ constexpr int BT_TX = 64;
 constexpr int BT_W = BT_TX + 3;
 
 __global__ void k_bt_copy_in(Dims d, const double *eta_in, const double *ubt_in, const double *vbt_in,
-                             double *eta, double *ubt, double *vbt, double *ubt_av, double *vbt_av)
+                             double *eta, double *ubt, double *vbt, double *ubt_av, double *vbt_av,
+                             double *eta_b, double *ubt_b, double *vbt_b)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= d.s2) return;
@@ -40,6 +42,13 @@ __global__ void k_bt_copy_in(Dims d, const double *eta_in, const double *ubt_in,
     vbt[q] = vbt_in[q];
     ubt_av[q] = 0.0;
     vbt_av[q] = 0.0;
+    // the second state set starts at zero (a control structure's is zeroed
+    // once at init; scratch shared between calls is cleared on every call)
+    if (eta_b) {
+        eta_b[q] = 0.0;
+        ubt_b[q] = 0.0;
+        vbt_b[q] = 0.0;
+    }
 }
 
 // Output tile BT_TX x BT_TY (BT_TY = 4, 8 or 16 rows: the launch's choice),
@@ -773,32 +782,66 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **out, int isd, int ied, int jsd, 
     return MOM6H_OK;
 }
 
-int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ubt_in,
-                 const double *vbt_in, double *u_av, double *v_av, double *eta_av, void *stream)
+}  // extern "C"
+
+namespace mom6h {
+
+// Scratch of mom6h_btstep_noalloc: the two state sets and the two velocity
+// accumulators, 8 planes of n1*n2 values packed from the start, for grids of
+// at most kNoallocPoints points (n1*n2).  Each GPU holds one copy for the
+// life of the process.  Host code must not pass the symbol itself to a
+// kernel (on the host it is a stand-in, not a device address): it uses the
+// address hipGetSymbolAddress gives for the current device.
+constexpr long long kNoallocPoints = 1LL << 20;
+static __device__ __align__(256) double scratch[8 * kNoallocPoints];
+
+// device address of `scratch` on the current device, looked up once per device
+static int noalloc_scratch(double **out)
 {
-    MOM6H_REQUIRE(cs != nullptr, "mom6h_btstep: NULL cs");
-    MOM6H_REQUIRE(eta_in && ubt_in && vbt_in && u_av && v_av && eta_av, "mom6h_btstep: NULL array");
+    static std::mutex mu;
+    static double *ptr[64] = {};
+    int dev = 0;
+    MOM6H_CHECK(hipGetDevice(&dev));
+    MOM6H_REQUIRE(dev >= 0 && dev < 64, "mom6h_btstep_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        MOM6H_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        MOM6H_REQUIRE(p != nullptr, "hipGetSymbolAddress(scratch) gave no address");
+        ptr[dev] = (double *)p;
+    }
+    *out = ptr[dev];
+    return MOM6H_OK;
+}
+
+// One btstep call with its state sets A = (eta, ubt, vbt), B = (eta_b, ubt_b,
+// vbt_b) and accumulators (ubt_av, vbt_av) in the 8 planes w[0..7]; zero_b:
+// clear B on the way in (scratch whose previous contents belong to another call)
+static int btstep_run(mom6h_barotropic_cs *cs, double *const w[8], bool zero_b, const double *eta_in,
+                      const double *ubt_in, const double *vbt_in, double *u_av, double *v_av, double *eta_av,
+                      hipStream_t strm)
+{
     BT p;
     p.d = cs->d;
     p.nstep = cs->nstep; p.first_direction = cs->first_direction;
     p.px = cs->halo.px; p.py = cs->halo.py;
     p.extended = (p.px || p.py) ? 1 : 0;
     p.dtbt = cs->dtbt; p.bebt = cs->bebt; p.dgeo_de = cs->dgeo_de;
-    p.ubt_av = cs->ubt_av; p.vbt_av = cs->vbt_av;
+    p.ubt_av = w[6]; p.vbt_av = w[7];
     // copy-in, then one launch per substep ping-ponging between the two state sets
-    hipStream_t strm = (hipStream_t)stream;
     const Dims &d = cs->d;
     {
         const int bs = 256;
-        k_bt_copy_in<<<(unsigned)((d.s2 + bs - 1) / bs), bs, 0, strm>>>(d, eta_in, ubt_in, vbt_in, cs->eta, cs->ubt,
-                                                                         cs->vbt, cs->ubt_av, cs->vbt_av);
+        k_bt_copy_in<<<(unsigned)((d.s2 + bs - 1) / bs), bs, 0, strm>>>(
+            d, eta_in, ubt_in, vbt_in, w[0], w[1], w[2], w[6], w[7], zero_b ? w[3] : nullptr,
+            zero_b ? w[4] : nullptr, zero_b ? w[5] : nullptr);
         MOM6H_LAUNCH_CHECK();
     }
     const int BT_TY = cs->tile_rows;
     dim3 grid((d.n1 + BT_TX - 1) / BT_TX, (d.n2 + BT_TY - 1) / BT_TY), block(256);
     const dim3 grid_m = bt_multi_grid(d, cs->substeps);
     const int per_launch = cs->substeps;
-    double *A[3] = {cs->eta, cs->ubt, cs->vbt}, *B[3] = {cs->eta_b, cs->ubt_b, cs->vbt_b};
+    double *A[3] = {w[0], w[1], w[2]}, *B[3] = {w[3], w[4], w[5]};
     for (int n = 1; n <= cs->nstep;) {
         // substeps 2 .. nstep-1 in blocks of up to per_launch
         if (per_launch > 1 && cs->blocked_ok && n >= 2 && n <= cs->nstep - 1) {
@@ -844,6 +887,37 @@ int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ub
         ++n;
     }
     return MOM6H_OK;
+}
+
+}  // namespace mom6h
+
+extern "C" {
+
+int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ubt_in,
+                 const double *vbt_in, double *u_av, double *v_av, double *eta_av, void *stream)
+{
+    MOM6H_REQUIRE(cs != nullptr, "mom6h_btstep: NULL cs");
+    MOM6H_REQUIRE(eta_in && ubt_in && vbt_in && u_av && v_av && eta_av, "mom6h_btstep: NULL array");
+    double *const w[8] = {cs->eta, cs->ubt, cs->vbt, cs->eta_b, cs->ubt_b, cs->vbt_b, cs->ubt_av, cs->vbt_av};
+    return btstep_run(cs, w, false, eta_in, ubt_in, vbt_in, u_av, v_av, eta_av, (hipStream_t)stream);
+}
+
+int mom6h_btstep_noalloc(mom6h_barotropic_cs *cs, const double *eta_in, const double *ubt_in,
+                         const double *vbt_in, double *u_av, double *v_av, double *eta_av, void *stream)
+{
+    MOM6H_REQUIRE(cs != nullptr, "mom6h_btstep_noalloc: NULL cs");
+    MOM6H_REQUIRE(eta_in && ubt_in && vbt_in && u_av && v_av && eta_av, "mom6h_btstep_noalloc: NULL array");
+    if (cs->d.s2 > kNoallocPoints)
+        return set_error(MOM6H_EINVAL,
+                         "mom6h_btstep_noalloc: the grid has %lld points (n1*n2), the scratch holds at most %lld",
+                         cs->d.s2, kNoallocPoints);
+    double *base = nullptr;
+    const int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    const long long n = cs->d.s2;
+    double *const w[8] = {base, base + n, base + 2 * n, base + 3 * n, base + 4 * n, base + 5 * n, base + 6 * n,
+                          base + 7 * n};
+    return btstep_run(cs, w, true, eta_in, ubt_in, vbt_in, u_av, v_av, eta_av, (hipStream_t)stream);
 }
 
 int mom6h_barotropic_end(mom6h_barotropic_cs *cs)

@@ -134,6 +134,16 @@ int mom6h_barotropic_init(mom6h_barotropic_cs **cs, int isd, int ied, int jsd, i
 /* eta_in/ubt_in/vbt_in may alias u_av/v_av/eta_av (the driver's corrector call). */
 int mom6h_btstep(mom6h_barotropic_cs *cs, const double *eta_in, const double *ubt_in,
                  const double *vbt_in, double *u_av, double *v_av, double *eta_av, void *stream);
+/* mom6h_btstep without device allocation or a control structure's scratch:
+ * no counterpart in the reference.  For callers that cannot allocate (code
+ * under stream/graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the two state sets and the accumulators live in a static
+ * device array of 64 MiB (8 planes) per GPU, so the grid may have at most
+ * 1,048,576 points n1*n2 (MOM6H_EINVAL beyond).  The array is shared by
+ * every call on a GPU and cleared on each call: calls that overlap on one GPU
+ * (different streams) race.  Same results as mom6h_btstep. */
+int mom6h_btstep_noalloc(mom6h_barotropic_cs *cs, const double *eta_in, const double *ubt_in,
+                         const double *vbt_in, double *u_av, double *v_av, double *eta_av, void *stream);
 int mom6h_barotropic_end(mom6h_barotropic_cs *cs);
 
 /* ------------------------------------------------------------- vert_visc --
