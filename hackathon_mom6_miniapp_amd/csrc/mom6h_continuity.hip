@@ -357,32 +357,6 @@ __device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
 template <typename T>
 __device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
 
-// zonal_flux_layer (:516-598) with the upwind cell selected instead of
-// branched; the same operations in the same order as zflux().  CFL = |c|:
-// dt > 0 and IdxT > 0, so c has the sign of u and |c| is the reference's
-// (u > 0 ? c : -c) (the u = 0 case is replaced below).  Both marginal
-// thicknesses are formed and one is selected (no divergent branch per level).
-template <typename T>
-__device__ __forceinline__ void zflux_sel(T u, T dt, T dy, T IdxT0, T IdxT1, T hW0, T hE0, T h0, T hW1, T hE1,
-                                          T h1, T &uh, T &hmarg)
-{
-    const bool pos = u > T(0.0);
-    const T A = pos ? hE0 : hW1;       // upwind edge facing the face
-    const T B = pos ? hW0 : hE1;       // upwind far edge
-    const T hc = pos ? h0 : h1;
-    const T c = u * dt * (pos ? IdxT0 : IdxT1);
-    const T CFL = vabs(c);
-    const T curv_3 = (B + A) - T(2.0) * hc;
-    const T D = B - A;
-    const T f = (dy * u) * (A + CFL * (T(0.5) * D + curv_3 * (CFL - T(1.5))));
-    T hm = A + CFL * (D + T(3.0) * curv_3 * (CFL - T(1.0)));
-    T h0m = T(0.5) * (hW1 + hE0);
-    asm volatile("" : "+v"(hm), "+v"(h0m));   // both computed: a select, not a branch
-    const bool zero = !(u > T(0.0)) && !(u < T(0.0));
-    uh = zero ? T(0.0) : f;
-    hmarg = zero ? h0m : hm;
-}
-
 // PPM_reconstruction_x (:1124-1247) slope from the three cells, branch-free
 template <typename T>
 __device__ __forceinline__ T ppm_slope_v(T mL, T m0, T mR, T hm, T h0, T hp)
@@ -483,13 +457,48 @@ __device__ __forceinline__ void ppm_limit(int monotonic, float hi, float h_min, 
     }
 }
 
+// The per-level terms of zonal_flux_layer (:516-598) that do not depend on
+// the velocity, formed once per level from the limited PPM edges of the two
+// cells beside the face, for each upwind side: p (u > 0, cell 0 upwind) and
+// n (u < 0, cell 1).  A = the upwind edge facing the face, D = far edge - A,
+// C = curv_3 = (far edge + A) - 2 h: the reference's expressions, evaluated
+// once instead of in every pass of the Newton iteration (same operations on
+// the same operands, so the same values).
 template <typename T, int K>
 struct ColRegs {
-    T wW[K], wE[K], w0[K], eW[K], eE[K], e0[K];
+    T Ap[K], Dp[K], Cp[K], An[K], Dn[K], Cn[K];
     T dt, dy, IdxT0, IdxT1;
-    __device__ __forceinline__ void eval(int kk, T un, T &f, T &hm) const
+    __device__ __forceinline__ void set(int kk, T hW0, T hE0, T h0, T hW1, T hE1, T h1)
     {
-        zflux_sel(un, dt, dy, IdxT0, IdxT1, wW[kk], wE[kk], w0[kk], eW[kk], eE[kk], e0[kk], f, hm);
+        Ap[kk] = hE0;
+        Dp[kk] = hW0 - hE0;
+        Cp[kk] = (hW0 + hE0) - T(2.0) * h0;
+        An[kk] = hW1;
+        Dn[kk] = hE1 - hW1;
+        Cn[kk] = (hE1 + hW1) - T(2.0) * h1;
+    }
+    // the layer transport uh and marginal thickness hm at velocity u: the
+    // upwind side selected instead of branched; CFL = |c| (dt > 0 and
+    // IdxT > 0, so c has the sign of u and |c| is the reference's
+    // (u > 0 ? c : -c)); u = 0 (or NaN) gives uh = 0 and hm = (hW1 + hE0)/2,
+    // handled in a branch that a wave takes only if one of its lanes has it.
+    __device__ __forceinline__ void eval(int kk, T u, T &uh, T &hm) const
+    {
+        const bool pos = u > T(0.0);
+        const T A = pos ? Ap[kk] : An[kk];
+        const T D = pos ? Dp[kk] : Dn[kk];
+        const T curv_3 = pos ? Cp[kk] : Cn[kk];
+        const T c = u * dt * (pos ? IdxT0 : IdxT1);
+        const T CFL = vabs(c);
+        uh = (dy * u) * (A + CFL * (T(0.5) * D + curv_3 * (CFL - T(1.5))));
+        hm = A + CFL * (D + T(3.0) * curv_3 * (CFL - T(1.0)));
+        const bool zero = !(u > T(0.0)) && !(u < T(0.0));
+        if (__any(zero)) {
+            if (zero) {
+                uh = T(0.0);
+                hm = T(0.5) * (An[kk] + Ap[kk]);
+            }
+        }
     }
     // Marks the stored column as rewritten, so that the compiler re-derives the
     // per-level terms of the flux in every pass instead of keeping them live
@@ -498,7 +507,7 @@ struct ColRegs {
     {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk)
-            asm volatile("" : "+v"(wW[kk]), "+v"(wE[kk]), "+v"(w0[kk]), "+v"(eW[kk]), "+v"(eE[kk]), "+v"(e0[kk]));
+            asm volatile("" : "+v"(Ap[kk]), "+v"(Dp[kk]), "+v"(Cp[kk]), "+v"(An[kk]), "+v"(Dn[kk]), "+v"(Cn[kk]));
     }
 };
 
@@ -509,10 +518,12 @@ struct ColRegs {
 // block_any.
 template <typename T>
 constexpr int walk_batch = 4;   // rows per batch of a column walk (two batches in flight)
-template <typename T, int CW, int K>
+// FW = faces per block (64: one level group per wave; 32: two, lanes 0-31
+// and 32-63 holding the same faces at different levels).
+template <typename T, int CW, int K, int FW = CF>
 struct ContShared {
-    static constexpr int L = CW * K;
-    T U[L][CF], S0[L][CF], S1[L][CF], R[9][CF];
+    static constexpr int L = CW * (64 / FW) * K;
+    T U[L][FW], S0[L][FW], S1[L][FW], R[9][FW];
     // per-wave votes of the block-wide "any" tests: 0, 1 alternate over the
     // Newton iterations, 2 = the barotropic-transport test, 3 = the du0 pass test
     int F[4][16];
@@ -526,8 +537,8 @@ struct ContShared {
 // part-way through an unrolled sequence (a switch that falls through) was
 // miscompiled for one entry point by the compiler of this toolchain: a
 // hoisted operand of the step was left undefined on that path.
-template <int L, typename T, typename Step>
-__device__ __forceinline__ void lds_walk(const T (*col)[CF], int nk, int lane, Step step)
+template <int L, typename T, int FW, typename Step>
+__device__ __forceinline__ void lds_walk(const T (*col)[FW], int nk, int lane, Step step)
 {
     constexpr int B = walk_batch<T>;
     int k0 = 0;
@@ -580,8 +591,8 @@ __device__ __forceinline__ bool block_any(bool pred, int *F, int w, int lane)
 }
 
 // Sequential fold acc = op(acc, x_k) over rows 0..nk-1 of one LDS column
-template <int L, typename T, typename Op>
-__device__ __forceinline__ T lds_fold(const T (*col)[CF], int nk, int lane, T acc, Op op)
+template <int L, typename T, int FW, typename Op>
+__device__ __forceinline__ T lds_fold(const T (*col)[FW], int nk, int lane, T acc, Op op)
 {
     lds_walk<L>(col, nk, lane, [&](T x) __attribute__((always_inline)) { acc = op(acc, x); });
     return acc;
@@ -589,21 +600,21 @@ __device__ __forceinline__ T lds_fold(const T (*col)[CF], int nk, int lane, T ac
 
 // Column sum x0 + sum_k col[k] in k order: plain FP64 adds (the reference's
 // summation), or for FP32 terms a Kahan-compensated sum.
-template <int L>
-__device__ __forceinline__ double col_sum(const double (*col)[CF], int nk, int lane, double x0)
+template <int L, int FW>
+__device__ __forceinline__ double col_sum(const double (*col)[FW], int nk, int lane, double x0)
 {
     return lds_fold<L>(col, nk, lane, x0, [](double a_, double x_) { return a_ + x_; });
 }
 
 // the same with each term scaled by w first: x0 + sum_k (w * col[k])
-template <int L>
-__device__ __forceinline__ double col_sum(const double (*col)[CF], int nk, int lane, double x0, double w)
+template <int L, int FW>
+__device__ __forceinline__ double col_sum(const double (*col)[FW], int nk, int lane, double x0, double w)
 {
     return lds_fold<L>(col, nk, lane, x0, [w](double a_, double x_) { return a_ + w * x_; });
 }
 
-template <int L>
-__device__ __forceinline__ float col_sum(const float (*col)[CF], int nk, int lane, float x0, float w = 1.0f,
+template <int L, int FW>
+__device__ __forceinline__ float col_sum(const float (*col)[FW], int nk, int lane, float x0, float w = 1.0f,
                                          bool scaled = false)
 {
     float acc = x0, c = 0.0f;
@@ -623,24 +634,33 @@ __device__ __forceinline__ float col_sum(const float (*col)[CF], int nk, int lan
 // of one face row J (lanes still run along i, so every load stays coalesced);
 // its convergence is a separate kernel (the faces J-1 belong to another block).
 //
+// FW = 32: a block covers 32 faces and each wave two level groups (lanes
+// 0-31 and 32-63), so a block needs half the registers and LDS of FW = 64 and
+// two blocks share a CU: one block's loads, folds and barriers overlap the
+// other's flux passes.  Both halves of a wave run the Newton logic of the same
+// face (the same inputs, so the same decisions); the column folds are done by
+// whole waves, both halves folding the same column (no divergence) and
+// storing the same value.
+//
 // FP32 form: the Newton stopping test of zonal_flux_adjust also accepts a
 // residual at the rounding level of the layer fluxes (4 ulp of sum |uh_k| of
 // the first pass); the FP64 tolerance of 1e-12 m lies below what FP32 sums
 // resolve, and the iteration would otherwise run to its 20-pass limit.
-template <typename T, bool Y, int CW, int K>
+template <typename T, bool Y, int CW, int K, int FW>
 __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monotonic, unsigned bx, unsigned by)
 {
     extern __shared__ double smem_raw[];
     constexpr bool F32 = sizeof(T) == 4;
-    constexpr int L = ContShared<T, CW, K>::L;
-    ContShared<T, CW, K> &S = *reinterpret_cast<ContShared<T, CW, K> *>(smem_raw);
+    constexpr int L = ContShared<T, CW, K, FW>::L;
+    ContShared<T, CW, K, FW> &S = *reinterpret_cast<ContShared<T, CW, K, FW> *>(smem_raw);
     const Dims &d = a.d;
-    const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & (FW - 1);                  // the face of the block this thread works on
+    const int grp = w * (64 / FW) + (int)((threadIdx.x & 63) / FW);   // its level group
     // face (fi, fj) between cell 0 = (fi, fj) and cell 1 = (fi+1, fj) [x] or (fi, fj+1) [y]
-    // x: a block advances by CF-2 faces (its last lane only lends its cell to the
-    // lane below; its first lane's face feeds the convergence of the second)
-    const int f_raw = Y ? d.isc + bx * CF + lane : d.isc - 1 + bx * (CF - 2) + lane;
+    // x: a block advances by FW-2 faces (its last face lane only lends its cell
+    // to the lane below; its first lane's face feeds the convergence of the second)
+    const int f_raw = Y ? d.isc + bx * FW + lane : d.isc - 1 + bx * (FW - 2) + lane;
     const int f_max = d.iec;
     const int fi = f_raw > f_max ? f_max : f_raw;           // clamped: evaluated, never written
     const int fj = Y ? d.jsc - 1 + (int)by : d.jsc + (int)by;
@@ -651,7 +671,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     const int i = fi, j = fj;
     const int nk = d.nk;
     const long long s = d.s2;
-    const int k0 = w * K;                                    // my levels: k0+1 .. k0+K
+    const int k0 = grp * K;                                  // my levels: k0+1 .. k0+K
     auto valid = [&](int kk) __attribute__((always_inline)) { return k0 + kk < nk; };
 
     ColRegs<T, K> col;
@@ -673,6 +693,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         T m5[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) m5[q] = (T)a.mask2dT[ccell(q)];
+        T wW[K], wE[K], w0[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
             const int kr = k0 + kk + 1;
@@ -684,20 +705,20 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             for (int q = 0; q < 5; ++q) h5[q] = a.hin[ccell(q) + o];
 #pragma unroll
             for (int q = 0; q < 3; ++q) sl[q] = ppm_slope_v(m5[q], m5[q + 1], m5[q + 2], h5[q], h5[q + 1], h5[q + 2]);
-            ppm_edges_v(m5, h5, sl, 2, col.wW[kk], col.wE[kk]);
-            col.w0[kk] = h5[2];
+            ppm_edges_v(m5, h5, sl, 2, wW[kk], wE[kk]);
+            w0[kk] = h5[2];
         }
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            ppm_limit(monotonic, col.w0[kk], (T)a.h_min, col.wW[kk], col.wE[kk]);
-            col.eW[kk] = __shfl_down(col.wW[kk], 1, 64);
-            col.eE[kk] = __shfl_down(col.wE[kk], 1, 64);
-            col.e0[kk] = __shfl_down(col.w0[kk], 1, 64);
+            ppm_limit(monotonic, w0[kk], (T)a.h_min, wW[kk], wE[kk]);
+            col.set(kk, wW[kk], wE[kk], w0[kk], __shfl_down(wW[kk], 1, FW), __shfl_down(wE[kk], 1, FW),
+                    __shfl_down(w0[kk], 1, FW));
         }
     } else {
         T m6[6];
 #pragma unroll
         for (int q = 0; q < 6; ++q) m6[q] = (T)a.mask2dT[cell(q)];
+        T wW[K], wE[K], w0[K], eW[K], eE[K], e0[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
             const int kr = k0 + kk + 1;
@@ -709,15 +730,16 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             for (int q = 0; q < 6; ++q) h6[q] = a.hin[cell(q) + o];
 #pragma unroll
             for (int q = 0; q < 4; ++q) sl[q] = ppm_slope_v(m6[q], m6[q + 1], m6[q + 2], h6[q], h6[q + 1], h6[q + 2]);
-            ppm_edges_v(m6, h6, sl, 2, col.wW[kk], col.wE[kk]);
-            ppm_edges_v(m6, h6, sl, 3, col.eW[kk], col.eE[kk]);
-            col.w0[kk] = h6[2];
-            col.e0[kk] = h6[3];
+            ppm_edges_v(m6, h6, sl, 2, wW[kk], wE[kk]);
+            ppm_edges_v(m6, h6, sl, 3, eW[kk], eE[kk]);
+            w0[kk] = h6[2];
+            e0[kk] = h6[3];
         }
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            ppm_limit(monotonic, col.w0[kk], (T)a.h_min, col.wW[kk], col.wE[kk]);
-            ppm_limit(monotonic, col.e0[kk], (T)a.h_min, col.eW[kk], col.eE[kk]);
+            ppm_limit(monotonic, w0[kk], (T)a.h_min, wW[kk], wE[kk]);
+            ppm_limit(monotonic, e0[kk], (T)a.h_min, eW[kk], eE[kk]);
+            col.set(kk, wW[kk], wE[kk], w0[kk], eW[kk], eE[kk], e0[kk]);
         }
     }
 
@@ -805,7 +827,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         T uh_err_best = vabs(uh_err);
         // x: lanes past the last face and the block's last lane (no cell 1 of
         // its own) take no Newton passes
-        bool do_I = Y || (live && lane < CF - 1);
+        bool do_I = Y || (live && lane < FW - 1);
         *du_eval = T(0.0);   // the first pass evaluated at du = 0
         for (int itt = 1; itt <= max_itts; ++itt) {
             if (do_I) {
@@ -869,7 +891,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 
     // outputs at the face: uh, u_cor, du_cor; a face shared with the western
     // neighbour block is written by that block
-    const bool writes_face = live && (Y || ((lane > 0 || bx == 0) && lane < CF - 1));
+    const bool writes_face = live && (Y || ((lane > 0 || bx == 0) && lane < FW - 1));
     if (writes_face) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
@@ -879,18 +901,19 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
                 if (a.u_cor) a.u_cor[o] = uk(kk) + du;
             }
         }
-        if (a.du_cor && w == 0) a.du_cor[c2] = (double)du;
+        if (a.du_cor && grp == 0) a.du_cor[c2] = (double)du;
     }
     // continuity_zonal_convergence (:148-176) for cell i = I (faces I-1 and I)
     if (!Y) {
         const T dtIareaT = (T)(a.dt * a.IareaT[IX2(d, i, j)]);
-        const bool writes_cell = live && lane > 0 && lane < CF - 1 && i >= d.isc;
+        const bool writes_cell = live && lane > 0 && lane < FW - 1 && i >= d.isc;
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
             if (writes_cell && valid(kk)) {
+                const long long c3 = IX3(d, i, j, k0 + kk + 1);
                 const T fw = S.S0[k0 + kk][lane - 1];   // face I-1 (the lane to the west), same level
-                const T v = col.w0[kk] - dtIareaT * (S.S0[k0 + kk][lane] - fw);
-                a.h[IX3(d, i, j, k0 + kk + 1)] = vmax(v, T(0.0));
+                const T v = a.hin[c3] - dtIareaT * (S.S0[k0 + kk][lane] - fw);
+                a.h[c3] = vmax(v, T(0.0));
             }
         }
     }
@@ -950,7 +973,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     eval_pass(duR, true);
     sum_pass(T(0.0), T(0.0), true);
     const T uhtot_R = S.R[0][lane], FAmt_R = S.R[1][lane];
-    if (!writes_face || w != 0) return;
+    if (!writes_face || grp != 0) return;
     T FA_0 = FAmt_0, FA_avg = FAmt_0;
     if ((duL - du0) != T(0.0)) FA_avg = uhtot_L / (duL - du0);
     if (FA_avg > vmax(FA_0, FAmt_L)) FA_avg = vmax(FA_0, FAmt_L);
@@ -972,11 +995,12 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
                                 : (T(1.5) * (duR - du0)) * ((FAmt_R - FA_avg) / (FAmt_R - FA_0)));
 }
 
-template <typename T, bool Y, int CW, int K>
-__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : (CW + 3) / 4, (CW >= 12 || sizeof(T) == 4) ? 4 : 2)))
+template <typename T, bool Y, int CW, int K, int FW = CF>
+__global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(
+    (sizeof(T) == 4 || FW == 32) ? 4 : (CW + 3) / 4, (CW >= 12 || sizeof(T) == 4 || FW == 32) ? 4 : 2)))
 k_continuity_blk(ContArgsT<T> a, int monotonic)
 {
-    continuity_body<T, Y, CW, K>(a, monotonic, blockIdx.x, blockIdx.y);
+    continuity_body<T, Y, CW, K, FW>(a, monotonic, blockIdx.x, blockIdx.y);
 }
 
 // every member of an ensemble in one launch (K for the deepest member)
@@ -987,7 +1011,7 @@ k_continuity_batch(const ContArgsT<T> *__restrict__ args, BatchMap mp)
     int m, local;
     batch_locate(mp, blockIdx.x, m, local);
     const ContArgsT<T> a = args[m];
-    continuity_body<T, Y, CW, K>(a, a.monotonic, local % mp.nbx[m], local / mp.nbx[m]);
+    continuity_body<T, Y, CW, K, CF>(a, a.monotonic, local % mp.nbx[m], local / mp.nbx[m]);
 }
 
 // the driver's `h = h0` reset for the cells the convergence does not write:
@@ -1092,17 +1116,38 @@ __global__ void k_convergence(Dims d, const double *hin, const double *__restric
     }
 }
 
+// FP64 block shape beyond 32 levels: false = 64 faces x 12 waves of 7 levels
+// (1.29 ms at 720x720x75), true = 32 faces x 8 waves x 2 groups of 5 levels,
+// two blocks per CU (1.35 ms: 128 VGPRs with spills, and a block's latency
+// chains -- the column folds, the barriers, the loads -- do not shorten with
+// fewer faces)
+constexpr bool kFP64Narrow = false;
+
+// Blocks in x of a pass with FW faces per block (x: blocks overlap by two faces)
+template <bool Y>
+unsigned blk_nbx(const Dims &d, int FW)
+{
+    return Y ? (unsigned)((d.iec - d.isc + 1 + FW - 1) / FW) : (unsigned)((d.iec - d.isc + 1 + FW - 3) / (FW - 2));
+}
+
+// One pass over nby rows of faces; the block shape follows nk and the type
 template <typename T, bool Y>
-int launch_blk(const ContArgsT<T> &a, int mono, dim3 g, hipStream_t s)
+int launch_blk(const ContArgsT<T> &a, int mono, unsigned nby, hipStream_t s)
 {
     const int nk = a.d.nk;
-    const dim3 b(64 * 8);
+    const dim3 b(64 * 8), g(blk_nbx<Y>(a.d, CF), nby);
     if (nk <= 8) k_continuity_blk<T, Y, 8, 1><<<g, b, sizeof(ContShared<T, 8, 1>), s>>>(a, mono);
     else if (nk <= 16) k_continuity_blk<T, Y, 8, 2><<<g, b, sizeof(ContShared<T, 8, 2>), s>>>(a, mono);
     else if (nk <= 32) k_continuity_blk<T, Y, 8, 4><<<g, b, sizeof(ContShared<T, 8, 4>), s>>>(a, mono);
-    // FP64 beyond 32 levels: 12 waves of 7 levels (three waves per SIMD; the
-    // FP32 form keeps 8 x 10, two blocks per CU)
-    else if (sizeof(T) == 8) k_continuity_blk<T, Y, 12, 7><<<g, dim3(64 * 12), sizeof(ContShared<T, 12, 7>), s>>>(a, mono);
+    // FP64 beyond 32 levels: 12 waves of 7 levels (three waves per SIMD); the
+    // FP32 form: 8 waves of 10 levels (two blocks per CU)
+    else if (sizeof(T) == 8 && !kFP64Narrow)
+        k_continuity_blk<T, Y, 12, 7><<<g, dim3(64 * 12), sizeof(ContShared<T, 12, 7>), s>>>(a, mono);
+    // the alternative: blocks of 32 faces, 8 waves x 2 level groups of 5
+    // levels (64 KB of LDS, two blocks per CU)
+    else if (sizeof(T) == 8)
+        k_continuity_blk<T, Y, 8, 5, 32><<<dim3(blk_nbx<Y>(a.d, 32), nby), b, sizeof(ContShared<T, 8, 5, 32>), s>>>(
+            a, mono);
     else k_continuity_blk<T, Y, 8, 10><<<g, b, sizeof(ContShared<T, 8, 10>), s>>>(a, mono);
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
@@ -1128,6 +1173,14 @@ int set_blk_lds(long long max_lds)
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
         MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, true, 12, 7>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
+    }
+    if ((int)sizeof(ContShared<T, 8, 5, 32>) <= max_lds) {
+        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, false, 8, 5, 32>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)sizeof(ContShared<T, 8, 5, 32>)));
+        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, true, 8, 5, 32>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)sizeof(ContShared<T, 8, 5, 32>)));
     }
     for (int q = 0; q < 8; ++q)
         if (bytes[q % 4] <= max_lds) {
@@ -1165,8 +1218,7 @@ int continuity_blk_x(mom6h_continuity_cs *cs, const T *u, const T *hin, T *h, T 
                    bt ? bt->FA_u_EE : nullptr, bt ? bt->FA_u_E0 : nullptr, bt ? bt->FA_u_W0 : nullptr,
                    bt ? bt->FA_u_WW : nullptr, bt ? bt->uBT_WW : nullptr, bt ? bt->uBT_EE : nullptr,
                    bt ? bt_h_u : nullptr};
-    const dim3 g((d.iec - d.isc + 1 + CF - 3) / (CF - 2), d.jec - d.jsc + 1);
-    int rc = launch_blk<T, false>(a, cs->monotonic, g, s);
+    int rc = launch_blk<T, false>(a, cs->monotonic, (unsigned)(d.jec - d.jsc + 1), s);
     if (rc) return rc;
     if (hin_halo_to_h) {
         launch_halo_copy(d, hin, h, s);
@@ -1186,8 +1238,7 @@ int continuity_blk_y(mom6h_continuity_cs *cs, const T *v, const T *hin, T *h, T 
                    bt ? bt->FA_u_EE : nullptr, bt ? bt->FA_u_E0 : nullptr, bt ? bt->FA_u_W0 : nullptr,
                    bt ? bt->FA_u_WW : nullptr, bt ? bt->uBT_WW : nullptr, bt ? bt->uBT_EE : nullptr,
                    bt ? bt_h_v : nullptr};
-    const dim3 g((d.iec - d.isc + 1 + CF - 1) / CF, d.jec - d.jsc + 2);
-    int rc = launch_blk<T, true>(a, cs->monotonic, g, s);
+    int rc = launch_blk<T, true>(a, cs->monotonic, (unsigned)(d.jec - d.jsc + 2), s);
     if (rc) return rc;
     {
         dim3 b(128), gc((d.iec - d.isc + 1 + 127) / 128, d.jec - d.jsc + 1, d.nk);
@@ -1213,7 +1264,7 @@ int continuity_batch(const ContArgsT<T> *d_args, const ContArgsT<T> *h_args, int
     int nk_max = 1;
     for (int m = 0; m < n; ++m) {
         const Dims &d = h_args[m].d;
-        mp.nbx[m] = y_pass ? (d.iec - d.isc + 1 + CF - 1) / CF : (d.iec - d.isc + 1 + CF - 3) / (CF - 2);
+        mp.nbx[m] = (int)(y_pass ? blk_nbx<true>(d, CF) : blk_nbx<false>(d, CF));
         mp.nby[m] = y_pass ? d.jec - d.jsc + 2 : d.jec - d.jsc + 1;
         mp.first[m + 1] = mp.first[m] + mp.nbx[m] * mp.nby[m];
         nk_max = d.nk > nk_max ? d.nk : nk_max;
