@@ -60,6 +60,12 @@ _SIGS = {
     "mom6h_btstep": ([P] + [P] * 6 + [P], C.c_int),
     "mom6h_btstep_noalloc": ([P] + [P] * 6 + [P], C.c_int),
     "mom6h_barotropic_end": ([P], C.c_int),
+    "mom6h_model_diagnostics": ([P, D], C.c_int),
+    "mom6h_diag_stats_work_doubles": ([], C.c_long),
+    "mom6h_diag_product_sum": ([C.c_int] * 9 + [P, P, C.c_int, P, P], C.c_int),
+    "mom6h_diag_ke": ([C.c_int] * 9 + [P, P, P, P], C.c_int),
+    "mom6h_diag_mass": ([C.c_int] * 9 + [P, P, P], C.c_int),
+    "mom6h_diag_stats": ([C.c_int] * 9 + [P, P, P, P], C.c_int),
     "mom6h_model_create": ([C.POINTER(P), C.POINTER(MConfig)], C.c_int),
     "mom6h_model_field": ([P, C.c_char_p, C.POINTER(P), C.POINTER(C.c_long)], C.c_int),
     "mom6h_model_field_bytes": ([P, C.c_char_p, C.POINTER(C.c_int)], C.c_int),

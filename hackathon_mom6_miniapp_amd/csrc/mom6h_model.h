@@ -75,6 +75,12 @@ struct mom6h_model {
     hipGraph_t graph_bt = nullptr;       // one barotropic step (stream s1)
     hipGraphExec_t gexec_bt = nullptr;
     int graph_fused = -1;    // fused_ok() at capture time
+    // diagnostics (option "diag"): the predictor's vertical sums of diffu*h and
+    // diffv*h in one shared 2D work array (the reference's work_2d), KE and
+    // mass fields, the reduction workspace and 16 statistics on the device
+    int diag = 0;
+    double *diag_work2d = nullptr, *diag_ke = nullptr, *diag_mass = nullptr, *diag_red = nullptr;
+    double *diag_out = nullptr;
     // profiling
     std::vector<hipEvent_t> ev;
     double stage_ms[mom6h::ST_COUNT] = {0};

@@ -250,6 +250,26 @@ int enqueue_step_f32(mom6h_model *m)
     return MOM6H_OK;
 }
 
+// The predictor's diagnostics (rk2_serial_driver.F90:300-308): the vertical
+// sums of diffu*h and diffv*h (h = the predicted thickness) posted in turn
+// through the one shared work array, each followed by its statistics
+// (diag_out[4..7] and [8..11]).  The fused stencil keeps diffu on chip: the
+// predictor's viscous tendencies are formed here by the module kernel.
+int post_predictor_diagnostics(mom6h_model *m)
+{
+    const Dims &d = m->d;
+    hipStream_t s0 = m->s0;
+    if (fused_ok(m)) RC(mom6h_hor_visc(m->hv, m->u, m->v, m->h0, m->diffu, m->diffv, s0));
+    RC(mom6h_diag_product_sum(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, d.nk, m->diffu, m->htmp, 0,
+                              m->diag_work2d, s0));
+    RC(mom6h_diag_stats(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, 1, m->diag_work2d, m->diag_red,
+                        m->diag_out + 4, s0));
+    RC(mom6h_diag_product_sum(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, d.nk, m->diffv, m->htmp, 1,
+                              m->diag_work2d, s0));
+    return mom6h_diag_stats(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, 1, m->diag_work2d, m->diag_red,
+                            m->diag_out + 8, s0);
+}
+
 // The 3D part of one RK2 step (rk2_serial_driver.F90:218-399) on stream s0.
 int enqueue_step(mom6h_model *m)
 {
@@ -265,6 +285,7 @@ int enqueue_step(mom6h_model *m)
       RC(mom6h_vert_visc(m->vv, m->up, m->vp, m->h0, dt, m->taux, m->tauy, rayu, rayv, s0)); }
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->up, nk}, {m->vp, nk}})); }
     RC(continuity_phase(m, m->up, m->vp, m->h0, m->htmp, dt));
+    if (m->diag) RC(post_predictor_diagnostics(m));
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->htmp, nk}})); }
 
     // ---- corrector (h = h_pred, held in htmp)
@@ -730,7 +751,42 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
     else if (k == "profile") {
         m->profile = value != 0.0;
         for (double &x : m->stage_ms) x = 0.0;
+    }
+    else if (k == "diag") {
+        MOM6H_REQUIRE(!m->mixed || value == 0.0, "mom6h_model_set_option: diagnostics need the FP64 form");
+        if (value != 0.0 && !m->diag_out) {
+            const size_t n2 = (size_t)m->d.s2, n3 = n2 * m->c.nk;
+            RC(dev_alloc(&m->diag_work2d, n2, true));   // the reference's work_2d starts at zero
+            RC(dev_alloc(&m->diag_ke, n3, true));
+            RC(dev_alloc(&m->diag_mass, n2, true));
+            RC(dev_alloc(&m->diag_red, (size_t)mom6h_diag_stats_work_doubles(), true));
+            RC(dev_alloc(&m->diag_out, 16, true));
+            m->fields.push_back(Field{"diag_work2d", (void **)&m->diag_work2d, n2, 8, false});
+            m->fields.push_back(Field{"diag_ke", (void **)&m->diag_ke, n3, 8, false});
+            m->fields.push_back(Field{"diag_mass", (void **)&m->diag_mass, n2, 8, false});
+        }
+        m->diag = value != 0.0;
+        drop_graphs(m);   // the captured step holds the posts or not
     } else return set_error(MOM6H_EINVAL, "mom6h_model_set_option: unknown option '%s'", key);
+    return MOM6H_OK;
+}
+
+int mom6h_model_diagnostics(mom6h_model *m, double *out)
+{
+    MOM6H_REQUIRE(m && out, "mom6h_model_diagnostics: NULL argument");
+    MOM6H_REQUIRE(m->diag && m->prepared, "mom6h_model_diagnostics: set option diag and prepare first");
+    const Dims &d = m->d;
+    hipStream_t s0 = m->s0;
+    MOM6H_CHECK(hipStreamSynchronize(m->s1));
+    // end-of-run diagnostics (rk2_serial_driver.F90:386-394): KE, then mass
+    RC(mom6h_diag_ke(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, d.nk, m->u, m->v, m->diag_ke, s0));
+    RC(mom6h_diag_stats(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, d.nk, m->diag_ke, m->diag_red,
+                        m->diag_out, s0));
+    RC(mom6h_diag_mass(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, d.nk, m->h, m->diag_mass, s0));
+    RC(mom6h_diag_stats(d.isd, d.ied, d.jsd, d.jed, d.isc, d.iec, d.jsc, d.jec, 1, m->diag_mass, m->diag_red,
+                        m->diag_out + 12, s0));
+    MOM6H_CHECK(hipMemcpyAsync(out, m->diag_out, 16 * sizeof(double), hipMemcpyDeviceToHost, s0));
+    MOM6H_CHECK(hipStreamSynchronize(s0));
     return MOM6H_OK;
 }
 
@@ -823,6 +879,7 @@ int mom6h_model_destroy(mom6h_model *m)
         if (f.owned && *f.p) (void)hipFree(*f.p);
     mom6h_bt_cont_free(&m->bt);
     mom6h_bt_cont_free(&m->bt_v);
+    for (double *p : {m->diag_work2d, m->diag_ke, m->diag_mass, m->diag_red, m->diag_out}) dev_free(p);
     drop_graphs(m);
     if (m->e_start) (void)hipEventDestroy(m->e_start);
     if (m->e_bt) (void)hipEventDestroy(m->e_bt);

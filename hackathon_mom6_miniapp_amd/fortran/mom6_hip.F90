@@ -209,6 +209,34 @@ module mom6_hip_capi
             real(c_double), intent(out) :: cfl, cfl_2d
             integer(c_int), intent(out) :: min_nsteps, status
         end function
+        ! diagnostics (src/common/mom6_diag.F90, app/rk2_serial_driver.F90:664-723)
+        integer(c_int) function mom6h_diag_product_sum(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, a, b, &
+                position, work_2d, stream) bind(c, name='mom6h_diag_product_sum')
+            import :: c_int, c_ptr
+            integer(c_int), value :: isd, ied, jsd, jed, isc, iec, jsc, jec, nk, position
+            type(c_ptr), value :: a, b, work_2d, stream
+        end function
+        integer(c_int) function mom6h_diag_ke(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, u, v, ke, stream) &
+                bind(c, name='mom6h_diag_ke')
+            import :: c_int, c_ptr
+            integer(c_int), value :: isd, ied, jsd, jed, isc, iec, jsc, jec, nk
+            type(c_ptr), value :: u, v, ke, stream
+        end function
+        integer(c_int) function mom6h_diag_mass(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, h, mass, stream) &
+                bind(c, name='mom6h_diag_mass')
+            import :: c_int, c_ptr
+            integer(c_int), value :: isd, ied, jsd, jed, isc, iec, jsc, jec, nk
+            type(c_ptr), value :: h, mass, stream
+        end function
+        integer(c_long) function mom6h_diag_stats_work_doubles() bind(c, name='mom6h_diag_stats_work_doubles')
+            import :: c_long
+        end function
+        integer(c_int) function mom6h_diag_stats(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, field, work, out, &
+                stream) bind(c, name='mom6h_diag_stats')
+            import :: c_int, c_ptr
+            integer(c_int), value :: isd, ied, jsd, jed, isc, iec, jsc, jec, nk
+            type(c_ptr), value :: field, work, out, stream
+        end function
     end interface
 end module mom6_hip_capi
 

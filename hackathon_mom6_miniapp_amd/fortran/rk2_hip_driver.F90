@@ -46,6 +46,8 @@ program rk2_hip_driver
     ! device arrays
     type(c_ptr) :: d_u, d_v, d_h, d_h0, d_htmp, d_uh, d_vh, d_CAu, d_CAv, d_up, d_vp, d_diffu, d_diffv
     type(c_ptr) :: d_eta, d_ubt, d_vbt, d_ubt_av, d_vbt_av, d_eta_av, d_taux, d_tauy, d_ray, d_dyCu, d_dxCv
+    ! diagnostics: the shared 2D work array, KE, mass, reduction workspace, 4 statistics
+    type(c_ptr) :: d_work2d, d_ke, d_mass, d_red, d_stats
 
     type(continuity_CS_hip) :: cont_CS
     type(BT_cont_type_hip), target :: BT_cont
@@ -82,6 +84,11 @@ program rk2_hip_driver
     print '(A,I5,A,I5,A,I4)', 'Grid: ', ni, ' x ', nj, ' x ', nk
     print '(A,I4)', 'RK2 iterations: ', niter
     print '(A,I4)', 'Barotropic substeps: ', bt_nsteps
+    if (diag_mode > 0) then
+        print '(A)', 'Diagnostics: ENABLED'
+    else
+        print '(A)', 'Diagnostics: disabled'
+    end if
 
     ! grid bounds (init_ocean_grid, src/common/mom6_types.F90:145-148)
     isd = 1; ied = ni + 2; jsd = 1; jed = nj + 2
@@ -177,6 +184,10 @@ program rk2_hip_driver
     d_ubt_av = alloc_zero_2d(n1, n2); d_vbt_av = alloc_zero_2d(n1, n2); d_eta_av = alloc_zero_2d(n1, n2)
     d_taux = alloc_copy_2d(taux, n1, n2); d_tauy = alloc_copy_2d(tauy, n1, n2); d_ray = alloc_copy_2d(ray, n1, n2)
     d_dyCu = alloc_copy_2d(dxv, n1, n2); d_dxCv = alloc_copy_2d(dxv, n1, n2)
+    if (diag_mode > 0) then   ! diag_init (mom6_diag.F90:76-114): work_2d starts at zero
+        d_work2d = alloc_zero_2d(n1, n2); d_ke = alloc_zero_3d(n1, n2, nk); d_mass = alloc_zero_2d(n1, n2)
+        d_red = alloc_zero_2d(int(mom6h_diag_stats_work_doubles()), 1); d_stats = alloc_zero_2d(4, 1)
+    end if
     call hip_check(mom6h_device_synchronize(), 'init')
 
     t_total = 0.0_dp
@@ -193,6 +204,14 @@ program rk2_hip_driver
         call vert_visc_cra_hip(d_up, d_vp, d_h0, dt, visc_CS, d_taux, d_tauy, d_ray, d_ray)
         call btstep_hip(d_eta, d_ubt, d_vbt, d_ubt_av, d_vbt_av, d_eta_av, bt_CS)
         call continuity_PPM_hip(d_up, d_h0, d_h, d_uh, dt, cont_CS, BT_cont=BT_cont, hin_halo_to_h=.true.)
+        if (diag_mode > 0) then   ! predictor diagnostics (:300-308)
+            call hip_check(mom6h_diag_product_sum(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, d_diffu, d_h, 0, &
+                                                  d_work2d, c_null_ptr), 'diag u')
+            call post_stats('diffu_sum', d_work2d, 1)
+            call hip_check(mom6h_diag_product_sum(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, d_diffv, d_h, 1, &
+                                                  d_work2d, c_null_ptr), 'diag v')
+            call post_stats('diffv_sum', d_work2d, 1)
+        end if
         ! ---- corrector (:308-392)
         call hip_check(mom6h_transports(isd, ied, jsd, jed, nk, d_up, d_vp, d_h, d_dyCu, d_dxCv, d_uh, d_vh, &
                                         c_null_ptr), 'transports')
@@ -204,6 +223,14 @@ program rk2_hip_driver
         call btstep_hip(d_eta_av, d_ubt_av, d_vbt_av, d_ubt_av, d_vbt_av, d_eta, bt_CS)
         call copy_d2d(d_htmp, d_h, n3v)
         call continuity_PPM_hip(d_u, d_htmp, d_h, d_uh, 0.5_dp * dt, cont_CS, BT_cont=BT_cont)
+        if (diag_mode > 0 .and. iter == niter) then   ! end-of-run diagnostics (:386-394)
+            call hip_check(mom6h_diag_ke(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, d_u, d_v, d_ke, c_null_ptr), &
+                           'diag KE')
+            call post_stats('KE', d_ke, nk)
+            call hip_check(mom6h_diag_mass(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, d_h, d_mass, c_null_ptr), &
+                           'diag mass')
+            call post_stats('mass', d_mass, 1)
+        end if
         call hip_check(mom6h_device_synchronize(), 'step')
         call system_clock(c1)
         t_iter = real(c1 - c0, dp) / real(rate, dp)
@@ -264,6 +291,20 @@ program rk2_hip_driver
     call hip_free(d_taux); call hip_free(d_tauy); call hip_free(d_ray); call hip_free(d_dyCu); call hip_free(d_dxCv)
 
 contains
+
+    ! post_data_2d/3d with output mode DIAG_STATS (src/common/mom6_diag.F90:166-274):
+    ! the statistics of a device field over the compute domain, printed as the reference prints them
+    subroutine post_stats(name, field, nlev)
+        character(len=*), intent(in) :: name
+        type(c_ptr), intent(in) :: field
+        integer, intent(in) :: nlev
+        real(dp), target :: st(4)
+        call hip_check(mom6h_diag_stats(isd, ied, jsd, jed, isc, iec, jsc, jec, nlev, field, d_red, d_stats, &
+                                        c_null_ptr), 'diag stats')
+        call hip_check(mom6h_device_synchronize(), 'diag stats')
+        call hip_check(mom6h_memcpy_d2h(c_loc(st), d_stats, 32_c_size_t), 'diag stats')
+        print '(A,A,A,4ES14.6)', '  Diag: ', trim(name), ' min/max/mean/rms: ', st
+    end subroutine
 
     !> The reference driver's barotropic CFL report (rk2_serial_driver.F90:625-662),
     !> numbers from mom6h_bt_cfl

@@ -155,6 +155,18 @@ class Model:
         check(self.L.mom6h_model_autotune(self.m, reps, buf, 512), "mom6h_model_autotune")
         return buf.value.decode()
 
+    def diagnostics(self) -> dict:
+        """The serial driver's diagnostics (option "diag" must be on): KE and
+        mass of the current state, diffu_sum and diffv_sum of the last
+        predictor; each (min, max, mean, rms)."""
+        out = (C.c_double * 16)()
+        check(self.L.mom6h_model_diagnostics(self.m, out), "mom6h_model_diagnostics")
+        return {name: tuple(out[4 * q:4 * q + 4]) for q, name in enumerate(DIAG_NAMES)}
+
+
+# order of mom6h_model_diagnostics' 16 values
+DIAG_NAMES = ("KE", "diffu_sum", "diffv_sum", "mass")
+
 
 class Ensemble:
     """Members of different sizes advanced together: one launch per stage over

@@ -225,6 +225,32 @@ int mom6h_bt_cfl(double dt, int nsteps, double dx_km, double depth, double *cfl,
 int mom6h_seeded_normal(double *d_out, long count, unsigned long long seed, int field, int add_mean,
                         double mean, double sigma, const double *d_mask2d, long n2, void *stream);
 
+/* ----------------------------------------------------------- diagnostics --
+ * The serial driver's diagnostics (diag_mode = 1) on the device:
+ * post_product_sum_u/v (src/common/mom6_diag.F90:276-340), the KE and mass
+ * fields of compute_and_post_KE/_mass (app/rk2_serial_driver.F90:664-723) and
+ * the min/max/mean/rms statistics of post_data_2d/3d (mom6_diag.F90:166-274),
+ * which the caller prints or writes (the reference's binary dump layout is
+ * written on the host).  Per-point fields are bit-identical to the serial
+ * loops; mean and rms come from a deterministic double-double reduction (the
+ * serial sum differs by its own rounding). */
+/* work_2d = sum_k a*b on the u range (position 0: isc..iec-1 x jsc..jec) or
+ * the v range (1: isc..iec x jsc..jec-1); other points are left as they are */
+int mom6h_diag_product_sum(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk,
+                           const double *a, const double *b, int position, double *work_2d, void *stream);
+/* KE at the compute h points (3D) */
+int mom6h_diag_ke(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk, const double *u,
+                  const double *v, double *ke, void *stream);
+/* mass = column sum of h on the compute points, zero elsewhere (2D) */
+int mom6h_diag_mass(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk, const double *h,
+                    double *mass, void *stream);
+/* doubles of device workspace mom6h_diag_stats needs */
+long mom6h_diag_stats_work_doubles(void);
+/* d_out[0..3] = min, max, mean, rms of field over the compute domain and
+ * nk layers (nk = 1: a 2D field) */
+int mom6h_diag_stats(int isd, int ied, int jsd, int jed, int isc, int iec, int jsc, int jec, int nk,
+                     const double *field, double *d_work, double *d_out, void *stream);
+
 /* ------------------------------------------------------------ the model --
  * A whole run of app/rk2_serial_driver.F90 on one GPU: grid, control
  * structures and state on the device; mom6h_model_step advances niter RK2
@@ -293,6 +319,12 @@ int mom6h_model_main_stream(mom6h_model *m, void **stream);
  * layer splits 1, 2, 4, 8 -- keeps the fastest as options "momentum_tile" and
  * "momentum_splits" and writes the timings into report. */
 int mom6h_model_autotune(mom6h_model *m, int reps, char *report, int nbytes);
+/* With option "diag" = 1 (FP64 form) every step posts the predictor's
+ * diffu_sum and diffv_sum statistics; this call adds KE and mass of the
+ * current state and returns the 16 values KE, diffu_sum, diffv_sum, mass
+ * (min, max, mean, rms each): the serial driver's Diag lines
+ * (rk2_serial_driver.F90:152-170, 300-308, 386-394).  Synchronous. */
+int mom6h_model_diagnostics(mom6h_model *m, double *out16);
 int mom6h_model_destroy(mom6h_model *m);
 
 /* ---------------------------------------------------------- the ensemble --

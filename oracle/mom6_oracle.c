@@ -37,6 +37,8 @@ typedef struct named { const char *name; double **p; long n; } named;
 
 struct om_model {
     om_config c;
+    double *diag_work2d, *diag_ke, *diag_mass;   /* diagnostics: work_2d, KE, mass */
+    double diag_posted[8];                       /* the last diffu_sum, diffv_sum statistics */
     int ni, nj, nk, n1, n2;
     int isd, ied, jsd, jed, isc, iec, jsc, jec;
     size_t N2, N3;
@@ -1505,6 +1507,64 @@ static void meridional_pass(om_model *m, const double *v, double dt)
     free(hx);
 }
 
+/* ------------------------------------------------------------ diagnostics */
+/* post_data_2d / post_data_3d statistics (mom6_diag.F90:166-274): min, max,
+ * mean, rms over the compute domain, summed in (k, j, i) order */
+static void diag_stats(om_model *m, const double *f, int nk, double *out)
+{
+    double fmin = 1.79769313486231571e308, fmax = -1.79769313486231571e308, fmean = 0.0, frms = 0.0;
+    long cnt = 0;
+    for (int k = 1; k <= nk; ++k)
+        for (int j = m->jsc; j <= m->jec; ++j)
+            for (int i = m->isc; i <= m->iec; ++i) {
+                const double x = A3(f, i, j, k);
+                fmin = dmin(fmin, x);
+                fmax = dmax(fmax, x);
+                fmean = fmean + x;
+                frms = frms + x * x;
+                cnt = cnt + 1;
+            }
+    if (cnt > 0) {
+        fmean = fmean / (double)cnt;
+        frms = sqrt(frms / (double)cnt);
+    }
+    out[0] = fmin; out[1] = fmax; out[2] = fmean; out[3] = frms;
+}
+
+/* post_product_sum_u (mom6_diag.F90:276-308, u range) and post_product_sum_v
+ * (:311-340, v range) into the one shared work array, then its statistics */
+static void diag_product_sum(om_model *m, const double *a, const double *b, int v_range, double *out)
+{
+    const int ie = v_range ? m->iec : m->iec - 1, je = v_range ? m->jec - 1 : m->jec;
+    for (int j = m->jsc; j <= je; ++j)
+        for (int i = m->isc; i <= ie; ++i) A2(m->diag_work2d, i, j) = 0.0;
+    for (int k = 1; k <= m->nk; ++k)
+        for (int j = m->jsc; j <= je; ++j)
+            for (int i = m->isc; i <= ie; ++i)
+                A2(m->diag_work2d, i, j) = A2(m->diag_work2d, i, j) + A3(a, i, j, k) * A3(b, i, j, k);
+    diag_stats(m, m->diag_work2d, 1, out);
+}
+
+void om_diag(om_model *m, double *out)
+{
+    /* compute_and_post_KE (rk2_serial_driver.F90:664-693) */
+    for (int k = 1; k <= m->nk; ++k)
+        for (int j = m->jsc; j <= m->jec; ++j)
+            for (int i = m->isc; i <= m->iec; ++i) {
+                const double u0 = A3(m->u, i, j, k), u1 = A3(m->u, i - 1, j, k);
+                const double v0 = A3(m->v, i, j, k), v1 = A3(m->v, i, j - 1, k);
+                A3(m->diag_ke, i, j, k) = 0.5 * (0.5 * (u0 * u0 + u1 * u1) + 0.5 * (v0 * v0 + v1 * v1));
+            }
+    diag_stats(m, m->diag_ke, m->nk, out);
+    memcpy(out + 4, m->diag_posted, 8 * sizeof(double));
+    /* compute_and_post_mass (:695-723) */
+    for (size_t q = 0; q < m->N2; ++q) m->diag_mass[q] = 0.0;
+    for (int k = 1; k <= m->nk; ++k)
+        for (int j = m->jsc; j <= m->jec; ++j)
+            for (int i = m->isc; i <= m->iec; ++i) A2(m->diag_mass, i, j) = A2(m->diag_mass, i, j) + A3(m->h, i, j, k);
+    diag_stats(m, m->diag_mass, 1, out + 12);
+}
+
 void om_step(om_model *m, int niter)
 {
     const double dt = m->c.dt;
@@ -1522,6 +1582,10 @@ void om_step(om_model *m, int niter)
         if (per) { halo_fill(m, m->up, nk); halo_fill(m, m->vp, nk); }
         om_continuity_ppm(m, m->up, m->h0, m->h, m->uh, dt);
         if (m->c.meridional) meridional_pass(m, m->vp, dt);
+        if (m->c.diag) {   /* the predictor's posts (rk2_serial_driver.F90:300-308) */
+            diag_product_sum(m, m->diffu, m->h, 0, m->diag_posted);
+            diag_product_sum(m, m->diffv, m->h, 1, m->diag_posted + 4);
+        }
         if (per) halo_fill(m, m->h, nk);
         /* corrector */
         om_compute_transports(m, m->up, m->vp, m->h, m->uh, m->vh);
@@ -1589,6 +1653,7 @@ om_model *om_create(const om_config *c)
     reg(m, "por_face_areaU", &m->por_face_areaU, N3, 1.0);
     reg(m, "visc_rem_u", &m->visc_rem_u, N3, 1.0);
     R2D(taux); R2D(tauy); R3D(Ray_u); R3D(Ray_v);
+    R2D(diag_work2d); R3D(diag_ke); R2D(diag_mass);
     R2D(FA_u_EE); R2D(FA_u_E0); R2D(FA_u_W0); R2D(FA_u_WW); R2D(uBT_WW); R2D(uBT_EE);
     reg(m, "BT_h_u", &m->BT_h_u, N3, 0.0);
     R3D(v_cor); R2D(dv_cor);

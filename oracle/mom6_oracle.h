@@ -54,6 +54,7 @@ typedef struct om_config {
     int simple_2nd, upwind_1st;
     int meridional;          /* continuity also takes a meridional pass after the zonal one
                                 (no reference counterpart; see om_continuity_ppm_y) */
+    int diag;                /* the driver's diag_mode: post the predictor's diffu/diffv sums */
 } om_config;
 
 void om_config_default(om_config *c);
@@ -96,6 +97,12 @@ void om_halo_fill(om_model *m, double *a, int nk);
 /* verify_state (rk2_serial_driver.F90:563-623): out[7] = mass_init, mass_final,
  * rel_error, max_u, max_v, max_eta, ke_total. */
 void om_verify(om_model *m, double *out7);
+/* The driver's diagnostics with diag_mode = 1 (rk2_serial_driver.F90:152-170,
+ * 300-308, 386-394; src/common/mom6_diag.F90): out[16] = min, max, mean, rms of
+ * KE (of the current state), diffu_sum, diffv_sum (the last predictor's
+ * posts) and mass (of the current state).  The fields are kept as
+ * "diag_ke", "diag_work2d", "diag_mass". */
+void om_diag(om_model *m, double *out16);
 
 #ifdef __cplusplus
 }

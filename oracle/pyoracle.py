@@ -30,7 +30,8 @@ class OmConfig(C.Structure):
                 ("Kv", C.c_double), ("Kv_ml", C.c_double), ("Kv_extra_bbl", C.c_double),
                 ("Hmix", C.c_double), ("Hbbl", C.c_double), ("ray_bottom", C.c_double),
                 ("coriolis_scheme", C.c_int), ("monotonic", C.c_int),
-                ("simple_2nd", C.c_int), ("upwind_1st", C.c_int), ("meridional", C.c_int)]
+                ("simple_2nd", C.c_int), ("upwind_1st", C.c_int), ("meridional", C.c_int),
+                ("diag", C.c_int)]
 
 
 _lib = None
@@ -48,6 +49,7 @@ def lib():
         _lib.om_prepare.argtypes = [C.c_void_p]
         _lib.om_step.argtypes = [C.c_void_p, C.c_int]
         _lib.om_verify.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        _lib.om_diag.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         _lib.om_config_default.argtypes = [C.POINTER(OmConfig)]
         d = C.POINTER(C.c_double)
         _lib.om_compute_transports.argtypes = [C.c_void_p, d, d, d, d, d]
@@ -69,7 +71,7 @@ def _ptr(a: np.ndarray):
 class Oracle:
     """One oracle model: grid + control structures + driver arrays."""
 
-    def __init__(self, cfg, case: dict | None = None):
+    def __init__(self, cfg, case: dict | None = None, diag: bool = False):
         L = lib()
         c = OmConfig()
         L.om_config_default(C.byref(c))
@@ -80,6 +82,7 @@ class Oracle:
         c.biharmonic = int(cfg.biharmonic)
         c.monotonic = int(cfg.monotonic)
         c.meridional = int(getattr(cfg, "meridional", False))
+        c.diag = int(diag)
         c.coriolis_mode = 0 if cfg.coriolis == "reference" else 1
         c.f0 = cfg.f0
         c.beta = cfg.beta if cfg.coriolis == "betaplane" else 0.0
@@ -124,6 +127,13 @@ class Oracle:
         out = (C.c_double * 7)()
         lib().om_verify(self.m, out)
         return list(out)
+
+    def diagnostics(self) -> dict:
+        """om_diag: KE, diffu_sum, diffv_sum, mass as (min, max, mean, rms)."""
+        out = (C.c_double * 16)()
+        lib().om_diag(self.m, out)
+        names = ("KE", "diffu_sum", "diffv_sum", "mass")
+        return {n: tuple(out[4 * q:4 * q + 4]) for q, n in enumerate(names)}
 
     def halo_fill(self, a: np.ndarray):
         lib().om_halo_fill(self.m, _ptr(a), a.shape[2] if a.ndim == 3 else 1)
