@@ -57,6 +57,12 @@ class Config:
     Ah: float = 1.0e10
     laplacian: bool = True
     biharmonic: bool = False
+    # hor_visc_init's other closures (mom6_hor_visc_serial.F90:160-193):
+    # Smagorinsky and Leith viscosities, no-slip walls, thickness-aware bounds
+    smagorinsky: bool = False
+    leith: bool = False
+    no_slip: bool = False
+    better_bound: bool = False
     Kv: float = 1.0e-4
     Kv_ml: float = 1.0e-2
     Kv_extra_bbl: float = 1.0e-2

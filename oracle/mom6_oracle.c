@@ -92,6 +92,10 @@ struct om_model {
     double *hv_dudx, *hv_dvdy, *hv_dvdx, *hv_dudy, *hv_sh_xx, *hv_sh_xy, *hv_str_xx, *hv_str_xy;
     double *hv_bhstr_xx, *hv_bhstr_xy, *hv_h_u, *hv_h_v, *hv_hq, *hv_Kh, *hv_Ah, *hv_Shear_mag;
     double *hv_Del2u, *hv_Del2v, *hv_hrat_min, *hv_visc_bound_rem;
+    int hv_Leith_Kh, hv_Leith_Ah;
+    double *hv_Laplac3_const_xx, *hv_Laplac3_const_xy, *hv_Biharm6_const_xx, *hv_Biharm6_const_xy;
+    double *hv_vort_xy, *hv_vort_xy_dx, *hv_vort_xy_dy, *hv_grad_vort_mag_h, *hv_grad_vort_mag_q;
+    double *hv_vert_vort_mag, *hv_Del2vort_q;
 
     /* meridional pass: the transposed model it runs the zonal pass on, and the
      * meridional outputs (v_cor, dv_cor, BT_cont v-side) */
@@ -1204,6 +1208,8 @@ static void hor_visc_init(om_model *m)
     m->hv_Smag_Kh = c->smagorinsky && c->laplacian;
     m->hv_Smag_Ah = c->smagorinsky && c->biharmonic;
     m->hv_no_slip = c->no_slip;
+    m->hv_Leith_Kh = c->leith;                        /* :184-187 */
+    m->hv_Leith_Ah = c->leith && c->biharmonic;
     m->hv_bound_Kh = c->bound_Kh || c->better_bound;
     m->hv_bound_Ah = c->better_bound;
     m->hv_better_bound_Kh = c->better_bound;
@@ -1211,7 +1217,7 @@ static void hor_visc_init(om_model *m)
     m->hv_use_land_mask = 1;
     m->hv_Kh_bg_min = 0.0;
     m->hv_h_neglect = dmax(1.0e-10, 1.0e-3);
-    const double Smag_Lap_const = 0.15, Smag_bi_const = 0.06;
+    const double Smag_Lap_const = 0.15, Smag_bi_const = 0.06, Leith_const = 1.0;   /* :55-57 */
     for (int j = m->jsd; j <= m->jed; ++j)
         for (int i = m->isd; i <= m->ied; ++i) {
             A2(m->hv_dx2h, i, j) = A2(m->dxT, i, j) * A2(m->dxT, i, j);
@@ -1229,7 +1235,12 @@ static void hor_visc_init(om_model *m)
             A2(m->hv_reduction_xx, i, j) = 1.0;
             A2(m->hv_reduction_xy, i, j) = 1.0;
             const double grid_sp_h2 = A2(m->areaT, i, j);
+            const double grid_sp_h3 = sqrt(grid_sp_h2) * grid_sp_h2;
             const double grid_sp_h4 = grid_sp_h2 * grid_sp_h2;
+            A2(m->hv_Laplac3_const_xx, i, j) = Leith_const * grid_sp_h3;
+            A2(m->hv_Laplac3_const_xy, i, j) = Leith_const * grid_sp_h3;
+            A2(m->hv_Biharm6_const_xx, i, j) = Leith_const * grid_sp_h3 * grid_sp_h3;
+            A2(m->hv_Biharm6_const_xy, i, j) = Leith_const * grid_sp_h3 * grid_sp_h3;
             A2(m->hv_Kh_bg_xx, i, j) = c->Kh;
             A2(m->hv_Kh_bg_xy, i, j) = c->Kh;
             A2(m->hv_Ah_bg_xx, i, j) = c->Ah;
@@ -1251,8 +1262,16 @@ static void hor_visc_init(om_model *m)
 
 static inline double sq(double x) { return x * x; }
 
+/* inv_PI3 = 1/(3.14159265358979_dp**3), inv_PI6 = 1/(3.14159265358979_dp**6)
+ * (mom6_hor_visc_serial.F90:154-155); the integer powers by squaring, as the
+ * compiled reference evaluates them (x**6 = (x*x * x*x) * x*x; a left-to-right
+ * product differs in the last bit, which the Leith golden vectors show) */
+#define PI_LEITH 3.14159265358979
+#define INV_PI3 (1.0 / (PI_LEITH * PI_LEITH * PI_LEITH))
+#define INV_PI6 (1.0 / (((PI_LEITH * PI_LEITH) * (PI_LEITH * PI_LEITH)) * (PI_LEITH * PI_LEITH)))
+
 /* hor_visc, mom6_hor_visc_serial.F90:464-1109 (Laplacian, biharmonic,
- * Smagorinsky and the Kh/Ah bounds; Leith and FrictWork are not on the path) */
+ * Smagorinsky, Leith and the Kh/Ah bounds; FrictWork and modified Leith are not on the path) */
 void om_hor_visc(om_model *m, const double *u, const double *v, const double *h, double *diffu, double *diffv)
 {
     const int is = m->isc, ie = m->iec, js = m->jsc, je = m->jec, nz = m->nk;
@@ -1295,6 +1314,45 @@ void om_hor_visc(om_model *m, const double *u, const double *v, const double *h,
                     A2(m->hv_Del2v, i, J) = A2(m->hv_Idxdy2v, i, J) * ((A2(m->hv_dy2q, i, J) * A2(m->hv_sh_xy, i, J)) - (A2(m->hv_dy2q, i - 1, J) * A2(m->hv_sh_xy, i - 1, J))) -
                                             A2(m->hv_Idx2dyCv, i, J) * ((A2(m->hv_dx2h, i, J + 1) * A2(m->hv_sh_xx, i, J + 1)) - (A2(m->hv_dx2h, i, J) * A2(m->hv_sh_xx, i, J)));
         }
+        if (m->hv_Leith_Kh) {   /* Leith: vorticity and its gradients (:610-727; modified Leith off) */
+            for (int J = Jsq; J <= Jeq; ++J)
+                for (int I = Isq; I <= Ieq; ++I)
+                    A2(m->hv_vort_xy, I, J) = (m->hv_no_slip ? (2.0 - A2(m->mask2dBu, I, J)) : A2(m->mask2dBu, I, J)) *
+                                              (A2(m->hv_dvdx, I, J) - A2(m->hv_dudy, I, J));
+            for (int j = js; j <= je; ++j)
+                for (int i = is; i <= ie; ++i) {
+                    const double DY_dxBu_loc = A2(m->dyBu, i, j) * A2(m->IdxBu, i, j);
+                    A2(m->hv_vort_xy_dx, i, j) = DY_dxBu_loc * ((A2(m->hv_vort_xy, i, j) * A2(m->IdyCu, i, j)) -
+                                                                (A2(m->hv_vort_xy, i - 1, j) * A2(m->IdyCu, i - 1, j)));
+                }
+            for (int j = js; j <= je; ++j)
+                for (int I = Isq; I <= Ieq; ++I) {
+                    const double DX_dyBu_loc = A2(m->dxBu, I, j) * A2(m->IdyBu, I, j);
+                    A2(m->hv_vort_xy_dy, I, j) = DX_dyBu_loc * ((A2(m->hv_vort_xy, I, j) * A2(m->IdxCv, I, j)) -
+                                                                (A2(m->hv_vort_xy, I, j - 1) * A2(m->IdxCv, I, j - 1)));
+                }
+            if (m->hv_Leith_Ah)
+                for (int J = Jsq; J <= Jeq; ++J)
+                    for (int I = Isq; I <= Ieq; ++I)
+                        A2(m->hv_Del2vort_q, I, J) =
+                            A2(m->hv_DY_dxBu, I, J) * ((A2(m->hv_vort_xy_dx, I + 1, J) * A2(m->IdyCv, I + 1, J)) -
+                                                       (A2(m->hv_vort_xy_dx, I, J) * A2(m->IdyCv, I, J))) +
+                            A2(m->hv_DX_dyBu, I, J) * ((A2(m->hv_vort_xy_dy, I, J + 1) * A2(m->IdxCu, I, J + 1)) -
+                                                       (A2(m->hv_vort_xy_dy, I, J) * A2(m->IdxCu, I, J)));
+            for (int j = js; j <= je; ++j)
+                for (int i = is; i <= ie; ++i) {
+                    const double gx = 0.5 * (A2(m->hv_vort_xy_dx, i, j) + A2(m->hv_vort_xy_dx, i, j - 1));
+                    const double gy = 0.5 * (A2(m->hv_vort_xy_dy, i, j) + A2(m->hv_vort_xy_dy, i - 1, j));
+                    A2(m->hv_grad_vort_mag_h, i, j) = sqrt(sq(gx) + sq(gy));
+                    A2(m->hv_vert_vort_mag, i, j) = A2(m->hv_grad_vort_mag_h, i, j) + 0.0;   /* + grad_div_mag_h = 0 */
+                }
+            for (int J = Jsq; J <= Jeq; ++J)
+                for (int I = Isq; I <= Ieq; ++I) {
+                    const double gx = 0.5 * (A2(m->hv_vort_xy_dx, I, J) + A2(m->hv_vort_xy_dx, I + 1, J));
+                    const double gy = 0.5 * (A2(m->hv_vort_xy_dy, I, J) + A2(m->hv_vort_xy_dy, I, J + 1));
+                    A2(m->hv_grad_vort_mag_q, I, J) = sqrt(sq(gx) + sq(gy));
+                }
+        }
         if (m->hv_Smag_Kh || m->hv_Smag_Ah)
             for (int j = js; j <= je; ++j)
                 for (int i = is; i <= ie; ++i) {
@@ -1316,6 +1374,10 @@ void om_hor_visc(om_model *m, const double *u, const double *v, const double *h,
                 for (int j = js; j <= je; ++j)
                     for (int i = is; i <= ie; ++i)
                         A2(m->hv_Kh, i, j) = dmax(A2(m->hv_Kh, i, j), A2(m->hv_Laplac2_const_xx, i, j) * A2(m->hv_Shear_mag, i, j));
+            if (m->hv_Leith_Kh)
+                for (int j = js; j <= je; ++j)
+                    for (int i = is; i <= ie; ++i)
+                        A2(m->hv_Kh, i, j) = dmax(A2(m->hv_Kh, i, j), A2(m->hv_Laplac3_const_xx, i, j) * A2(m->hv_vert_vort_mag, i, j) * INV_PI3);
             for (int j = js; j <= je; ++j)
                 for (int i = is; i <= ie; ++i) A2(m->hv_Kh, i, j) = dmax(A2(m->hv_Kh, i, j), m->hv_Kh_bg_min);
             if (m->hv_better_bound_Kh && m->hv_better_bound_Ah) {
@@ -1351,6 +1413,13 @@ void om_hor_visc(om_model *m, const double *u, const double *v, const double *h,
                 for (int j = js; j <= je; ++j)
                     for (int i = is; i <= ie; ++i)
                         A2(m->hv_Ah, i, j) = dmax(A2(m->hv_Ah, i, j), A2(m->hv_Biharm_const_xx, i, j) * A2(m->hv_Shear_mag, i, j));
+            if (m->hv_Leith_Ah)
+                for (int j = js; j <= je; ++j)
+                    for (int i = is; i <= ie; ++i) {
+                        const double Del2vort_h = 0.25 * ((A2(m->hv_Del2vort_q, i, j) + A2(m->hv_Del2vort_q, i - 1, j - 1)) +
+                                                          (A2(m->hv_Del2vort_q, i - 1, j) + A2(m->hv_Del2vort_q, i, j - 1)));
+                        A2(m->hv_Ah, i, j) = dmax(A2(m->hv_Ah, i, j), A2(m->hv_Biharm6_const_xx, i, j) * fabs(Del2vort_h) * INV_PI6);
+                    }
             if (m->hv_better_bound_Ah) {
                 if (m->hv_better_bound_Kh) {
                     for (int j = js; j <= je; ++j)
@@ -1388,6 +1457,10 @@ void om_hor_visc(om_model *m, const double *u, const double *v, const double *h,
                         const double sh_xy_sq = sq(A2(m->hv_sh_xy, I, J));
                         A2(m->hv_Kh, I, J) = dmax(A2(m->hv_Kh, I, J), A2(m->hv_Laplac2_const_xy, I, J) * sqrt(sh_xx_sq + sh_xy_sq));
                     }
+            if (m->hv_Leith_Kh)
+                for (int J = Jsq; J <= Jeq; ++J)
+                    for (int I = Isq; I <= Ieq; ++I)
+                        A2(m->hv_Kh, I, J) = dmax(A2(m->hv_Kh, I, J), A2(m->hv_Laplac3_const_xy, I, J) * A2(m->hv_grad_vort_mag_q, I, J) * INV_PI3);
             for (int J = Jsq; J <= Jeq; ++J)
                 for (int I = Isq; I <= Ieq; ++I) A2(m->hv_Kh, I, J) = dmax(A2(m->hv_Kh, I, J), m->hv_Kh_bg_min);
             if (m->hv_better_bound_Kh) {
@@ -1418,6 +1491,10 @@ void om_hor_visc(om_model *m, const double *u, const double *v, const double *h,
                         const double sh_xy_sq = sq(A2(m->hv_sh_xy, I, J));
                         A2(m->hv_Ah, I, J) = dmax(A2(m->hv_Ah, I, J), A2(m->hv_Biharm_const_xy, I, J) * sqrt(sh_xx_sq + sh_xy_sq));
                     }
+            if (m->hv_Leith_Ah)
+                for (int J = Jsq; J <= Jeq; ++J)
+                    for (int I = Isq; I <= Ieq; ++I)
+                        A2(m->hv_Ah, I, J) = dmax(A2(m->hv_Ah, I, J), A2(m->hv_Biharm6_const_xy, I, J) * fabs(A2(m->hv_Del2vort_q, I, J)) * INV_PI6);
             if (m->hv_better_bound_Ah) {
                 for (int J = Jsq; J <= Jeq; ++J)
                     for (int I = Isq; I <= Ieq; ++I) {
@@ -1682,6 +1759,9 @@ om_model *om_create(const om_config *c)
     R2D(hv_str_xx); R2D(hv_str_xy); R2D(hv_bhstr_xx); R2D(hv_bhstr_xy); R2D(hv_h_u); R2D(hv_h_v);
     R2D(hv_hq); R2D(hv_Kh); R2D(hv_Ah); R2D(hv_Shear_mag); R2D(hv_Del2u); R2D(hv_Del2v);
     R2D(hv_hrat_min); R2D(hv_visc_bound_rem);
+    R2D(hv_Laplac3_const_xx); R2D(hv_Laplac3_const_xy); R2D(hv_Biharm6_const_xx); R2D(hv_Biharm6_const_xy);
+    R2D(hv_vort_xy); R2D(hv_vort_xy_dx); R2D(hv_vort_xy_dy); R2D(hv_grad_vort_mag_h); R2D(hv_grad_vort_mag_q);
+    R2D(hv_vert_vort_mag); R2D(hv_Del2vort_q);
 #undef R2D
 #undef R3D
     grid_init(m);

@@ -55,6 +55,7 @@ typedef struct om_config {
     int meridional;          /* continuity also takes a meridional pass after the zonal one
                                 (no reference counterpart; see om_continuity_ppm_y) */
     int diag;                /* the driver's diag_mode: post the predictor's diffu/diffv sums */
+    int leith;               /* hor_visc_init Leith=.true. (Leith_Kh, and Leith_Ah with biharmonic) */
 } om_config;
 
 void om_config_default(om_config *c);

@@ -41,7 +41,7 @@ program mom6_ref_harness
     real(dp), allocatable, dimension(:, :) :: eta, ubt, vbt, ubt_av, vbt_av, eta_av, uhbt, du_cor
 
     integer :: ni, nj, nk, niter, bt_nsteps, bc_x, bc_y, dump_mode
-    integer :: lap, bih, smag, noslip, bbound, scheme, mono
+    integer :: lap, bih, smag, noslip, bbound, leith, scheme, mono
     real(dp) :: dt, dx_km, Kh, Ah, Kv, Kv_ml, Kv_bbl, Hmix, Hbbl, ray_bot
     character(len=512) :: cfgfile, infile, outfile
     integer :: iu, ou, iter, k, i, j
@@ -52,7 +52,7 @@ program mom6_ref_harness
     read (iu, *) ni, nj, nk, niter, bt_nsteps
     read (iu, *) dt, dx_km
     read (iu, *) bc_x, bc_y
-    read (iu, *) Kh, Ah, lap, bih, smag, noslip, bbound
+    read (iu, *) Kh, Ah, lap, bih, smag, noslip, bbound, leith
     read (iu, *) Kv, Kv_ml, Kv_bbl, Hmix, Hbbl, ray_bot
     read (iu, *) scheme, mono
     read (iu, '(A)') infile
@@ -102,7 +102,8 @@ program mom6_ref_harness
     visc_CS%a_u = 0; visc_CS%a_v = 0; visc_CS%h_u = 0; visc_CS%h_v = 0
     visc_CS%visc_rem_u = 0; visc_CS%visc_rem_v = 0
     call hor_visc_init(hvisc_CS, G, GV, Kh=Kh, Ah=Ah, Laplacian=(lap /= 0), biharmonic=(bih /= 0), &
-                       Smagorinsky=(smag /= 0), no_slip=(noslip /= 0), better_bound=(bbound /= 0))
+                       Smagorinsky=(smag /= 0), Leith=(leith /= 0), no_slip=(noslip /= 0), &
+                       better_bound=(bbound /= 0))
 
     if (per) then
         call fill3(u); call fill3(v); call fill3(h); call fill3(h0)

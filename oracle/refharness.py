@@ -39,7 +39,8 @@ def run(cfg, case: dict, niter: int, dump_mode: int = 0) -> dict:
         with open(fcfg, "w") as f:
             f.write(f"{cfg.ni} {cfg.nj} {cfg.nk} {niter} {cfg.bt_nsteps}\n")
             f.write(f"{cfg.dt!r} {cfg.dx_km!r}\n{cfg.bc_x} {cfg.bc_y}\n")
-            f.write(f"{cfg.Kh!r} {cfg.Ah!r} {int(cfg.laplacian)} {int(cfg.biharmonic)} 0 0 0\n")
+            f.write(f"{cfg.Kh!r} {cfg.Ah!r} {int(cfg.laplacian)} {int(cfg.biharmonic)} {int(cfg.smagorinsky)} "
+                    f"{int(cfg.no_slip)} {int(cfg.better_bound)} {int(cfg.leith)}\n")
             f.write(f"{cfg.Kv!r} {cfg.Kv_ml!r} {cfg.Kv_extra_bbl!r} {cfg.Hmix!r} {cfg.Hbbl!r} {cfg.ray_bottom!r}\n")
             f.write(f"{cfg.coriolis_scheme} {int(cfg.monotonic)}\n{fin}\n{fout}\n{dump_mode}\n")
         env = dict(os.environ)

@@ -2,7 +2,7 @@
 serial modules (compiled from /root/reference by oracle/ref/Makefile) through
 oracle/ref/ref_harness.F90.  Run in the build container:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 
 Each case stores its configuration, the inputs (seeded 3D fields are stored
 as their recipe, regenerated bit-identically by the host generator) and every
@@ -34,6 +34,19 @@ CASES = {
     "cw84_20x20x6": (dict(ni=20, nj=20, nk=6, bt_nsteps=15, monotonic=True), "seeded", 2),
     "arakawa_hsu90_20x20x6": (dict(ni=20, nj=20, nk=6, bt_nsteps=15, coriolis_scheme=2), "reference", 2),
     "arakawa_lamb81_20x20x6": (dict(ni=20, nj=20, nk=6, bt_nsteps=15, coriolis_scheme=3), "seeded", 2),
+    # hor_visc_init's other closures (mom6_hor_visc_serial.F90:160-193, 612-1040)
+    "smag_bounds_noslip_walls_24x20x6": (dict(ni=24, nj=20, nk=6, bt_nsteps=16, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                              coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, biharmonic=True,
+                                              Ah=1e11, smagorinsky=True, no_slip=True, better_bound=True),
+                                         "seeded", 3),
+    "smag_laplacian_bound_periodic_22x22x5": (dict(ni=22, nj=22, nk=5, bt_nsteps=16, bc_x=BC_PERIODIC,
+                                                   bc_y=BC_PERIODIC, coriolis="fplane", f0=1e-4, Kh=20.0,
+                                                   smagorinsky=True, better_bound=True), "seeded", 3),
+    "leith_biharmonic_walls_24x20x6": (dict(ni=24, nj=20, nk=6, bt_nsteps=16, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                            coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=10.0, biharmonic=True,
+                                            Ah=1e9, leith=True), "seeded", 3),
+    "leith_smag_noslip_static_20x20x6": (dict(ni=20, nj=20, nk=6, bt_nsteps=15, Kh=10.0, smagorinsky=True,
+                                              leith=True, no_slip=True), "reference", 2),
 }
 SEED = 7
 
@@ -41,7 +54,10 @@ SEED = 7
 def main():
     if not refharness.available():
         sys.exit("the reference tree is not available; golden vectors cannot be regenerated here")
+    only = set(sys.argv[1:])   # case names to (re)write; all when none are given
     for name, (kw, init, niter) in CASES.items():
+        if only and name not in only:
+            continue
         cfg = Config(**kw)
         case = make_case(cfg, init=init, seed=SEED)
         out = refharness.run(cfg, case, niter, dump_mode=1)
