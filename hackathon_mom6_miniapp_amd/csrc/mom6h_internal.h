@@ -336,4 +336,11 @@ struct mom6h_hor_visc_cs {
     double *dy2h, *dx2h, *dy2q, *dx2q;
     double *Idx2dyCu, *Idxdy2u, *Idx2dyCv, *Idxdy2v;
     double *Del2u, *Del2v;   // biharmonic scratch (3D, zero outside their ranges)
+    // Smagorinsky / Leith viscosities and the thickness-aware bounds
+    // (hor_visc_init Smagorinsky=, Leith=, better_bound=)
+    int smagorinsky = 0, leith = 0, better_bound = 0;
+    double *areaT = nullptr, *dxBu = nullptr, *dyBu = nullptr, *IdxBu = nullptr, *IdyBu = nullptr;
+    // q-point Kh and Ah of the last layer of the previous call, read by the
+    // first layer's stress ring (zero before the first call)
+    double *ring_Kh = nullptr, *ring_Ah = nullptr;
 };

@@ -309,7 +309,8 @@ int mom6h_model_create(mom6h_model **out, const mom6h_config *c)
     MOM6H_REQUIRE(!(c->periodic_y && c->walls_y), "mom6h_model_create: periodic_y and walls_y exclude each other");
     MOM6H_REQUIRE(c->coriolis_scheme >= 1 && c->coriolis_scheme <= 3, "mom6h_model_create: bad coriolis_scheme");
     MOM6H_REQUIRE(c->precision == 0 || c->precision == 1, "mom6h_model_create: precision must be 0 (FP64) or 1 (mixed)");
-    MOM6H_REQUIRE(c->precision == 0 || (c->coriolis_scheme == 1 && c->laplacian && c->nk <= 80),
+    MOM6H_REQUIRE(c->precision == 0 || (c->coriolis_scheme == 1 && c->laplacian && c->nk <= 80 &&
+                                        !c->smagorinsky && !c->leith && !c->better_bound),
                   "mom6h_model_create: the mixed-precision form runs the fused stencils: Sadourny Coriolis, "
                   "Laplacian (+ biharmonic) viscosity, nk <= 80");
     auto *m = new mom6h_model();
@@ -559,6 +560,8 @@ int mom6h_model_prepare(mom6h_model *m)
                                dy2q.data(), dx2q.data()));
         RC(mom6h_hor_visc_set_options(m->hv, m->c.laplacian, m->c.biharmonic, m->c.Ah, m->c.no_slip,
                                       I2u.data(), Iu2.data(), I2v.data(), Iv2.data()));
+        RC(mom6h_hor_visc_set_closures(m->hv, m->c.smagorinsky, m->c.leith, m->c.better_bound, m->areaT.data(),
+                                       m->dxBu.data(), m->dyBu.data(), m->IdxBu.data(), m->IdyBu.data()));
     }
     // uniform-metric values for the fused kernels (exact array contents)
     {
@@ -894,6 +897,7 @@ int mom6h_model_destroy(mom6h_model *m)
 namespace mom6h {
 bool fused_ok(const mom6h_model *m)
 {
-    return m->fused && m->uniform && m->c.laplacian && m->c.coriolis_scheme == 1;
+    return m->fused && m->uniform && m->c.laplacian && m->c.coriolis_scheme == 1 && !m->c.smagorinsky &&
+           !m->c.leith && !m->c.better_bound;
 }
 }  // namespace mom6h

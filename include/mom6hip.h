@@ -188,6 +188,14 @@ int mom6h_hor_visc_init(mom6h_hor_visc_cs **cs, int isd, int ied, int jsd, int j
 int mom6h_hor_visc_set_options(mom6h_hor_visc_cs *cs, int laplacian, int biharmonic, double Ah,
                                int no_slip, const double *h_Idx2dyCu, const double *h_Idxdy2u,
                                const double *h_Idx2dyCv, const double *h_Idxdy2v);
+/* hor_visc_init's Smagorinsky=, Leith= and better_bound= options
+ * (mom6_hor_visc_serial.F90:160-286; Smagorinsky_Kh/_Ah, Leith_Kh/_Ah follow
+ * the Laplacian/biharmonic switches; better_bound sets the Kh and Ah bounds)
+ * with the grid metrics they need: areaT (the Smagorinsky, Leith and bound
+ * constants), dxBu, dyBu, IdxBu, IdyBu (Leith).  Before the first call. */
+int mom6h_hor_visc_set_closures(mom6h_hor_visc_cs *cs, int smagorinsky, int leith, int better_bound,
+                                const double *h_areaT, const double *h_dxBu, const double *h_dyBu,
+                                const double *h_IdxBu, const double *h_IdyBu);
 int mom6h_hor_visc(mom6h_hor_visc_cs *cs, const double *u, const double *v, const double *h,
                    double *diffu, double *diffv, void *stream);
 int mom6h_hor_visc_end(mom6h_hor_visc_cs *cs);
@@ -274,6 +282,8 @@ typedef struct mom6h_config {
                          Laplacian viscosity); FP64 vertical-viscosity solve, barotropic solver,
                          2D fields and BT_cont 2D fields.  Needs the fused stencils: uniform
                          metrics, Sadourny Coriolis, Laplacian viscosity, nk <= 80. */
+    int smagorinsky, leith, better_bound;   /* hor_visc closures (mom6h_hor_visc_set_closures);
+                                               FP64 form, module-by-module stencils */
 } mom6h_config;
 
 typedef struct mom6h_model mom6h_model;

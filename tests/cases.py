@@ -45,3 +45,28 @@ MERID_CASES = {
     "merid_static_ref": (Config(ni=20, nj=20, nk=5, bt_nsteps=10, meridional=True), "reference", 2),
 }
 MERID_COMPARE = ("v_cor", "dv_cor", "FA_v_NN", "FA_v_N0", "FA_v_S0", "FA_v_SS", "vBT_SS", "vBT_NN", "BT_h_v")
+
+# hor_visc closures (Smagorinsky, Leith, thickness-aware bounds, no-slip):
+# the general hor_visc kernel, module by module (the fused stencil has no
+# closures); the same configurations as the golden fixtures that pin the oracle
+CLOSURE_CASES = {
+    "smag_bounds_noslip_walls": (Config(ni=24, nj=20, nk=6, bt_nsteps=16, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                        coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, biharmonic=True, Ah=1e11,
+                                        smagorinsky=True, no_slip=True, better_bound=True), "seeded", 3),
+    "smag_laplacian_bound_periodic": (Config(ni=22, nj=22, nk=5, bt_nsteps=16, bc_x=BC_PERIODIC, bc_y=BC_PERIODIC,
+                                             coriolis="fplane", f0=1e-4, Kh=20.0, smagorinsky=True,
+                                             better_bound=True), "seeded", 3),
+    "leith_biharmonic_walls": (Config(ni=24, nj=20, nk=6, bt_nsteps=16, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                      coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=10.0, biharmonic=True, Ah=1e9,
+                                      leith=True), "seeded", 3),
+    "leith_smag_noslip_static": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, Kh=10.0, smagorinsky=True, leith=True,
+                                        no_slip=True), "reference", 2),
+    # across tile seams, all closures at once
+    "closures_seams_200x140": (Config(ni=200, nj=140, nk=8, bt_nsteps=16, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                      coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, biharmonic=True, Ah=1e11,
+                                      smagorinsky=True, leith=True, no_slip=True, better_bound=True), "seeded", 2),
+    # no-slip alone (the constant-viscosity kernels)
+    "noslip_walls_biharmonic": (Config(ni=24, nj=24, nk=6, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                       coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, biharmonic=True, Ah=1e11,
+                                       no_slip=True), "seeded", 2),
+}

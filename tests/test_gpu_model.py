@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from hackathon_mom6_miniapp_amd.state import make_case
-from tests.cases import CASES, COMPARE, COMPARE_FUSED, MERID_CASES, MERID_COMPARE
+from tests.cases import CASES, CLOSURE_CASES, COMPARE, COMPARE_FUSED, MERID_CASES, MERID_COMPARE
 
 pytestmark = pytest.mark.gpu
 
@@ -15,7 +15,7 @@ def _run_pair(name, fused=True):
     from hackathon_mom6_miniapp_amd.model import Model
     from oracle.pyoracle import Oracle
 
-    cfg, init, niter = CASES[name] if name in CASES else MERID_CASES[name]
+    cfg, init, niter = {**CASES, **MERID_CASES, **CLOSURE_CASES}[name]
     case = make_case(cfg, init=init, seed=11)
     gpu = Model(cfg, case)
     gpu.set_option("fused", 1.0 if fused else 0.0)
@@ -221,3 +221,22 @@ def test_every_level_count_bit_exact(direction):
                 bad.append(f"nk={nk} {f}")
         gpu.close()
     assert not bad, ", ".join(bad[:40])
+
+
+@pytest.mark.parametrize("name", sorted(CLOSURE_CASES))
+def test_viscosity_closures_bit_exact(name):
+    """Smagorinsky and Leith viscosities, the thickness-aware bounds and
+    no-slip walls (the general hor_visc kernel), every array the driver
+    holds, against the oracle (pinned by golden vectors of the same
+    configurations)."""
+    print(f"[gpu closures] case={name}", flush=True)
+    gpu, ora = _run_pair(name, fused=False)
+    cfg = CLOSURE_CASES[name][0]
+    bad = []
+    for f in COMPARE:
+        g, o = gpu.get(f), ora.field(f)
+        if not np.array_equal(g, o):
+            d = np.abs(g - o)
+            bad.append(f"{f}: max|diff|={d.max():.3e} at {np.unravel_index(d.argmax(), d.shape)} n={(d > 0).sum()}")
+    gpu.close()
+    assert not bad, "\n".join(bad)
