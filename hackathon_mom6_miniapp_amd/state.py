@@ -74,6 +74,10 @@ class Config:
     # continuity also takes a meridional PPM pass after the zonal one (no
     # counterpart in the reference, whose continuity is zonal only)
     meridional: bool = False
+    # continuity's barotropic-transport targets from the barotropic solver
+    # (uhbt = Datu*ubt_av, vhbt = Datv*vbt_av of the concurrent btstep call);
+    # the reference leaves them zero
+    coupled: bool = False
     # "fp64": the reference's arithmetic throughout; "mixed": FP32 layer
     # fields and stencils, FP64 vertical solve, barotropic solver and 2D fields
     precision: str = "fp64"

@@ -54,6 +54,7 @@ struct om_model {
     double *u, *v, *h, *h0, *htmp, *uh, *vh, *CAu, *CAv, *up, *vp, *diffu, *diffv;
     double *eta, *ubt, *vbt, *ubt_av, *vbt_av, *eta_av;
     double *uhbt, *u_cor, *du_cor, *por_face_areaU, *visc_rem_u;  /* continuity_init */
+    double *vhbt;            /* the meridional pass's barotropic-transport target */
     double *taux, *tauy, *Ray_u, *Ray_v;                             /* forces / visc */
     /* BT_cont (mom6_types.F90:89-110) */
     double *FA_u_EE, *FA_u_E0, *FA_u_W0, *FA_u_WW, *uBT_WW, *uBT_EE, *BT_h_u;
@@ -818,6 +819,7 @@ void om_continuity_ppm_y(om_model *m, const double *v, const double *hin, double
     transpose3(m, hin, t, hinT, nk);
     transpose3(m, h, t, hT, nk);
     transpose3(m, vh, t, vhT, nk);
+    transpose3(m, m->vhbt, t, t->uhbt, 1);
     om_continuity_ppm(t, vT, hinT, hT, vhT, dt);
     transpose3_back(t, hT, m, h, nk);
     transpose3_back(t, vhT, m, vh, nk);
@@ -1642,6 +1644,19 @@ void om_diag(om_model *m, double *out)
     diag_stats(m, m->diag_mass, 1, out + 12);
 }
 
+/* The coupled form's targets (no reference counterpart: the reference driver
+ * passes uhbt = 0): the barotropic transports of the time-averaged barotropic
+ * velocities of the btstep call just made, uhbt = Datu*ubt_av and
+ * vhbt = Datv*vbt_av over the data domain (zero where btstep leaves the
+ * averages zero: outside its narrow ranges, so the domain stays closed). */
+static void coupled_targets(om_model *m)
+{
+    for (size_t q = 0; q < m->N2; ++q) {
+        m->uhbt[q] = m->bt_Datu[q] * m->ubt_av[q];
+        m->vhbt[q] = m->bt_Datv[q] * m->vbt_av[q];
+    }
+}
+
 void om_step(om_model *m, int niter)
 {
     const double dt = m->c.dt;
@@ -1656,6 +1671,7 @@ void om_step(om_model *m, int niter)
         vel_update(m, m->up, m->vp, m->u, m->v, dt);
         om_vert_visc(m, m->up, m->vp, m->h, dt);
         om_btstep(m, m->eta, m->ubt, m->vbt, m->ubt_av, m->vbt_av, m->eta_av);
+        if (m->c.coupled) coupled_targets(m);
         if (per) { halo_fill(m, m->up, nk); halo_fill(m, m->vp, nk); }
         om_continuity_ppm(m, m->up, m->h0, m->h, m->uh, dt);
         if (m->c.meridional) meridional_pass(m, m->vp, dt);
@@ -1672,6 +1688,7 @@ void om_step(om_model *m, int niter)
         vel_update(m, m->u, m->v, m->u, m->v, 0.5 * dt);
         om_vert_visc(m, m->u, m->v, m->h, 0.5 * dt);
         om_btstep(m, m->eta_av, m->ubt_av, m->vbt_av, m->ubt_av, m->vbt_av, m->eta);
+        if (m->c.coupled) coupled_targets(m);
         /* htmp <- h over the full data domain (the driver's loop bound
          * `i=isd,jed` at :375 is read as isd..ied; identical when ni == nj) */
         memcpy(m->htmp, m->h, m->N3 * sizeof(double));
@@ -1726,7 +1743,7 @@ om_model *om_create(const om_config *c)
     R3D(u); R3D(v); R3D(h); R3D(h0); R3D(htmp); R3D(uh); R3D(vh); R3D(CAu); R3D(CAv);
     R3D(up); R3D(vp); R3D(diffu); R3D(diffv);
     R2D(eta); R2D(ubt); R2D(vbt); R2D(ubt_av); R2D(vbt_av); R2D(eta_av);
-    R2D(uhbt); R3D(u_cor); R2D(du_cor);
+    R2D(uhbt); R2D(vhbt); R3D(u_cor); R2D(du_cor);
     reg(m, "por_face_areaU", &m->por_face_areaU, N3, 1.0);
     reg(m, "visc_rem_u", &m->visc_rem_u, N3, 1.0);
     R2D(taux); R2D(tauy); R3D(Ray_u); R3D(Ray_v);

@@ -56,6 +56,8 @@ typedef struct om_config {
                                 (no reference counterpart; see om_continuity_ppm_y) */
     int diag;                /* the driver's diag_mode: post the predictor's diffu/diffv sums */
     int leith;               /* hor_visc_init Leith=.true. (Leith_Kh, and Leith_Ah with biharmonic) */
+    int coupled;             /* continuity's barotropic-transport targets from btstep: uhbt = Datu*ubt_av,
+                                vhbt = Datv*vbt_av after each btstep call (the reference leaves uhbt = 0) */
 } om_config;
 
 void om_config_default(om_config *c);

@@ -31,7 +31,7 @@ class OmConfig(C.Structure):
                 ("Hmix", C.c_double), ("Hbbl", C.c_double), ("ray_bottom", C.c_double),
                 ("coriolis_scheme", C.c_int), ("monotonic", C.c_int),
                 ("simple_2nd", C.c_int), ("upwind_1st", C.c_int), ("meridional", C.c_int),
-                ("diag", C.c_int), ("leith", C.c_int)]
+                ("diag", C.c_int), ("leith", C.c_int), ("coupled", C.c_int)]
 
 
 _lib = None
@@ -84,6 +84,7 @@ class Oracle:
         c.leith = int(cfg.leith)
         c.no_slip = int(cfg.no_slip)
         c.better_bound = int(cfg.better_bound)
+        c.coupled = int(cfg.coupled)
         c.monotonic = int(cfg.monotonic)
         c.meridional = int(getattr(cfg, "meridional", False))
         c.diag = int(diag)

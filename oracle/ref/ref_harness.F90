@@ -41,7 +41,7 @@ program mom6_ref_harness
     real(dp), allocatable, dimension(:, :) :: eta, ubt, vbt, ubt_av, vbt_av, eta_av, uhbt, du_cor
 
     integer :: ni, nj, nk, niter, bt_nsteps, bc_x, bc_y, dump_mode
-    integer :: lap, bih, smag, noslip, bbound, leith, scheme, mono
+    integer :: lap, bih, smag, noslip, bbound, leith, scheme, mono, coupled
     real(dp) :: dt, dx_km, Kh, Ah, Kv, Kv_ml, Kv_bbl, Hmix, Hbbl, ray_bot
     character(len=512) :: cfgfile, infile, outfile
     integer :: iu, ou, iter, k, i, j
@@ -54,7 +54,7 @@ program mom6_ref_harness
     read (iu, *) bc_x, bc_y
     read (iu, *) Kh, Ah, lap, bih, smag, noslip, bbound, leith
     read (iu, *) Kv, Kv_ml, Kv_bbl, Hmix, Hbbl, ray_bot
-    read (iu, *) scheme, mono
+    read (iu, *) scheme, mono, coupled
     read (iu, '(A)') infile
     read (iu, '(A)') outfile
     read (iu, *) dump_mode
@@ -142,6 +142,8 @@ program mom6_ref_harness
             call put2('p_taux_bot', visc_CS%taux_bot); call put2('p_tauy_bot', visc_CS%tauy_bot)
         end if
         call barotropic(eta, ubt, vbt, ubt_av, vbt_av, eta_av)
+        ! coupled form: continuity's target is btstep's time-averaged transport
+        if (coupled /= 0) uhbt = bt_CS%Datu*ubt_av
         if (dump_mode > 0 .and. iter == 1) then
             call put2('p_ubt_av', ubt_av); call put2('p_vbt_av', vbt_av); call put2('p_eta_av', eta_av)
         end if
@@ -167,6 +169,7 @@ program mom6_ref_harness
         call vert_visc_coef(u, v, h, visc_CS, G, GV)
         call vert_visc_apply(u, v, h, 0.5_dp*dt, visc_CS, G, GV, forces, visc)
         call barotropic(eta_av, ubt_av, vbt_av, ubt_av, vbt_av, eta)
+        if (coupled /= 0) uhbt = bt_CS%Datu*ubt_av
         htmp = h
         call continuity_PPM(u, htmp, h, uh, 0.5_dp*dt, G, GV, cont_CS, por_face_areaU, uhbt, &
                             visc_rem_u, u_cor, BT_cont, du_cor)
@@ -181,6 +184,7 @@ program mom6_ref_harness
     call put3('CAu', CAu); call put3('CAv', CAv); call put3('diffu', diffu); call put3('diffv', diffv)
     call put2('eta', eta); call put2('ubt_av', ubt_av); call put2('vbt_av', vbt_av)
     call put2('eta_av', eta_av); call put3('u_cor', u_cor); call put2('du_cor', du_cor)
+    if (coupled /= 0) call put2('uhbt', uhbt)
     call put_btc('')
     call put2('taux_bot', visc_CS%taux_bot); call put2('tauy_bot', visc_CS%tauy_bot)
     close (ou)

@@ -42,7 +42,7 @@ def run(cfg, case: dict, niter: int, dump_mode: int = 0) -> dict:
             f.write(f"{cfg.Kh!r} {cfg.Ah!r} {int(cfg.laplacian)} {int(cfg.biharmonic)} {int(cfg.smagorinsky)} "
                     f"{int(cfg.no_slip)} {int(cfg.better_bound)} {int(cfg.leith)}\n")
             f.write(f"{cfg.Kv!r} {cfg.Kv_ml!r} {cfg.Kv_extra_bbl!r} {cfg.Hmix!r} {cfg.Hbbl!r} {cfg.ray_bottom!r}\n")
-            f.write(f"{cfg.coriolis_scheme} {int(cfg.monotonic)}\n{fin}\n{fout}\n{dump_mode}\n")
+            f.write(f"{cfg.coriolis_scheme} {int(cfg.monotonic)} {int(cfg.coupled)}\n{fin}\n{fout}\n{dump_mode}\n")
         env = dict(os.environ)
         subprocess.check_call(["bash", "-c", f'ulimit -s unlimited && exec "{HARNESS}" "{fcfg}"'], env=env)
         return read_dump(fout, cfg)

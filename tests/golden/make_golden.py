@@ -47,6 +47,17 @@ CASES = {
                                             Ah=1e9, leith=True), "seeded", 3),
     "leith_smag_noslip_static_20x20x6": (dict(ni=20, nj=20, nk=6, bt_nsteps=15, Kh=10.0, smagorinsky=True,
                                               leith=True, no_slip=True), "reference", 2),
+    # the coupled form: continuity_PPM's uhbt argument set to btstep's
+    # time-averaged transport Datu*ubt_av after each btstep call
+    # (a seeded state with eta = ubt = vbt = 0 keeps the barotropic solution at
+    # zero; the seeded cases carry a Gaussian sea-surface bump of 0.5 m, 40 km)
+    "coupled_walls_beta_24x20x7": (dict(ni=24, nj=20, nk=7, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                        coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, coupled=True),
+                                   "seeded", 3, (0.5, 40.0e3)),
+    "coupled_static_reference_20x20x6": (dict(ni=20, nj=20, nk=6, bt_nsteps=15, coupled=True), "reference", 3),
+    "coupled_periodic_fplane_24x24x6": (dict(ni=24, nj=24, nk=6, bt_nsteps=20, bc_x=BC_PERIODIC,
+                                             bc_y=BC_PERIODIC, coriolis="fplane", f0=1e-4, coupled=True),
+                                        "seeded", 3, (0.5, 40.0e3)),
 }
 SEED = 7
 
@@ -55,11 +66,13 @@ def main():
     if not refharness.available():
         sys.exit("the reference tree is not available; golden vectors cannot be regenerated here")
     only = set(sys.argv[1:])   # case names to (re)write; all when none are given
-    for name, (kw, init, niter) in CASES.items():
+    for name, spec in CASES.items():
         if only and name not in only:
             continue
+        kw, init, niter = spec[:3]
+        bump = spec[3] if len(spec) > 3 else None
         cfg = Config(**kw)
-        case = make_case(cfg, init=init, seed=SEED)
+        case = make_case(cfg, init=init, seed=SEED, bump=bump)
         out = refharness.run(cfg, case, niter, dump_mode=1)
         meta = dict(config=kw, init=init, seed=SEED, niter=niter)
         arrays = {f"in_{k}": v for k, v in case.items() if k not in ("u", "v", "h0") or init == "reference"}

@@ -84,6 +84,8 @@ def test_oracle_modules_match_reference_golden(path):
     if per:
         o.halo_fill(up)
         o.halo_fill(vp)
+    if cfg.coupled:   # the target of the coupled form: Datu*ubt_av
+        f("uhbt")[...] = f("bt_Datu") * f("ubt_av")
     o.call("om_continuity_ppm", up, h0, h, uh, scalar=cfg.dt)
     if per:
         o.halo_fill(h)
@@ -261,3 +263,49 @@ def test_meridional_pass_is_the_transposed_zonal_pass():
     o.call("om_continuity_ppm_y", vs, hs, hy, vh, scalar=cfg.dt)
     assert np.array_equal(vh, uh.transpose(1, 0, 2))
     assert np.array_equal(hy, hx.transpose(1, 0, 2))
+
+
+@pytest.mark.parametrize("bc", ["walls", "periodic"])
+def test_coupled_form_meets_the_barotropic_transport_and_conserves_volume(bc):
+    """The coupled form (continuity's target uhbt = Datu*ubt_av of the btstep
+    call before it): after a step every face of the zonal pass carries, summed
+    over the layers, the barotropic transport of the corrector's btstep to
+    within the Newton tolerance, and the volume stays constant to round-off
+    (btstep's averages vanish outside its narrow ranges, so the box stays
+    closed)."""
+    from oracle.pyoracle import Oracle
+    if bc == "walls":
+        cfg = Config(ni=30, nj=24, nk=8, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL, coriolis="betaplane",
+                     f0=1e-4, beta=2e-11, Kh=1e3, coupled=True)
+    else:
+        cfg = Config(ni=26, nj=26, nk=6, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_PERIODIC, coriolis="fplane",
+                     f0=1e-4, coupled=True)
+    o = Oracle(cfg, make_case(cfg, init="seeded", seed=5, bump=(0.5, 40.0e3)))
+    o.step(3)
+    isd, ied, jsd, jed, isc, iec, jsc, jec = cfg.bounds
+    uh, uhbt = o.field("uh"), o.field("uhbt")
+    face = np.s_[isc - 2:iec, jsc - 1:jec]                     # I = is-1..ie, j = js..je
+    colsum = np.zeros_like(uhbt)
+    for k in range(cfg.nk):                                     # k order, as the Newton sums
+        colsum = colsum + uh[:, :, k]
+    scale = np.abs(uh[face]).sum(axis=2).max()
+    assert np.abs(uhbt[face]).max() > 1e-3 * scale              # a real target
+    assert np.abs(colsum[face] - uhbt[face]).max() <= 1e-9 * scale
+    mass0, mass1, rel = o.verify()[:3]
+    assert rel < 1e-13, rel
+
+
+def test_coupled_form_is_the_reference_form_when_the_barotropic_state_is_zero():
+    """With eta = ubt = vbt = 0 the barotropic solution stays zero, the targets
+    vanish and the coupled form reproduces the uncoupled one bit for bit."""
+    from dataclasses import replace
+
+    from oracle.pyoracle import Oracle
+    cfg = Config(ni=22, nj=20, nk=5, bt_nsteps=12, bc_x=BC_PERIODIC, bc_y=BC_WALL, coriolis="betaplane", f0=1e-4,
+                 beta=2e-11, Kh=1e3)
+    case = make_case(cfg, init="seeded", seed=9)
+    a, b = Oracle(cfg, case), Oracle(replace(cfg, coupled=True), case)
+    a.step(2)
+    b.step(2)
+    for f in ("u", "v", "h", "uh", "eta", "u_cor", "FA_u_W0", "BT_h_u"):
+        assert np.array_equal(a.field(f), b.field(f)), f
