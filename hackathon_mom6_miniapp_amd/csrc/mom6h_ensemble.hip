@@ -282,8 +282,10 @@ int mom6h_ensemble_create(mom6h_ensemble **out, const mom6h_config *member_cfg, 
     MOM6H_REQUIRE(n >= 1 && n <= 21, "mom6h_ensemble_create: 1..21 members (3 halo jobs each in one launch)");
     const mom6h_config &c0 = member_cfg[0];
     for (int q = 0; q < n; ++q)
-        MOM6H_REQUIRE(!member_cfg[q].smagorinsky && !member_cfg[q].leith && !member_cfg[q].better_bound,
-                      "mom6h_ensemble_create: the batched stencils have no Smagorinsky/Leith/bounded viscosity");
+        MOM6H_REQUIRE(!member_cfg[q].smagorinsky && !member_cfg[q].leith && !member_cfg[q].better_bound &&
+                          !member_cfg[q].coupled,
+                      "mom6h_ensemble_create: the batched stencils have no Smagorinsky/Leith/bounded viscosity "
+                      "and no coupled form");
     for (int q = 1; q < n; ++q) {
         const mom6h_config &c = member_cfg[q];
         MOM6H_REQUIRE(c.bt_nsteps == c0.bt_nsteps && c.precision == c0.precision && c.meridional == c0.meridional &&

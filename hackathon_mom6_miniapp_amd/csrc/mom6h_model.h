@@ -75,6 +75,8 @@ struct mom6h_model {
     hipGraph_t graph_bt = nullptr;       // one barotropic step (stream s1)
     hipGraphExec_t gexec_bt = nullptr;
     int graph_fused = -1;    // fused_ok() at capture time
+    // the coupled form: continuity's barotropic-transport targets (2D)
+    double *uhbt = nullptr, *vhbt = nullptr;
     // diagnostics (option "diag"): the predictor's vertical sums of diffu*h and
     // diffv*h in one shared 2D work array (the reference's work_2d), KE and
     // mass fields, the reduction workspace and 16 statistics on the device

@@ -141,6 +141,37 @@ struct StageTimer {
     }
 };
 
+// The coupled form's targets: uhbt = Datu*ubt_av, vhbt = Datv*vbt_av over the
+// data domain, from the btstep call just made (ubt_av, vbt_av vanish outside
+// btstep's ranges, so the boundary faces of a closed domain get no transport)
+__global__ void k_coupled_targets(long long n, const double *__restrict__ Datu, const double *__restrict__ ubt_av,
+                                  const double *__restrict__ Datv, const double *__restrict__ vbt_av,
+                                  double *__restrict__ uhbt, double *__restrict__ vhbt)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    uhbt[q] = Datu[q] * ubt_av[q];
+    vhbt[q] = Datv[q] * vbt_av[q];
+}
+
+// One btstep call of the coupled form on the 3D stream, then the targets of
+// the continuity call that follows (predictor: eta, ubt, vbt -> averages;
+// corrector: the averages -> eta, as in the reference driver, :285 and :365)
+int coupled_btstep(mom6h_model *m, bool corrector)
+{
+    hipStream_t s0 = m->s0;
+    {
+        StageTimer t(m, ST_BTSTEP, s0);
+        if (!corrector) RC(mom6h_btstep(m->bt_cs, m->eta, m->ubt, m->vbt, m->ubt_av, m->vbt_av, m->eta_av, s0));
+        else RC(mom6h_btstep(m->bt_cs, m->eta_av, m->ubt_av, m->vbt_av, m->ubt_av, m->vbt_av, m->eta, s0));
+    }
+    const long long n = m->d.s2;
+    k_coupled_targets<<<(unsigned)((n + 255) / 256), 256, 0, s0>>>(n, m->bt_cs->Datu, m->ubt_av, m->bt_cs->Datv,
+                                                                    m->vbt_av, m->uhbt, m->vhbt);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
+}
+
 // The barotropic chain of one step on stream s1 (:285, :365): predictor and
 // corrector btstep, then (periodic grids) the end-of-step refill of the 2D
 // fields it owns.  It never waits for the 3D stream.
@@ -193,11 +224,12 @@ int continuity_phase(mom6h_model *m, const double *u, const double *v, const dou
 {
     hipStream_t s0 = m->s0;
     StageTimer t(m, ST_CONTINUITY, s0);
+    const double *uhbt = m->c.coupled ? m->uhbt : nullptr, *vhbt = m->c.coupled ? m->vhbt : nullptr;
     if (!m->c.meridional)
-        return mom6h_continuity_ppm(m->cont, u, hin, hout, m->uh, dt, nullptr, &m->bt, m->u_cor, m->du_cor, 1, s0);
-    RC(mom6h_continuity_ppm(m->cont, u, hin, m->hx, m->uh, dt, nullptr, &m->bt, m->u_cor, m->du_cor, 1, s0));
+        return mom6h_continuity_ppm(m->cont, u, hin, hout, m->uh, dt, uhbt, &m->bt, m->u_cor, m->du_cor, 1, s0);
+    RC(mom6h_continuity_ppm(m->cont, u, hin, m->hx, m->uh, dt, uhbt, &m->bt, m->u_cor, m->du_cor, 1, s0));
     RC(fill_halos(m, {{m->hx, m->c.nk}}));
-    return mom6h_continuity_ppm_y(m->cont, v, m->hx, hout, m->vh, dt, nullptr, &m->bt_v, m->v_cor, m->dv_cor, 1,
+    return mom6h_continuity_ppm_y(m->cont, v, m->hx, hout, m->vh, dt, vhbt, &m->bt_v, m->v_cor, m->dv_cor, 1,
                                   s0);
 }
 
@@ -284,6 +316,7 @@ int enqueue_step(mom6h_model *m)
     { StageTimer t(m, ST_VERT_VISC, s0);
       RC(mom6h_vert_visc(m->vv, m->up, m->vp, m->h0, dt, m->taux, m->tauy, rayu, rayv, s0)); }
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->up, nk}, {m->vp, nk}})); }
+    if (m->c.coupled) RC(coupled_btstep(m, false));
     RC(continuity_phase(m, m->up, m->vp, m->h0, m->htmp, dt));
     if (m->diag) RC(post_predictor_diagnostics(m));
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->htmp, nk}})); }
@@ -292,8 +325,11 @@ int enqueue_step(mom6h_model *m)
     RC(momentum_phase(m, m->up, m->vp, m->htmp, m->u, m->v, 0.5 * dt));
     { StageTimer t(m, ST_VERT_VISC, s0);
       RC(mom6h_vert_visc(m->vv, m->u, m->v, m->htmp, 0.5 * dt, m->taux, m->tauy, rayu, rayv, s0)); }
+    if (m->c.coupled) RC(coupled_btstep(m, true));
     RC(continuity_phase(m, m->u, m->v, m->htmp, m->h, 0.5 * dt));
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}})); }
+    if (m->c.coupled && (m->halo.px || m->halo.py))   // the barotropic chain's end-of-step refill
+        for (double *a : {m->eta, m->ubt, m->vbt}) RC(halo_fill(m->d, m->halo, a, 1, s0));
     return MOM6H_OK;
 }
 
@@ -310,7 +346,7 @@ int mom6h_model_create(mom6h_model **out, const mom6h_config *c)
     MOM6H_REQUIRE(c->coriolis_scheme >= 1 && c->coriolis_scheme <= 3, "mom6h_model_create: bad coriolis_scheme");
     MOM6H_REQUIRE(c->precision == 0 || c->precision == 1, "mom6h_model_create: precision must be 0 (FP64) or 1 (mixed)");
     MOM6H_REQUIRE(c->precision == 0 || (c->coriolis_scheme == 1 && c->laplacian && c->nk <= 80 &&
-                                        !c->smagorinsky && !c->leith && !c->better_bound),
+                                        !c->smagorinsky && !c->leith && !c->better_bound && !c->coupled),
                   "mom6h_model_create: the mixed-precision form runs the fused stencils: Sadourny Coriolis, "
                   "Laplacian (+ biharmonic) viscosity, nk <= 80");
     auto *m = new mom6h_model();
@@ -333,6 +369,7 @@ int mom6h_model_create(mom6h_model **out, const mom6h_config *c)
     F2(eta); F2(ubt); F2(vbt); F2(ubt_av); F2(vbt_av); F2(eta_av); F2(du_cor);
     F2(taux); F2(tauy); F2(ray_u); F2(ray_v); F2(CoriolisBu); F2(mask2dT); F2(mask2dCu);
     F2(mask2dCv); F2(mask2dBu);
+    if (c->coupled) { F2(uhbt); F2(vhbt); }
 #undef F3
 #undef G3
 #undef F2
@@ -647,6 +684,22 @@ int mom6h_model_step(mom6h_model *m, int niter)
     // The fused path consumes vert_visc's coupling coefficients on chip; the
     // module path keeps them in the control structure, as the reference does.
     RC(mom6h_vert_visc_keep_coefficients(m->vv, fused_ok(m) ? 0 : 1));
+    if (m->c.coupled) {
+        // the coupled form: each continuity call waits for its btstep call, so
+        // the whole step runs in order on the 3D stream (one graph when captured)
+        if (m->use_graph && !m->profile) {
+            const int key = fused_ok(m) ? 1 : 0;
+            if (m->gexec && m->graph_fused != key) drop_graphs(m);
+            if (!m->gexec) {
+                RC(capture_on(m, m->s0, enqueue_step, &m->graph, &m->gexec));
+                m->graph_fused = key;
+            }
+            for (int it = 0; it < niter; ++it) MOM6H_CHECK(hipGraphLaunch(m->gexec, m->s0));
+            return MOM6H_OK;
+        }
+        for (int it = 0; it < niter; ++it) RC(enqueue_step(m));
+        return MOM6H_OK;
+    }
     if (m->use_graph && !m->profile && !m->serialize) {
         // One 3D step (s0) and one barotropic step (s1), each captured as a
         // graph on its own stream and replayed as the eager path enqueues:

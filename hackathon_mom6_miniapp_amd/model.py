@@ -26,6 +26,7 @@ def to_mconfig(cfg: Config) -> MConfig:
     c.Kh, c.Ah = cfg.Kh, cfg.Ah
     c.laplacian, c.biharmonic, c.no_slip = int(cfg.laplacian), int(cfg.biharmonic), int(cfg.no_slip)
     c.smagorinsky, c.leith, c.better_bound = int(cfg.smagorinsky), int(cfg.leith), int(cfg.better_bound)
+    c.coupled = int(cfg.coupled)
     c.Kv, c.Kv_ml, c.Kv_extra_bbl, c.Hmix, c.Hbbl = cfg.Kv, cfg.Kv_ml, cfg.Kv_extra_bbl, cfg.Hmix, cfg.Hbbl
     c.coriolis_scheme, c.monotonic = cfg.coriolis_scheme, int(cfg.monotonic)
     c.meridional = int(cfg.meridional)

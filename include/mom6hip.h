@@ -284,6 +284,10 @@ typedef struct mom6h_config {
                          metrics, Sadourny Coriolis, Laplacian viscosity, nk <= 80. */
     int smagorinsky, leith, better_bound;   /* hor_visc closures (mom6h_hor_visc_set_closures);
                                                FP64 form, module-by-module stencils */
+    int coupled;      /* continuity's barotropic-transport targets from the barotropic solver:
+                         uhbt = Datu*ubt_av (and vhbt = Datv*vbt_av for the meridional pass) of the
+                         btstep call before each continuity call; the reference passes uhbt = 0.
+                         FP64 form; the barotropic chain then runs in the 3D stream's order. */
 } mom6h_config;
 
 typedef struct mom6h_model mom6h_model;

@@ -70,3 +70,23 @@ CLOSURE_CASES = {
                                        coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, biharmonic=True, Ah=1e11,
                                        no_slip=True), "seeded", 2),
 }
+
+# the coupled form (continuity's barotropic-transport targets from btstep):
+# the golden configurations (pinned by the reference's continuity_PPM with
+# uhbt present) and seams, with a Gaussian sea-surface bump so that the
+# barotropic solution is not zero.  (Config, init, niter, bump)
+COUPLED_CASES = {
+    "coupled_walls_beta": (Config(ni=24, nj=20, nk=7, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                  coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, coupled=True),
+                           "seeded", 3, (0.5, 40.0e3)),
+    "coupled_static_reference": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, coupled=True), "reference", 3, None),
+    "coupled_periodic_fplane_merid": (Config(ni=24, nj=24, nk=6, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_PERIODIC,
+                                             coriolis="fplane", f0=1e-4, coupled=True, meridional=True),
+                                      "seeded", 3, (0.5, 40.0e3)),
+    "coupled_seams_200x140": (Config(ni=200, nj=140, nk=12, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                     coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, coupled=True),
+                              "seeded", 2, (0.5, 100.0e3)),
+    "coupled_seams_deep_merid": (Config(ni=140, nj=100, nk=40, bt_nsteps=20, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                        coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, coupled=True,
+                                        meridional=True), "seeded", 2, (0.5, 100.0e3)),
+}
