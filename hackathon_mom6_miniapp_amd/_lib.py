@@ -26,7 +26,7 @@ class MConfig(C.Structure):
                 ("Hmix", C.c_double), ("Hbbl", C.c_double),
                 ("coriolis_scheme", C.c_int), ("monotonic", C.c_int), ("meridional", C.c_int),
                 ("precision", C.c_int), ("smagorinsky", C.c_int), ("leith", C.c_int), ("better_bound", C.c_int),
-                ("coupled", C.c_int)]
+                ("coupled", C.c_int), ("ray3d", C.c_int)]
 
 
 _lib = None

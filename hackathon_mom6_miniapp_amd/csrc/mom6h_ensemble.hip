@@ -283,9 +283,9 @@ int mom6h_ensemble_create(mom6h_ensemble **out, const mom6h_config *member_cfg, 
     const mom6h_config &c0 = member_cfg[0];
     for (int q = 0; q < n; ++q)
         MOM6H_REQUIRE(!member_cfg[q].smagorinsky && !member_cfg[q].leith && !member_cfg[q].better_bound &&
-                          !member_cfg[q].coupled,
+                          !member_cfg[q].coupled && !member_cfg[q].ray3d,
                       "mom6h_ensemble_create: the batched stencils have no Smagorinsky/Leith/bounded viscosity "
-                      "and no coupled form");
+                      "and no coupled form or 3D Rayleigh drag");
     for (int q = 1; q < n; ++q) {
         const mom6h_config &c = member_cfg[q];
         MOM6H_REQUIRE(c.bt_nsteps == c0.bt_nsteps && c.precision == c0.precision && c.meridional == c0.meridional &&

@@ -137,6 +137,7 @@ struct VVArgs {
     const double *taux, *tauy, *ray_u, *ray_v, *mask2dCu, *mask2dCv;
     double *a_u, *a_v, *h_u, *h_v, *taux_bot, *tauy_bot;
     double Kv, Kv_ml, Kv_extra_bbl, Hmix, Hbbl;
+    const double *ray_u3, *ray_v3;   // 3D Rayleigh coefficients (the 3D variant; NULL otherwise)
 };
 
 // Arguments of one pass.  Names are those of the zonal pass; the meridional

@@ -77,6 +77,8 @@ struct mom6h_model {
     int graph_fused = -1;    // fused_ok() at capture time
     // the coupled form: continuity's barotropic-transport targets (2D)
     double *uhbt = nullptr, *vhbt = nullptr;
+    // Rayleigh drag at every level (config ray3d)
+    double *ray_u3d = nullptr, *ray_v3d = nullptr;
     // diagnostics (option "diag"): the predictor's vertical sums of diffu*h and
     // diffv*h in one shared 2D work array (the reference's work_2d), KE and
     // mass fields, the reduction workspace and 16 statistics on the device

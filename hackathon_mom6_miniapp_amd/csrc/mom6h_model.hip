@@ -302,6 +302,15 @@ int post_predictor_diagnostics(mom6h_model *m)
                             m->diag_out + 8, s0);
 }
 
+// vert_visc_coef + vert_visc_apply of one phase: Rayleigh drag in the bottom
+// layer (2D) or at every level (3D fields)
+int vert_visc_phase(mom6h_model *m, double *u, double *v, const double *h, double dt)
+{
+    if (m->c.ray3d)
+        return mom6h_vert_visc_ray3d(m->vv, u, v, h, dt, m->taux, m->tauy, m->ray_u3d, m->ray_v3d, m->s0);
+    return mom6h_vert_visc(m->vv, u, v, h, dt, m->taux, m->tauy, m->ray_u, m->ray_v, m->s0);
+}
+
 // The 3D part of one RK2 step (rk2_serial_driver.F90:218-399) on stream s0.
 int enqueue_step(mom6h_model *m)
 {
@@ -309,12 +318,10 @@ int enqueue_step(mom6h_model *m)
     const double dt = m->c.dt;
     const int nk = m->c.nk;
     hipStream_t s0 = m->s0;
-    const double *rayu = m->ray_u, *rayv = m->ray_v;
 
     // ---- predictor (h = h0 throughout)
     RC(momentum_phase(m, m->u, m->v, m->h0, m->up, m->vp, dt));
-    { StageTimer t(m, ST_VERT_VISC, s0);
-      RC(mom6h_vert_visc(m->vv, m->up, m->vp, m->h0, dt, m->taux, m->tauy, rayu, rayv, s0)); }
+    { StageTimer t(m, ST_VERT_VISC, s0); RC(vert_visc_phase(m, m->up, m->vp, m->h0, dt)); }
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->up, nk}, {m->vp, nk}})); }
     if (m->c.coupled) RC(coupled_btstep(m, false));
     RC(continuity_phase(m, m->up, m->vp, m->h0, m->htmp, dt));
@@ -323,8 +330,7 @@ int enqueue_step(mom6h_model *m)
 
     // ---- corrector (h = h_pred, held in htmp)
     RC(momentum_phase(m, m->up, m->vp, m->htmp, m->u, m->v, 0.5 * dt));
-    { StageTimer t(m, ST_VERT_VISC, s0);
-      RC(mom6h_vert_visc(m->vv, m->u, m->v, m->htmp, 0.5 * dt, m->taux, m->tauy, rayu, rayv, s0)); }
+    { StageTimer t(m, ST_VERT_VISC, s0); RC(vert_visc_phase(m, m->u, m->v, m->htmp, 0.5 * dt)); }
     if (m->c.coupled) RC(coupled_btstep(m, true));
     RC(continuity_phase(m, m->u, m->v, m->htmp, m->h, 0.5 * dt));
     { StageTimer t(m, ST_HALO, s0); RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}})); }
@@ -346,7 +352,8 @@ int mom6h_model_create(mom6h_model **out, const mom6h_config *c)
     MOM6H_REQUIRE(c->coriolis_scheme >= 1 && c->coriolis_scheme <= 3, "mom6h_model_create: bad coriolis_scheme");
     MOM6H_REQUIRE(c->precision == 0 || c->precision == 1, "mom6h_model_create: precision must be 0 (FP64) or 1 (mixed)");
     MOM6H_REQUIRE(c->precision == 0 || (c->coriolis_scheme == 1 && c->laplacian && c->nk <= 80 &&
-                                        !c->smagorinsky && !c->leith && !c->better_bound && !c->coupled),
+                                        !c->smagorinsky && !c->leith && !c->better_bound && !c->coupled &&
+                                        !c->ray3d),
                   "mom6h_model_create: the mixed-precision form runs the fused stencils: Sadourny Coriolis, "
                   "Laplacian (+ biharmonic) viscosity, nk <= 80");
     auto *m = new mom6h_model();
@@ -370,6 +377,7 @@ int mom6h_model_create(mom6h_model **out, const mom6h_config *c)
     F2(taux); F2(tauy); F2(ray_u); F2(ray_v); F2(CoriolisBu); F2(mask2dT); F2(mask2dCu);
     F2(mask2dCv); F2(mask2dBu);
     if (c->coupled) { F2(uhbt); F2(vhbt); }
+    if (c->ray3d) { F3(ray_u3d); F3(ray_v3d); }
 #undef F3
 #undef G3
 #undef F2

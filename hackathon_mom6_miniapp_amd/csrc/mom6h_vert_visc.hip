@@ -202,7 +202,10 @@ using VVSlot = std::conditional_t<sizeof(T) == 4, float, double>;
 // Block (bx, row) of one grid: 64 columns of one row; Zs = (nk+1)*64 slots of LDS.
 // Rows are interleaved v, u, v, u, ... (v row J = js-1+r/2, u row j = js+(r-1)/2)
 // so that consecutive rows share their h rows.
-template <typename T, bool KEEP>
+// RAY3: Rayleigh drag Ray(k) at every level from the 3D arrays ray_u3/ray_v3
+// (vert_visc_apply :427-458 with a general visc%Ray_u); otherwise the 2D
+// bottom-layer coefficient ray_u/ray_v and zero above it.
+template <typename T, bool KEEP, bool RAY3 = false>
 __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVSlot<T> *Zs)
 {
     const Dims &d = p.d;
@@ -213,6 +216,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
     const T *hA, *hB;
     T *vel;
     double *a_out, *hu_out, *tb_out;
+    const double *ray3 = nullptr;   // this column's Ray(1..nk) (RAY3)
     double m, sfc, Ray_bot;
     {
         const double dt_Rho0 = p.dt / 1035.0;
@@ -226,6 +230,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
             m = p.mask2dCu[c2];
             sfc = p.taux ? dt_Rho0 * p.taux[c2] * m : 0.0;
             Ray_bot = p.ray_u ? p.ray_u[c2] : 0.0;
+            if (RAY3) ray3 = p.ray_u3 + o;
         } else {              // v column (i, J), J = js-1..je
             const int i = d.isc + bx * 64 + lane, J = d.jsc - 1 + row / 2;
             if (i > d.iec) return;
@@ -236,6 +241,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
             m = p.mask2dCv[c2];
             sfc = p.tauy ? dt_Rho0 * p.tauy[c2] * m : 0.0;
             Ray_bot = p.ray_v ? p.ray_v[c2] : 0.0;
+            if (RAY3) ray3 = p.ray_v3 + o;
         }
     }
     const double h_neglect = 1.0e-10;
@@ -363,7 +369,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         if (KEEP) a_out[0] = 0.0;
         const double hu_k = (double)hv1 + h_neglect;
         if (KEEP) hu_out[0] = hu_k;
-        const double Ray = (nk == 1) ? Ray_bot : 0.0;
+        const double Ray = RAY3 ? ray3[0] : ((nk == 1) ? Ray_bot : 0.0);
         const double b_denom_1 = hu_k + dt * (Ray + 0.0);
         b1 = 1.0 / (b_denom_1 + dt * a_n);
         d1 = b_denom_1 * b1;
@@ -378,7 +384,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         const double hu_k = (double)hv_k + h_neglect;
         if (KEEP) hu_out[(k - 1) * s] = hu_k;
         Z(k) = (ZT)(dt * a_k * b1);                        // c1(k); z_i(k) is no longer read
-        const double b_denom_1 = hu_k + dt * (0.0 + a_k * d1);
+        const double b_denom_1 = hu_k + dt * ((RAY3 ? ray3[(k - 1) * s] : 0.0) + a_k * d1);
         b1 = 1.0 / (b_denom_1 + dt * a_next);
         d1 = b_denom_1 * b1;
         vprev = (hu_k * vk_cur + dt * a_k * vprev) * b1;
@@ -446,7 +452,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         const double hu_k = (double)hv_k + h_neglect;
         if (KEEP) hu_out[(k - 1) * s] = hu_k;
         Z(k) = (ZT)(dt * a_k * b1);
-        const double b_denom_1 = hu_k + dt * (Ray_bot + a_k * d1);
+        const double b_denom_1 = hu_k + dt * ((RAY3 ? ray3[(k - 1) * s] : Ray_bot) + a_k * d1);
         b1 = 1.0 / (b_denom_1 + dt * a_n);
         d1 = b_denom_1 * b1;
         vprev = (hu_k * vk_cur + dt * a_k * vprev) * b1;
@@ -502,12 +508,16 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
     // bottom stress (:452-458)
     const double RHO_0 = 1035.0;
     double t = RHO_0 * vprev * a_n;
-    t = t + RHO_0 * Ray_bot * vprev;
+    if (RAY3) {   // + sum_k RHO_0*Ray(k)*u(k) of the solved column, in k order
+        for (int k = 1; k <= nk; ++k) t = t + RHO_0 * ray3[(k - 1) * s] * (double)vel[(k - 1) * s];
+    } else {
+        t = t + RHO_0 * Ray_bot * vprev;
+    }
     *tb_out = t;
 }
 
 // one grid, tiles renumbered per XCD
-template <typename T, bool KEEP>
+template <typename T, bool KEEP, bool RAY3 = false>
 // FP64: one wave per SIMD (the LDS column), registers up to 512 with AGPRs;
 // FP32 state: the half-size column allows two, and the register budget keeps them
 __global__ void __launch_bounds__(64, 1) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1)))
@@ -516,7 +526,7 @@ k_vert_visc_chunked(VVArgs<T> p)
     extern __shared__ double Zs_raw[];
     VVSlot<T> *Zs = reinterpret_cast<VVSlot<T> *>(Zs_raw);
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    vv_body<T, KEEP>(p, tile % gridDim.x, tile / gridDim.x, Zs);
+    vv_body<T, KEEP, RAY3>(p, tile % gridDim.x, tile / gridDim.x, Zs);
 }
 
 // every member of an ensemble in one launch (LDS sized for the deepest member)
@@ -648,6 +658,30 @@ int mom6h_vert_visc(mom6h_vert_visc_cs *cs, double *u, double *v, const double *
                                                   cs->h_v, cs->c1, cs->c1 + n3, cs->taux_bot,
                                                   cs->tauy_bot, cs->Kv, cs->Kv_ml, cs->Kv_extra_bbl,
                                                   cs->Hmix, cs->Hbbl);
+    MOM6H_LAUNCH_CHECK();
+    return MOM6H_OK;
+}
+
+int mom6h_vert_visc_ray3d(mom6h_vert_visc_cs *cs, double *u, double *v, const double *h, double dt,
+                          const double *taux, const double *tauy, const double *ray_u, const double *ray_v,
+                          void *stream)
+{
+    MOM6H_REQUIRE(cs != nullptr, "mom6h_vert_visc_ray3d: NULL cs");
+    MOM6H_REQUIRE(u && v && h && ray_u && ray_v, "mom6h_vert_visc_ray3d: NULL array");
+    const Dims &d = cs->d;
+    const int ni = d.iec - d.isc + 2;
+    const int rows = (d.jec - d.jsc + 1) + (d.jec - d.jsc + 2);
+    const size_t lds = (size_t)(d.nk + 1) * 64 * sizeof(double);
+    MOM6H_REQUIRE((long long)lds <= cs->max_lds, "mom6h_vert_visc_ray3d: nk = %d exceeds the LDS column (%lld B)",
+                  d.nk, cs->max_lds);
+    VVArgs<double> p{d, u, v, h, dt, taux, tauy, nullptr, nullptr, cs->mask2dCu, cs->mask2dCv,
+                     cs->a_u, cs->a_v, cs->h_u, cs->h_v, cs->taux_bot, cs->tauy_bot, cs->Kv, cs->Kv_ml,
+                     cs->Kv_extra_bbl, cs->Hmix, cs->Hbbl, ray_u, ray_v};
+    const dim3 g((ni + 63) / 64, rows);
+    if (cs->keep_coefs)
+        k_vert_visc_chunked<double, true, true><<<g, dim3(64), lds, (hipStream_t)stream>>>(p);
+    else
+        k_vert_visc_chunked<double, false, true><<<g, dim3(64), lds, (hipStream_t)stream>>>(p);
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
 }

@@ -27,6 +27,7 @@ def to_mconfig(cfg: Config) -> MConfig:
     c.laplacian, c.biharmonic, c.no_slip = int(cfg.laplacian), int(cfg.biharmonic), int(cfg.no_slip)
     c.smagorinsky, c.leith, c.better_bound = int(cfg.smagorinsky), int(cfg.leith), int(cfg.better_bound)
     c.coupled = int(cfg.coupled)
+    c.ray3d = int(cfg.ray_profile != "bottom")
     c.Kv, c.Kv_ml, c.Kv_extra_bbl, c.Hmix, c.Hbbl = cfg.Kv, cfg.Kv_ml, cfg.Kv_extra_bbl, cfg.Hmix, cfg.Hbbl
     c.coriolis_scheme, c.monotonic = cfg.coriolis_scheme, int(cfg.monotonic)
     c.meridional = int(cfg.meridional)
@@ -69,6 +70,9 @@ class Model:
         ray = np.full((cfg.n1, cfg.n2), cfg.ray_bottom, order="F")
         self.upload("ray_u", ray)
         self.upload("ray_v", ray)
+        if cfg.ray_profile != "bottom":
+            self.upload("ray_u3d", case["Ray_u"])
+            self.upload("ray_v3d", case["Ray_v"])
 
     def close(self):
         if self.m and self.owned:

@@ -69,6 +69,10 @@ class Config:
     Hmix: float = 50.0
     Hbbl: float = 10.0
     ray_bottom: float = 1.0e-4
+    # Rayleigh drag profile: "bottom" (the reference driver's: ray_bottom in
+    # the bottom layer only) or "linear" (Ray(k) = ray_bottom*k/nk at every
+    # level, the 3D vertvisc_type fields)
+    ray_profile: str = "bottom"
     coriolis_scheme: int = 1
     monotonic: bool = False
     # continuity also takes a meridional PPM pass after the zonal one (no
@@ -246,6 +250,13 @@ def make_case(cfg: Config, init: str = "seeded", seed: int = 1234, bump: tuple |
         vbt = vbt * mCv
     out = dict(u=u, v=v, h0=h0, eta=eta, ubt=ubt, vbt=vbt, taux=taux, tauy=tauy,
                CoriolisBu=coriolis_bu(cfg), mask2dT=mT, mask2dCu=mCu, mask2dCv=mCv, mask2dBu=mBu)
+    if cfg.ray_profile == "linear":   # Ray(k) = ray_bottom*k/nk, as (ray_bottom*k)/nk
+        kk = np.arange(1, nk + 1, dtype=np.float64)
+        ray = np.broadcast_to((cfg.ray_bottom * kk / float(nk))[None, None, :], shp3)
+        out["Ray_u"] = ray.copy()
+        out["Ray_v"] = ray.copy()
+    elif cfg.ray_profile != "bottom":
+        raise ValueError(f"unknown ray_profile {cfg.ray_profile!r}")
     case = {key: (val if isinstance(val, Seeded) else np.asfortranarray(np.asarray(val, dtype=np.float64)))
             for key, val in out.items()}
     if not lazy3d:

@@ -160,6 +160,13 @@ int mom6h_vert_visc_init(mom6h_vert_visc_cs **cs, int isd, int ied, int jsd, int
 int mom6h_vert_visc(mom6h_vert_visc_cs *cs, double *u, double *v, const double *h, double dt,
                     const double *taux, const double *tauy, const double *ray_u_bot,
                     const double *ray_v_bot, void *stream);
+/* mom6h_vert_visc with the reference's general Rayleigh drag: ray_u/ray_v
+ * are 3D (n1*n2*nk, vertvisc_type Ray_u/Ray_v, src/common/mom6_types.F90:120-125),
+ * Ray(k) enters every level of the tridiagonal system and the bottom stress
+ * adds sum_k rho0*Ray(k)*u(k) (vert_visc_apply :427-458). */
+int mom6h_vert_visc_ray3d(mom6h_vert_visc_cs *cs, double *u, double *v, const double *h, double dt,
+                          const double *taux, const double *tauy, const double *ray_u, const double *ray_v,
+                          void *stream);
 /* Whether mom6h_vert_visc stores the coupling coefficients and effective
  * thicknesses (default 1).  With 0 they stay on chip: used when nothing
  * reads them after the call. */
@@ -288,6 +295,8 @@ typedef struct mom6h_config {
                          uhbt = Datu*ubt_av (and vhbt = Datv*vbt_av for the meridional pass) of the
                          btstep call before each continuity call; the reference passes uhbt = 0.
                          FP64 form; the barotropic chain then runs in the 3D stream's order. */
+    int ray3d;        /* Rayleigh drag from the 3D fields "ray_u3d"/"ray_v3d" (mom6h_vert_visc_ray3d)
+                         instead of the 2D bottom-layer "ray_u"/"ray_v"; FP64 form */
 } mom6h_config;
 
 typedef struct mom6h_model mom6h_model;

@@ -17,6 +17,13 @@ CASES = {
     "arakawa_hsu90": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, coriolis_scheme=2), "reference", 2),
     "arakawa_lamb81": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, coriolis_scheme=3), "seeded", 2),
     "deep_column_64": (Config(ni=64, nj=48, nk=30, bt_nsteps=30), "reference", 2),
+    # Rayleigh drag at every level (the 3D vertvisc_type fields), pinned by its golden fixture
+    "ray3d_walls_beta": (Config(ni=24, nj=20, nk=7, bt_nsteps=16, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, ray_bottom=1e-3,
+                                ray_profile="linear"), "seeded", 3),
+    "ray3d_seams_deep": (Config(ni=140, nj=90, nk=40, bt_nsteps=16, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, ray_bottom=1e-3,
+                                ray_profile="linear"), "seeded", 2),
 }
 
 # fields compared after the run in the module-by-module mode: every array the

@@ -58,6 +58,10 @@ CASES = {
     "coupled_periodic_fplane_24x24x6": (dict(ni=24, nj=24, nk=6, bt_nsteps=20, bc_x=BC_PERIODIC,
                                              bc_y=BC_PERIODIC, coriolis="fplane", f0=1e-4, coupled=True),
                                         "seeded", 3, (0.5, 40.0e3)),
+    # Rayleigh drag at every level (vertvisc_type Ray_u/Ray_v 3D): ray_bottom*k/nk
+    "ray3d_walls_beta_24x20x7": (dict(ni=24, nj=20, nk=7, bt_nsteps=16, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                      coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, ray_bottom=1e-3,
+                                      ray_profile="linear"), "seeded", 3),
 }
 SEED = 7
 
