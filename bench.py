@@ -97,7 +97,69 @@ def measured_traffic(kernel: str):
     return None
 
 
-def cpu_baseline(cfg, steps_cap=1):
+def host_cpu():
+    """(CPU model string, number of CPUs this process may use)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 0
+    return model, ncpu
+
+
+def cpu_baseline_same_state(cfg, args):
+    """The reference's own serial modules (oracle/_ref/mom6_ref_harness, built
+    from the reference sources) on the bench's seeded state: one RK2 step, one
+    thread.  The state is generated on the device (the host recipe's bits) and
+    read back."""
+    from oracle import refharness
+
+    from hackathon_mom6_miniapp_amd.model import Model
+    from hackathon_mom6_miniapp_amd.state import make_case
+    if not os.path.exists(refharness.HARNESS):
+        return None
+    case = make_case(cfg, init="seeded", seed=2024, lazy3d=True, bump=args.bump)
+    m = Model(cfg, case)
+    for f in ("u", "v", "h0"):
+        case[f] = m.get(f)
+    m.close()
+    t = refharness.time_steps(cfg, case, 1)
+    del case
+    if not t:
+        return None
+    return {"s_per_step": t[0], "value": cfg.cells / t[0],
+            "sample": f"1 RK2 step of the reference's serial modules (oracle/_ref/mom6_ref_harness) on the "
+                      f"benchmark's own seeded state, {cfg.ni}x{cfg.nj}x{cfg.nk}, bt_nsteps={cfg.bt_nsteps}"}
+
+
+def cpu_baseline(cfg, args=None, steps_cap=1):
+    """The reference serial path timed on this host, one thread: its modules
+    on the benchmark's own state (primary), and its unmodified driver on its
+    analytic state (beside it)."""
+    model, ncpu = host_cpu()
+    same = cpu_baseline_same_state(cfg, args) if args is not None else None
+    drv = cpu_baseline_driver(cfg, steps_cap)
+    if same is not None:
+        out = {"value": same["value"], "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+               "sample": same["sample"], "s_per_step": same["s_per_step"]}
+        if drv is not None:
+            out["driver_analytic_state"] = drv
+    else:
+        out = dict(drv)
+    out["cpu_model"] = model
+    out["host_cpus"] = ncpu
+    return out
+
+
+def cpu_baseline_driver(cfg, steps_cap=1):
     """The reference serial driver (built from its own sources into oracle/_ref)
     timed on this host, one thread, for one RK2 step of the same grid."""
     exe = os.path.join(ROOT, "oracle", "_ref", "rk2_serial_driver")
@@ -125,6 +187,51 @@ def cpu_baseline(cfg, steps_cap=1):
     return {"value": small.cells / t_step, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": "2 RK2 steps of the C oracle at 180x180x75 (same per-cell work)",
             "s_per_step": t_step}
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Achievable HBM bandwidth: device-to-device copies of nbytes (read +
+    write counted), timed between device synchronisations."""
+    import ctypes as C
+
+    from hackathon_mom6_miniapp_amd._lib import check, lib
+    L = lib()
+    a, b = C.c_void_p(), C.c_void_p()
+    check(L.mom6h_malloc(C.byref(a), nbytes), "malloc")
+    check(L.mom6h_malloc(C.byref(b), nbytes), "malloc")
+    try:
+        check(L.mom6h_memset(a, 1, nbytes), "memset")
+        for _ in range(2):
+            check(L.mom6h_memcpy_d2d(b, a, nbytes), "copy")
+        check(L.mom6h_device_synchronize(), "sync")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            check(L.mom6h_memcpy_d2d(b, a, nbytes), "copy")
+        check(L.mom6h_device_synchronize(), "sync")
+        dt = time.perf_counter() - t0
+    finally:
+        L.mom6h_free(a)
+        L.mom6h_free(b)
+    return 2.0 * nbytes * reps / dt / 1e9
+
+
+def dropin_driver(cfg, niter=10):
+    """The Fortran RK2 driver through the cuda_c-shaped module API
+    (bin/rk2_hip_driver ni nj nk niter bt_nsteps): its per-iteration wall times
+    (each ends in a device synchronisation), the first dropped."""
+    from hackathon_mom6_miniapp_amd.build import DRIVER
+    if not os.path.exists(DRIVER) or cfg.ni != cfg.nj:
+        return None
+    cmd = [DRIVER, str(cfg.ni), str(cfg.nj), str(cfg.nk), str(niter), str(cfg.bt_nsteps)]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    t = [float(x) for x in re.findall(r"RK2 iteration\s+\d+:\s+([0-9.]+) s", out.stdout)]
+    if out.returncode != 0 or len(t) < 2:
+        return None
+    t_step = sum(t[1:]) / len(t[1:])
+    return {"command": " ".join(["bin/rk2_hip_driver"] + cmd[1:]), "ms_per_step": t_step * 1e3,
+            "value": cfg.cells / t_step, "unit": "cell-updates/s",
+            "what": "the reference driver's configuration (analytic state, static halo, Kh = 100) through the "
+                    "module-by-module C ABI (one call per module, as app/rk2_cuda_c_driver.F90 makes them)"}
 
 
 def mixed_form(cfg, args, t_step_fp64):
@@ -273,6 +380,7 @@ def main():
                     help="add biharmonic viscosity with this Kh4 (m^4/s), e.g. 1e11; 0 = Laplacian only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="no per-stage HIP events")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the module-API Fortran driver leg")
     ap.add_argument("--mixed-steps", type=int, default=100,
                     help="steps of the FP64-vs-mixed error check (0: timing only, -1: skip the mixed form)")
     ap.add_argument("--config", choices=("dropin", "large", "ensemble"), default="dropin",
@@ -379,10 +487,21 @@ def main():
         "stages": per_stage,
     }
     model.close()
+    copy = copy_bandwidth()
+    if roofline is not None:
+        roofline["achievable_copy_GBps"] = round(copy, 1)
+        roofline["frac_of_copy"] = round(roofline["achieved"] / copy, 4)
+    out["hbm_copy_GBps"] = copy
+    print(f"[bench] device copy {copy:.0f} GB/s", file=sys.stderr, flush=True)
     if args.mixed_steps >= 0:
         out["mixed"] = mixed_form(cfg, args, t_step)
+    if not args.no_dropin and args.config == "dropin":
+        dropin = dropin_driver(cfg)
+        if dropin is not None:
+            out["dropin_module_api"] = dropin
+            print(f"[bench] module-API driver {dropin['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg)
+        out["cpu_baseline"] = cpu_baseline(cfg, args)
     print(json.dumps(out))
 
 

@@ -12,6 +12,8 @@
 !!   * for periodic grids the halo is refilled at the exchange points of
 !!     app/rk2_mpi_serial_driver.F90 and btstep uses the split API there.
 !! Usage: mom6_ref_harness <config.txt>
+!! dump_mode < 0: no output file; each iteration's wall time is printed
+!! ("RK2 iteration N: t s", as the reference driver prints it) instead.
 program mom6_ref_harness
     use iso_fortran_env, only: dp => real64, int64, int32
     use mom6_types, only: ocean_grid_type, verticalGrid_type, init_ocean_grid, init_verticalGrid, &
@@ -41,10 +43,11 @@ program mom6_ref_harness
     real(dp), allocatable, dimension(:, :) :: eta, ubt, vbt, ubt_av, vbt_av, eta_av, uhbt, du_cor
 
     integer :: ni, nj, nk, niter, bt_nsteps, bc_x, bc_y, dump_mode
-    integer :: lap, bih, smag, noslip, bbound, leith, scheme, mono, coupled
+    integer :: lap, bih, smag, noslip, bbound, leith, scheme, mono, coupled, ray3d
     real(dp) :: dt, dx_km, Kh, Ah, Kv, Kv_ml, Kv_bbl, Hmix, Hbbl, ray_bot
     character(len=512) :: cfgfile, infile, outfile
     integer :: iu, ou, iter, k, i, j
+    integer(int64) :: clk0, clk1, clk_rate
     logical :: per
 
     call get_command_argument(1, cfgfile)
@@ -53,7 +56,7 @@ program mom6_ref_harness
     read (iu, *) dt, dx_km
     read (iu, *) bc_x, bc_y
     read (iu, *) Kh, Ah, lap, bih, smag, noslip, bbound, leith
-    read (iu, *) Kv, Kv_ml, Kv_bbl, Hmix, Hbbl, ray_bot
+    read (iu, *) Kv, Kv_ml, Kv_bbl, Hmix, Hbbl, ray_bot, ray3d
     read (iu, *) scheme, mono, coupled
     read (iu, '(A)') infile
     read (iu, '(A)') outfile
@@ -76,6 +79,12 @@ program mom6_ref_harness
     call init_vertvisc_visc(visc, G, GV, use_rayleigh=.true.)
     visc%Ray_u = 0.0_dp; visc%Ray_v = 0.0_dp
     visc%Ray_u(:, :, nk) = ray_bot; visc%Ray_v(:, :, nk) = ray_bot
+    if (ray3d /= 0) then   ! Rayleigh drag at every level: ray_bot*k/nk
+        do k = 1, nk
+            visc%Ray_u(:, :, k) = ray_bot * real(k, dp) / real(nk, dp)
+            visc%Ray_v(:, :, k) = ray_bot * real(k, dp) / real(nk, dp)
+        end do
+    end if
 
     ! ---- inputs: u v h0 (3D); eta ubt vbt taux tauy CoriolisBu masks T Cu Cv Bu (2D)
     open (newunit=iu, file=trim(infile), access='stream', form='unformatted', status='old', action='read')
@@ -110,9 +119,11 @@ program mom6_ref_harness
         call fill2(eta); call fill2(ubt); call fill2(vbt)
     end if
 
-    open (newunit=ou, file=trim(outfile), access='stream', form='unformatted', status='replace')
+    if (dump_mode >= 0) &
+        open (newunit=ou, file=trim(outfile), access='stream', form='unformatted', status='replace')
 
     do iter = 1, niter
+        call system_clock(clk0, clk_rate)
         h = h0
         ! ---------------- predictor
         call transports(u, v, h)
@@ -177,7 +188,11 @@ program mom6_ref_harness
             call fill3(u); call fill3(v); call fill3(h)
             call fill2(eta); call fill2(ubt); call fill2(vbt)
         end if
+        call system_clock(clk1)
+        if (dump_mode < 0) print '(A,I4,A,F10.6,A)', '  RK2 iteration ', iter, ':  ', &
+            real(clk1 - clk0, dp) / real(clk_rate, dp), ' s'
     end do
+    if (dump_mode < 0) stop
 
     call put3('u', u); call put3('v', v); call put3('h', h); call put3('h0', h0)
     call put3('up', up); call put3('vp', vp); call put3('uh', uh); call put3('vh', vh)

@@ -27,22 +27,43 @@ def build():
     return HARNESS
 
 
-def run(cfg, case: dict, niter: int, dump_mode: int = 0) -> dict:
+def _write_inputs(tmp: str, cfg, case: dict, niter: int, dump_mode: int):
     from hackathon_mom6_miniapp_amd.state import INPUT_ORDER
 
+    fin, fout, fcfg = (os.path.join(tmp, n) for n in ("in.bin", "out.bin", "cfg.txt"))
+    with open(fin, "wb") as f:
+        for name in INPUT_ORDER:
+            f.write(np.asarray(case[name], dtype="<f8").tobytes(order="F"))
+    with open(fcfg, "w") as f:
+        f.write(f"{cfg.ni} {cfg.nj} {cfg.nk} {niter} {cfg.bt_nsteps}\n")
+        f.write(f"{cfg.dt!r} {cfg.dx_km!r}\n{cfg.bc_x} {cfg.bc_y}\n")
+        f.write(f"{cfg.Kh!r} {cfg.Ah!r} {int(cfg.laplacian)} {int(cfg.biharmonic)} {int(cfg.smagorinsky)} "
+                f"{int(cfg.no_slip)} {int(cfg.better_bound)} {int(cfg.leith)}\n")
+        f.write(f"{cfg.Kv!r} {cfg.Kv_ml!r} {cfg.Kv_extra_bbl!r} {cfg.Hmix!r} {cfg.Hbbl!r} {cfg.ray_bottom!r} "
+                f"{int(cfg.ray_profile == 'linear')}\n")
+        f.write(f"{cfg.coriolis_scheme} {int(cfg.monotonic)} {int(cfg.coupled)}\n{fin}\n{fout}\n{dump_mode}\n")
+    return fcfg, fout
+
+
+def time_steps(cfg, case: dict, niter: int = 1, timeout: float = 600.0) -> list:
+    """Wall time of each RK2 iteration of the reference's serial modules on
+    this state, run by the prebuilt harness (no output file); [] without it."""
+    if not os.path.exists(HARNESS):
+        return []
+    with tempfile.TemporaryDirectory() as tmp:
+        fcfg, _ = _write_inputs(tmp, cfg, case, niter, -1)
+        out = subprocess.run(["bash", "-c", f'ulimit -s unlimited 2>/dev/null || ulimit -s hard; '
+                              f'exec "{HARNESS}" "{fcfg}"'], capture_output=True, text=True, timeout=timeout)
+    if out.returncode != 0:
+        return []
+    import re
+    return [float(x) for x in re.findall(r"RK2 iteration\s+\d+:\s+([0-9.]+) s", out.stdout)]
+
+
+def run(cfg, case: dict, niter: int, dump_mode: int = 0) -> dict:
     build()
     with tempfile.TemporaryDirectory() as tmp:
-        fin, fout, fcfg = (os.path.join(tmp, n) for n in ("in.bin", "out.bin", "cfg.txt"))
-        with open(fin, "wb") as f:
-            for name in INPUT_ORDER:
-                f.write(np.asarray(case[name], dtype="<f8").tobytes(order="F"))
-        with open(fcfg, "w") as f:
-            f.write(f"{cfg.ni} {cfg.nj} {cfg.nk} {niter} {cfg.bt_nsteps}\n")
-            f.write(f"{cfg.dt!r} {cfg.dx_km!r}\n{cfg.bc_x} {cfg.bc_y}\n")
-            f.write(f"{cfg.Kh!r} {cfg.Ah!r} {int(cfg.laplacian)} {int(cfg.biharmonic)} {int(cfg.smagorinsky)} "
-                    f"{int(cfg.no_slip)} {int(cfg.better_bound)} {int(cfg.leith)}\n")
-            f.write(f"{cfg.Kv!r} {cfg.Kv_ml!r} {cfg.Kv_extra_bbl!r} {cfg.Hmix!r} {cfg.Hbbl!r} {cfg.ray_bottom!r}\n")
-            f.write(f"{cfg.coriolis_scheme} {int(cfg.monotonic)} {int(cfg.coupled)}\n{fin}\n{fout}\n{dump_mode}\n")
+        fcfg, fout = _write_inputs(tmp, cfg, case, niter, dump_mode)
         env = dict(os.environ)
         subprocess.check_call(["bash", "-c", f'ulimit -s unlimited && exec "{HARNESS}" "{fcfg}"'], env=env)
         return read_dump(fout, cfg)
