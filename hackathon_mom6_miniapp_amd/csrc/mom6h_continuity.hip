@@ -524,6 +524,13 @@ template <typename T, int CW, int K, int FW = CF>
 struct ContShared {
     static constexpr int L = CW * (64 / FW) * K;
     T U[L][FW], S0[L][FW], S1[L][FW], R[9][FW];
+    // per-face 2D values, loaded with the block's set-up loads and parked here
+    // until they are used rather than held in registers: P[0] = dt*IareaT of
+    // the lane's cell (x), P[1] = dxCu of its face, P[2] = its barotropic
+    // transport target, P[3] = dt*min(IareaT) of its two cells, P[4], P[5] =
+    // dxT of cells 0 and 1; P[6], P[7] = du_eval and sum_dyhm of the face's
+    // Newton iteration, kept over the output stores for BT_cont
+    T P[8][FW];
     // per-wave votes of the block-wide "any" tests: 0, 1 alternate over the
     // Newton iterations, 2 = the barotropic-transport test, 3 = the du0 pass test
     int F[4][16];
@@ -573,6 +580,18 @@ __device__ __forceinline__ void lds_walk(const T (*col)[FW], int nk, int lane, S
     for (; k0 < nk; ++k0) step(col[k0][lane]);
 }
 
+// Barrier of the block column path.  The waves of a block exchange data
+// through LDS only (nothing a block stores to global memory is read back by
+// it), so the barrier orders LDS operations alone: __syncthreads' workgroup
+// fence would also wait for every global store still in flight, a memory
+// round trip at the first barrier after the output stores.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Block-wide "any lane has pred" with one barrier (the runtime's
 // __syncthreads_or takes three): each wave stores its vote in its own word of
 // F, then every thread reads the CW words.  The caller passes a buffer that no
@@ -583,7 +602,7 @@ __device__ __forceinline__ bool block_any(bool pred, int *F, int w, int lane)
 {
     const bool vote = __any(pred);
     if (lane == 0) F[w] = vote ? 1 : 0;
-    __syncthreads();
+    lds_barrier();
     int r = 0;
 #pragma unroll
     for (int q = 0; q < CW; ++q) r |= F[q];
@@ -682,6 +701,23 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     col.IdxT1 = (T)a.IdxT[cell(3)];
     const T mCu = (T)a.mask2dCu[c2];
     const long long cu = IX3(d, fi, fj, 1);
+    // the 2D values read after the column passes, issued here with the column
+    // loads: a load issued where its value is first used costs the whole
+    // block a memory round trip at that point
+    const double IareaT0 = a.IareaT[cell(2)], IareaT1 = a.IareaT[cell(3)];
+    const double dxT0 = a.dxT[cell(2)], dxT1 = a.dxT[cell(3)];
+    const double uhbt_f = a.uhbt ? a.uhbt[c2] : 0.0;
+    const double dxCu_f = a.dxCu[c2];
+    auto park = [&]() __attribute__((always_inline)) {
+        if (w == 0) {
+            if (!Y) S.P[0][lane] = (T)(a.dt * IareaT0);
+            S.P[1][lane] = (T)dxCu_f;
+            S.P[2][lane] = (T)uhbt_f;
+            S.P[3][lane] = (T)(a.dt * dmin(IareaT0, IareaT1));
+            S.P[4][lane] = (T)dxT0;
+            S.P[5][lane] = (T)dxT1;
+        }
+    };
     // PPM edges of cells 0 and 1 (zonal_edge_thickness :185-207); levels past
     // nk repeat level nk.  x: each lane reconstructs its own cell 0 from the
     // 5-cell stencil and takes cell 1 from the next lane (whose cell 0 it is),
@@ -690,19 +726,25 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     if (!Y) {
         const int ci = f_raw < d.iec + 1 ? f_raw : d.iec + 1;   // this lane's cell (in bounds)
         auto ccell = [&](int q) __attribute__((always_inline)) -> long long { return IX2(d, ci - 2 + q, fj); };
-        T m5[5];
+        T m5[5], hs[K][5], us[K];
 #pragma unroll
         for (int q = 0; q < 5; ++q) m5[q] = (T)a.mask2dT[ccell(q)];
-        T wW[K], wE[K], w0[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
             const int kr = k0 + kk + 1;
             const int k = kr < nk ? kr : nk;
             const long long o = (long long)(k - 1) * s;
-            S.U[k0 + kk][lane] = a.u[cu + o];
-            T h5[5], sl[3];
+            us[kk] = a.u[cu + o];
 #pragma unroll
-            for (int q = 0; q < 5; ++q) h5[q] = a.hin[ccell(q) + o];
+            for (int q = 0; q < 5; ++q) hs[kk][q] = a.hin[ccell(q) + o];
+        }
+        park();
+        T wW[K], wE[K], w0[K];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            S.U[k0 + kk][lane] = us[kk];
+            const T *h5 = hs[kk];
+            T sl[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) sl[q] = ppm_slope_v(m5[q], m5[q + 1], m5[q + 2], h5[q], h5[q + 1], h5[q + 2]);
             ppm_edges_v(m5, h5, sl, 2, wW[kk], wE[kk]);
@@ -715,19 +757,25 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
                     __shfl_down(w0[kk], 1, FW));
         }
     } else {
-        T m6[6];
+        T m6[6], hs[K][6], us[K];
 #pragma unroll
         for (int q = 0; q < 6; ++q) m6[q] = (T)a.mask2dT[cell(q)];
-        T wW[K], wE[K], w0[K], eW[K], eE[K], e0[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
             const int kr = k0 + kk + 1;
             const int k = kr < nk ? kr : nk;
             const long long o = (long long)(k - 1) * s;
-            S.U[k0 + kk][lane] = a.u[cu + o];
-            T h6[6], sl[4];
+            us[kk] = a.u[cu + o];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) h6[q] = a.hin[cell(q) + o];
+            for (int q = 0; q < 6; ++q) hs[kk][q] = a.hin[cell(q) + o];
+        }
+        park();
+        T wW[K], wE[K], w0[K], eW[K], eE[K], e0[K];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            S.U[k0 + kk][lane] = us[kk];
+            const T *h6 = hs[kk];
+            T sl[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) sl[q] = ppm_slope_v(m6[q], m6[q + 1], m6[q + 2], h6[q], h6[q + 1], h6[q + 2]);
             ppm_edges_v(m6, h6, sl, 2, wW[kk], wE[kk]);
@@ -772,17 +820,17 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     };
     // column sums in k order by wavefronts 0 and 1 (R[0] = x0 + sum S0, R[1] = x1 + sum dy*S1)
     auto sum_pass = [&](T x0, T x1, bool on) __attribute__((always_inline)) {
-        __syncthreads();
+        lds_barrier();
         if (w < 2 && on)
             S.R[w][lane] = (w == 0) ? col_sum<L>(S.S0, nk, lane, x0) : sum_dy(x1);
-        __syncthreads();
+        lds_barrier();
     };
 
     // first layer fluxes, their sums and the CFL bracket (zonal_mass_flux :300-393)
     const T CFL_dt = (T)a.CFL_limit_adjust / (T)a.dt;
-    const T dx_W = (T)a.dxT[cell(2)], dx_E = (T)a.dxT[cell(3)];
     eval_pass(T(0.0), true);
-    __syncthreads();
+    lds_barrier();
+    const T dx_W = S.P[4][lane], dx_E = S.P[5][lane];
     if (w < 2) {
         S.R[w][lane] = (w == 0) ? col_sum<L>(S.S0, nk, lane, T(0.0)) : sum_dy(T(0.0));
     } else if (w == 2) {
@@ -796,7 +844,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     } else if (F32 && w == 4) {   // scale of the rounding of the flux sum
         S.R[4][lane] = lds_fold<L>(S.S0, nk, lane, T(0.0), [](T x, T f) { return x + vabs(f); });
     }
-    __syncthreads();
+    lds_barrier();
     // the Newton iteration's starting point: the first pass's sums and the CFL
     // bracket; LDS rows 5..8 keep it for the zero-target iteration of BT_cont
     // (so that no register holds it across the first iteration)
@@ -810,9 +858,8 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         S.R[7][lane] = start.du_max_CFL;
         S.R[8][lane] = start.du_min_CFL;
     }
-    const T dtIarea = (T)(a.dt * dmin(a.IareaT[cell(2)], a.IareaT[cell(3)]));
-    const T bt = a.uhbt ? (T)a.uhbt[c2] : T(0.0);
-    const T tol_floor = F32 ? dtIarea * (T(4.0f * 1.1920929e-7f) * S.R[4][lane]) : T(0.0);
+    const T bt = S.P[2][lane];
+    const T tol_floor = F32 ? S.P[3][lane] * (T(4.0f * 1.1920929e-7f) * S.R[4][lane]) : T(0.0);
     const T tol_eta0 = (T)a.tol_eta0, tol_vel = (T)a.tol_vel;
 
     // zonal_flux_adjust (:739-916).  keep: uh_3d present (fluxes of the last
@@ -822,6 +869,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     auto flux_adjust = [&](const Start &st, T uhbt, bool keep, T *du_eval, T *dyhm_eval)
                            __attribute__((always_inline)) -> T {
         const int max_itts = 20;
+        const T dtIarea = S.P[3][lane];
         T du = T(0.0), du_max = st.du_max_CFL, du_min = st.du_min_CFL;
         T uh_err = st.uh_tot_0 - uhbt, duhdu_tot = st.duhdu_tot_0;
         T uh_err_best = vabs(uh_err);
@@ -888,6 +936,21 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // here S0 holds the layer fluxes of the last evaluation and, when
     // du_eval == du, sum_dyhm the sum of dy*hm at u + du
     const bool last_at_du = (du_eval == du);
+    // every wave stores its own copy (all hold the same values), so each
+    // reads back what it stored whether or not a barrier lies between
+    S.P[6][lane] = du_eval;
+    S.P[7][lane] = sum_dyhm;
+    // the convergence's cell thicknesses (cell I = I0 + lane, every level of
+    // the lane's group; clamped levels repeat nk), issued before the face
+    // stores so that one round trip covers all levels
+    T hc[K];
+    if (!Y) {
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const int kr = k0 + kk + 1;
+            hc[kk] = a.hin[IX3(d, i, j, kr < nk ? kr : nk)];
+        }
+    }
 
     // outputs at the face: uh, u_cor, du_cor; a face shared with the western
     // neighbour block is written by that block
@@ -905,14 +968,14 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     }
     // continuity_zonal_convergence (:148-176) for cell i = I (faces I-1 and I)
     if (!Y) {
-        const T dtIareaT = (T)(a.dt * a.IareaT[IX2(d, i, j)]);
+        const T dtIareaT = S.P[0][lane];   // dt*IareaT(i, j): cell 0 of the face
         const bool writes_cell = live && lane > 0 && lane < FW - 1 && i >= d.isc;
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
             if (writes_cell && valid(kk)) {
                 const long long c3 = IX3(d, i, j, k0 + kk + 1);
                 const T fw = S.S0[k0 + kk][lane - 1];   // face I-1 (the lane to the west), same level
-                const T v = a.hin[c3] - dtIareaT * (S.S0[k0 + kk][lane] - fw);
+                const T v = hc[kk] - dtIareaT * (S.S0[k0 + kk][lane] - fw);
                 a.h[c3] = vmax(v, T(0.0));
             }
         }
@@ -935,8 +998,10 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 
     // set_zonal_BT_cont (:920-1121).  The zero-transport correction equals du
     // whenever the target is zero (identical Newton iterations).
-    T du0 = du, du0_eval = du_eval, FAmt_0 = sum_dyhm;
-    if (a.uhbt && block_any<CW>(bt != T(0.0), S.F[2], w, lane)) {   // without uhbt, bt = 0 everywhere
+    const bool bt_any = a.uhbt && block_any<CW>(S.P[2][lane] != T(0.0), S.F[2], w, lane);
+    T du0 = du, du0_eval = S.P[6][lane], FAmt_0 = S.P[7][lane];
+    if (bt_any) {   // without uhbt, bt = 0 everywhere
+        const T bt = S.P[2][lane];
         T de, s0;
         const Start again{S.R[5][lane], S.R[6][lane], S.R[7][lane], S.R[8][lane]};
         const T d2 = flux_adjust(again, T(0.0), false, &de, &s0);
@@ -953,8 +1018,8 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // test velocities duR, duL (visc_rem_lim = 1)
     const T Idt = T(1.0) / (T)a.dt;
     const T CFL_min = (T)1e-6f;
-    const T du_CFL = (CFL_min * Idt) * (T)a.dxCu[c2];
-    __syncthreads();
+    const T du_CFL = (CFL_min * Idt) * S.P[1][lane];
+    lds_barrier();
     if (w == 0) {
         S.R[2][lane] = lds_fold<L>(S.U, nk, lane, vmin(T(0.0), du0 - du_CFL), [&](T x, T uu) {
             return (uu + x > -du_CFL) ? -(uu + du_CFL) : x;
@@ -964,7 +1029,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             return (uu + x < du_CFL) ? -(uu - du_CFL) : x;
         });
     }
-    __syncthreads();
+    lds_barrier();
     const T duR = S.R[2][lane], duL = S.R[3][lane];
     // FAmt_L, uhtot_L at u + duL; FAmt_R, uhtot_R at u + duR
     eval_pass(duL, true);

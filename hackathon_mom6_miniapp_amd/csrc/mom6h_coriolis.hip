@@ -7,7 +7,11 @@
 // The serial code never writes hArea_u on row js-1 nor hArea_v on column
 // is-1 (computed ranges :142-152) and reads them as zero when forming q on the
 // southern row / western column (:162-168); the same zeros are used here.
+// On grids whose metric arrays are constant both stages run as one tiled
+// layer walk instead (k_cor_tile), bit-identical to the two general kernels.
 #include "mom6h_internal.h"
+
+#include <utility>
 
 namespace mom6h {
 
@@ -112,6 +116,172 @@ __global__ void k_cor_ca(Dims d, int scheme, const double *__restrict__ q, const
     }
 }
 
+// ---------------------------------------------------------------------------
+// Grids whose metric arrays are constant (over is-2..ie+2 x js-2..je+2, every
+// point the stencil reads; Area_q over its q points): both stages in one
+// tiled layer walk.  A block owns CT_X x TY points of the compute domain and
+// walks a range of layers; per layer it stages u, v, h, uh, vh of the tile
+// with a ring of one cell below and two above in LDS, forms q on the q points
+// and KE on the h points the tile's CAu/CAv read, then CAu and CAv.  q and KE
+// never leave the chip.  The expressions are k_cor_qke's and k_cor_ca's with
+// the constant metric values, so CAu and CAv are bit-identical to theirs.
+struct CorUniform {
+    double areaT, IareaBu, dyCv, dxCu, dyCu, dxCv, IdxCu, IdyCv, Area_q;
+};
+
+constexpr int CT_X = 64, CW = CT_X + 3;
+
+template <int SCHEME, int TY>
+__global__ void __launch_bounds__(256) k_cor_tile(Dims d, CorUniform g, const double *__restrict__ u,
+                                                  const double *__restrict__ v, const double *__restrict__ h,
+                                                  const double *__restrict__ uh, const double *__restrict__ vh,
+                                                  const double *__restrict__ CoriolisBu,
+                                                  const double *__restrict__ mask2dBu, double *__restrict__ CAu,
+                                                  double *__restrict__ CAv)
+{
+    constexpr int CH = TY + 3, RPT = TY / 4, NS = (CW * CH + 255) / 256;
+    __shared__ double s_u[CH][CW], s_v[CH][CW], s_h[CH][CW], s_uh[CH][CW], s_vh[CH][CW];
+    __shared__ double s_q[CH][CW], s_KE[CH][CW], s_cor[CH][CW], s_mBu[CH][CW];
+    const int nxy = gridDim.x * gridDim.y;
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nxy * gridDim.z);
+    const int xy = tile % nxy, kz = tile / nxy;
+    const int per = (d.nk + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int k0 = 1 + kz * per, k1 = k0 + per - 1 < d.nk ? k0 + per - 1 : d.nk;
+    if (k0 > k1) return;
+    const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
+    const int i0 = is + (xy % (int)gridDim.x) * CT_X, j0 = js + (xy / (int)gridDim.x) * TY;
+    const int bi = i0 - 1, bj = j0 - 1;   // staged region [i0-1, i1+2] x [j0-1, j1+2]
+    const int tid = threadIdx.x, tx = tid % CT_X, ty = tid / CT_X;
+
+    double pu[NS], pv[NS], ph[NS], puh[NS], pvh[NS];
+    auto load_layer = [&](int k) {
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            const int t = tid + 256 * r, li = t % CW, lj = t / CW, i = bi + li, j = bj + lj;
+            pu[r] = 0.0; pv[r] = 0.0; ph[r] = 0.0; puh[r] = 0.0; pvh[r] = 0.0;
+            if (t < CW * CH && i <= d.ied && j <= d.jed) {
+                const long long c = IX3(d, i, j, k);
+                pu[r] = u[c]; pv[r] = v[c]; ph[r] = h[c]; puh[r] = uh[c]; pvh[r] = vh[c];
+            }
+        }
+    };
+    for (int t = tid; t < CW * CH; t += 256) {
+        const int li = t % CW, lj = t / CW, i = bi + li, j = bj + lj;
+        double cor = 0.0, mb = 0.0;
+        if (i <= d.ied && j <= d.jed) {
+            cor = CoriolisBu[IX2(d, i, j)];
+            mb = mask2dBu[IX2(d, i, j)];
+        }
+        s_cor[lj][li] = cor;
+        s_mBu[lj][li] = mb;
+    }
+    load_layer(k0);
+    for (int k = k0; k <= k1; ++k) {
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            const int t = tid + 256 * r;
+            if (t < CW * CH) {
+                const int li = t % CW, lj = t / CW;
+                s_u[lj][li] = pu[r]; s_v[lj][li] = pv[r]; s_h[lj][li] = ph[r];
+                s_uh[lj][li] = puh[r]; s_vh[lj][li] = pvh[r];
+            }
+        }
+        __syncthreads();
+        if (k < k1) load_layer(k + 1);
+        for (int t = tid; t < CW * CH; t += 256) {
+            const int li = t % CW, lj = t / CW, I = bi + li, J = bj + lj;
+            // q on I in is-1..ie, J in js-1..je (:134-168); hArea_u on row
+            // js-1 and hArea_v on column is-1 are the never-written zeros
+            if (li < CW - 1 && lj < CH - 1 && I <= ie && J <= je) {
+                const double dvdx = (s_v[lj][li + 1] * g.dyCv) - (s_v[lj][li] * g.dyCv);
+                const double dudy = (s_u[lj + 1][li] * g.dxCu) - (s_u[lj][li] * g.dxCu);
+                const double hAu0 = J < js ? 0.0 : 0.5 * ((g.areaT * s_h[lj][li]) + (g.areaT * s_h[lj][li + 1]));
+                const double hAu1 = 0.5 * ((g.areaT * s_h[lj + 1][li]) + (g.areaT * s_h[lj + 1][li + 1]));
+                const double hAv0 = I < is ? 0.0 : 0.5 * ((g.areaT * s_h[lj][li]) + (g.areaT * s_h[lj + 1][li]));
+                const double hAv1 = 0.5 * ((g.areaT * s_h[lj][li + 1]) + (g.areaT * s_h[lj + 1][li + 1]));
+                const double rel_vort = s_mBu[lj][li] * (dvdx - dudy) * g.IareaBu;
+                const double abs_vort = s_cor[lj][li] + rel_vort;
+                const double hArea_q = (hAu0 + hAu1) + (hAv0 + hAv1);
+                const double Ih_q = g.Area_q / (hArea_q + 1.0e-20);
+                s_q[lj][li] = abs_vort * Ih_q;
+            }
+            // KE on is..ie x js..je (:196-203)
+            if (li >= 1 && lj >= 1 && I >= is && I <= ie && J >= js && J <= je) {
+                const double uc = s_u[lj][li], uw = s_u[lj][li - 1];
+                const double vc = s_v[lj][li], vs = s_v[lj - 1][li];
+                s_KE[lj][li] = 0.25 * ((g.dyCu * (uc * uc) + g.dyCu * (uw * uw)) +
+                                       (g.dxCv * (vc * vc) + g.dxCv * (vs * vs))) /
+                               g.areaT;
+            }
+        }
+        __syncthreads();
+        const double C1_12 = 1.0 / 12.0, C1_24 = 1.0 / 24.0;
+        auto Q = [&](int pi, int pj) { return s_q[pj][pi]; };
+        // Arakawa & Hsu (1990) / Arakawa & Lamb (1981) coefficients (:171-193)
+        auto A = [&](int pi, int pj) {
+            return SCHEME == 2 ? (Q(pi, pj) + (Q(pi + 1, pj) + Q(pi, pj - 1))) * C1_12
+                               : (2.0 * (Q(pi + 1, pj) + Q(pi, pj - 1)) + (Q(pi, pj) + Q(pi + 1, pj - 1))) * C1_24;
+        };
+        auto D = [&](int pi, int pj) {
+            return SCHEME == 2 ? ((Q(pi, pj) + Q(pi + 1, pj - 1)) + Q(pi, pj - 1)) * C1_12
+                               : ((Q(pi + 1, pj) + Q(pi, pj - 1)) + 2.0 * (Q(pi, pj) + Q(pi + 1, pj - 1))) * C1_24;
+        };
+        auto B = [&](int pi, int pj) {
+            return SCHEME == 2 ? (Q(pi, pj) + (Q(pi - 1, pj) + Q(pi, pj - 1))) * C1_12
+                               : ((Q(pi, pj) + Q(pi - 1, pj - 1)) + 2.0 * (Q(pi - 1, pj) + Q(pi, pj - 1))) * C1_24;
+        };
+        auto Cc = [&](int pi, int pj) {
+            return SCHEME == 2 ? ((Q(pi, pj) + Q(pi - 1, pj - 1)) + Q(pi, pj - 1)) * C1_12
+                               : (2.0 * (Q(pi, pj) + Q(pi - 1, pj - 1)) + (Q(pi - 1, pj) + Q(pi, pj - 1))) * C1_24;
+        };
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = i0 + tx, j = j0 + ty + 4 * r;
+            if (i > ie || j > je) continue;
+            const int li = tx + 1, lj = ty + 4 * r + 1;
+            const long long c = IX3(d, i, j, k);
+            const double KEc = s_KE[lj][li];
+            if (i <= ie - 1) {
+                const double KEx = (s_KE[lj][li + 1] - KEc) * g.IdxCu;
+                if (SCHEME == 1) {   // Sadourny (1975) energy conserving (:206-216)
+                    CAu[c] = 0.25 * ((Q(li, lj) * (s_vh[lj][li + 1] + s_vh[lj][li])) +
+                                     (Q(li, lj - 1) * (s_vh[lj - 1][li] + s_vh[lj - 1][li + 1]))) *
+                                 g.IdxCu - KEx;
+                } else {             // (:229-237)
+                    CAu[c] = (((A(li, lj) * s_vh[lj][li + 1]) + (Cc(li, lj) * s_vh[lj - 1][li])) +
+                              ((B(li, lj) * s_vh[lj][li]) + (D(li, lj) * s_vh[lj - 1][li + 1]))) *
+                                 g.IdxCu - KEx;
+                }
+            }
+            if (j <= je - 1) {
+                const double KEy = (s_KE[lj + 1][li] - KEc) * g.IdyCv;
+                if (SCHEME == 1) {   // (:218-226)
+                    CAv[c] = -(0.25 * ((Q(li - 1, lj) * (s_uh[lj][li - 1] + s_uh[lj + 1][li - 1])) +
+                                       (Q(li, lj) * (s_uh[lj][li] + s_uh[lj + 1][li]))) *
+                               g.IdyCv) - KEy;
+                } else {             // (:239-247)
+                    CAv[c] = -((((A(li - 1, lj) * s_uh[lj][li - 1]) + (Cc(li, lj + 1) * s_uh[lj + 1][li])) +
+                                ((B(li, lj) * s_uh[lj][li]) + (D(li - 1, lj + 1) * s_uh[lj + 1][li - 1]))) *
+                               g.IdyCv) - KEy;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int SCHEME>
+static void launch_cor_tile(const mom6h_coriolis_cs *cs, const CorUniform &g, const double *u, const double *v,
+                            const double *h, const double *uh, const double *vh, double *CAu, double *CAv,
+                            hipStream_t s)
+{
+    const Dims &d = cs->d;
+    constexpr int TY = 8;
+    const int nkz = (d.nk + 14) / 15;
+    const dim3 grid((unsigned)((d.iec - d.isc + CT_X) / CT_X), (unsigned)((d.jec - d.jsc + TY) / TY), (unsigned)nkz);
+    k_cor_tile<SCHEME, TY><<<grid, 256, 0, s>>>(d, g, u, v, h, uh, vh, cs->CoriolisBu, cs->mask2dBu, CAu, CAv);
+}
+
 }  // namespace mom6h
 
 using namespace mom6h;
@@ -155,6 +325,17 @@ int mom6h_coriolis_init(mom6h_coriolis_cs **out, int isd, int ied, int jsd, int 
     rc = rc ? rc : dev_upload(&cs->Area_q, aq, n2);
     rc = rc ? rc : dev_alloc(&cs->q, n3, true);
     rc = rc ? rc : dev_alloc(&cs->KE, n3, true);
+    {   // constant metrics over the points the stencil reads
+        const int ilo = d.isc - 2, ihi = d.iec + 2, jlo = d.jsc - 2, jhi = d.jec + 2;
+        bool ok = true;
+        for (auto f : {std::make_pair(areaT, &cs->u_areaT), std::make_pair(IareaBu, &cs->u_IareaBu),
+                       std::make_pair(dyCv, &cs->u_dyCv), std::make_pair(dxCu, &cs->u_dxCu),
+                       std::make_pair(dyCu, &cs->u_dyCu), std::make_pair(dxCv, &cs->u_dxCv),
+                       std::make_pair(IdxCu, &cs->u_IdxCu), std::make_pair(IdyCv, &cs->u_IdyCv)})
+            ok = ok && uniform_over(d, f.first, ilo, ihi, jlo, jhi, f.second);
+        ok = ok && uniform_over(d, aq, d.isc - 1, d.iec, d.jsc - 1, d.jec, &cs->u_Area_q);
+        cs->uniform = ok;
+    }
     free(aq);
     if (rc) {
         mom6h_coriolis_end(cs);
@@ -171,6 +352,15 @@ int mom6h_coradcalc(mom6h_coriolis_cs *cs, const double *u, const double *v, con
     MOM6H_REQUIRE(u && v && h && uh && vh && CAu && CAv, "mom6h_coradcalc: NULL array");
     const Dims &d = cs->d;
     hipStream_t s = (hipStream_t)stream;
+    if (cs->uniform && !cs->general) {
+        const CorUniform g{cs->u_areaT, cs->u_IareaBu, cs->u_dyCv, cs->u_dxCu, cs->u_dyCu,
+                           cs->u_dxCv, cs->u_IdxCu, cs->u_IdyCv, cs->u_Area_q};
+        if (cs->scheme == 1) launch_cor_tile<1>(cs, g, u, v, h, uh, vh, CAu, CAv, s);
+        else if (cs->scheme == 2) launch_cor_tile<2>(cs, g, u, v, h, uh, vh, CAu, CAv, s);
+        else launch_cor_tile<3>(cs, g, u, v, h, uh, vh, CAu, CAv, s);
+        MOM6H_LAUNCH_CHECK();
+        return MOM6H_OK;
+    }
     {
         const int ni = d.iec - (d.isc - 1) + 1, nj = d.jec - (d.jsc - 1) + 1;
         dim3 b(64), g((ni + 63) / 64, nj, d.nk);

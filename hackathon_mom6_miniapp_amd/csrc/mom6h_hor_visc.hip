@@ -5,7 +5,9 @@
 // no-slip; and (k_hv_closures, further below) the Smagorinsky and Leith
 // viscosities with the thickness-aware bounds.  One 3D launch computes every layer: each thread forms the
 // strain/stress values its diffu/diffv stencil needs directly from u, v, h
-// (no 2D work arrays, no per-layer launches).
+// (no 2D work arrays, no per-layer launches).  On grids whose metric arrays
+// are constant the constant-coefficient operator runs as a tiled layer walk
+// with LDS staging instead (k_hv_tile), bit-identical to the general kernel.
 //
 // The serial code keeps ONE 2D array for Kh (and Ah) at h- and q-points: the
 // h-point pass writes is..ie x js..je but the stress uses is-1..ie+1 x
@@ -16,6 +18,9 @@
 // values only reach diffu on column is-1 and ie and diffv on rows js-1 and je
 // (biharmonic: also the first interior column/row), and are reproduced here.
 #include "mom6h_internal.h"
+
+#include <initializer_list>
+#include <utility>
 
 namespace mom6h {
 
@@ -162,6 +167,146 @@ __global__ void k_hv_main(HV p, const double *__restrict__ u, const double *__re
     }
     diffu[c] = du;
     diffv[c] = dv;
+}
+
+// ---------------------------------------------------------------------------
+// The constant-coefficient operator on a grid whose metric arrays are constant
+// (over is-2..ie+2 x js-2..je+2, every point the stencil reads): a tiled layer
+// walk.  A block owns HT_X x TY points of the data domain and walks a range of
+// layers.  Per layer it stages u, v, h (and Del2u, Del2v) of the tile and a
+// one-cell ring in LDS, forms str_xx*h at every staged h point and str_xy*hq
+// at every staged q point once, then diffu and diffv of its points (zero
+// outside their ranges).  The masks stay in LDS for the whole walk, and the
+// next layer's loads are in flight while the current one is computed.  The
+// expressions are k_hv_main's with the constant metric values, so the results
+// are bit-identical to it.
+constexpr int HT_X = 64, HW = HT_X + 2;
+
+template <bool BIH, int TY>
+__global__ void __launch_bounds__(256) k_hv_tile(HV p, HVUniform g, const double *__restrict__ u,
+                                                 const double *__restrict__ v, const double *__restrict__ h,
+                                                 double *__restrict__ diffu, double *__restrict__ diffv)
+{
+    constexpr int MH = TY + 2, RPT = TY / 4, NS = (HW * MH + 255) / 256;
+    __shared__ double s_u[MH][HW], s_v[MH][HW], s_h[MH][HW], s_mT[MH][HW], s_mBu[MH][HW];
+    __shared__ double s_xx[MH][HW], s_xy[MH][HW];
+    __shared__ double s_2u[BIH ? MH : 1][BIH ? HW : 1], s_2v[BIH ? MH : 1][BIH ? HW : 1];
+    const Dims &d = p.d;
+    const int nxy = gridDim.x * gridDim.y;
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nxy * gridDim.z);
+    const int xy = tile % nxy, kz = tile / nxy;
+    const int per = (d.nk + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int k0 = 1 + kz * per, k1 = k0 + per - 1 < d.nk ? k0 + per - 1 : d.nk;
+    if (k0 > k1) return;
+    const int i0 = d.isd + (xy % (int)gridDim.x) * HT_X, j0 = d.jsd + (xy / (int)gridDim.x) * TY;
+    const int bi = i0 - 1, bj = j0 - 1;
+    const int tid = threadIdx.x, tx = tid % HT_X, ty = tid / HT_X;
+
+    double pu[NS], pv[NS], ph[NS], p2u[BIH ? NS : 1], p2v[BIH ? NS : 1];
+    auto load_layer = [&](int k) {
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            const int t = tid + 256 * r, li = t % HW, lj = t / HW, i = bi + li, j = bj + lj;
+            pu[r] = 0.0; pv[r] = 0.0; ph[r] = 0.0;
+            if (BIH) { p2u[BIH ? r : 0] = 0.0; p2v[BIH ? r : 0] = 0.0; }
+            if (t < HW * MH && i >= d.isd && i <= d.ied && j >= d.jsd && j <= d.jed) {
+                const long long c = IX3(d, i, j, k);
+                pu[r] = u[c]; pv[r] = v[c]; ph[r] = h[c];
+                if (BIH) { p2u[BIH ? r : 0] = p.Del2u[c]; p2v[BIH ? r : 0] = p.Del2v[c]; }
+            }
+        }
+    };
+    for (int t = tid; t < HW * MH; t += 256) {
+        const int li = t % HW, lj = t / HW, i = bi + li, j = bj + lj;
+        double mt = 0.0, mb = 0.0;
+        if (i >= d.isd && i <= d.ied && j >= d.jsd && j <= d.jed) {
+            mt = p.mask2dT[IX2(d, i, j)];
+            mb = p.mask2dBu[IX2(d, i, j)];
+        }
+        s_mT[lj][li] = mt;
+        s_mBu[lj][li] = mb;
+    }
+    const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
+    load_layer(k0);
+    for (int k = k0; k <= k1; ++k) {
+#pragma unroll
+        for (int r = 0; r < NS; ++r) {
+            const int t = tid + 256 * r;
+            if (t < HW * MH) {
+                const int li = t % HW, lj = t / HW;
+                s_u[lj][li] = pu[r]; s_v[lj][li] = pv[r]; s_h[lj][li] = ph[r];
+                if (BIH) { s_2u[BIH ? lj : 0][BIH ? li : 0] = p2u[BIH ? r : 0]; s_2v[BIH ? lj : 0][BIH ? li : 0] = p2v[BIH ? r : 0]; }
+            }
+        }
+        __syncthreads();
+        if (k < k1) load_layer(k + 1);
+        // Kh and Ah at the h points of the stress ring: what the previous
+        // layer's q-point pass left there, zero on the first layer of the
+        // first call (visc_h)
+        const bool ring_zero = k == 1 && p.first_call;
+        for (int t = tid; t < HW * MH; t += 256) {
+            const int li = t % HW, lj = t / HW, i = bi + li, j = bj + lj;
+            if (li >= 1 && lj >= 1) {   // str_xx * h at h point (i, j) (:807-895)
+                const double dudx = g.DY_dxT * ((g.IdyCu * s_u[lj][li]) - (g.IdyCu * s_u[lj][li - 1]));
+                const double dvdy = g.DX_dyT * ((g.IdxCv * s_v[lj][li]) - (g.IdxCv * s_v[lj - 1][li]));
+                const double sh = dudx - dvdy;
+                const bool in = i >= is && i <= ie && j >= js && j <= je;
+                const bool ring = !in && i >= is - 1 && i <= ie && j >= js - 1 && j <= je;
+                const bool on = in || (ring && !ring_zero);
+                const double Kh = on ? p.Kh : 0.0;
+                double str = p.laplacian ? -(Kh * sh) : 0.0;
+                if (BIH) {
+                    const int qi = BIH ? li : 0, qj = BIH ? lj : 0;
+                    const double d_del2u = (g.IdyCu * s_2u[qj][qi]) - (g.IdyCu * s_2u[qj][BIH ? qi - 1 : 0]);
+                    const double d_del2v = (g.IdxCv * s_2v[qj][qi]) - (g.IdxCv * s_2v[BIH ? qj - 1 : 0][qi]);
+                    const double Ah = on ? p.Ah : 0.0;
+                    const double d_str = Ah * ((g.DY_dxT * d_del2u) - (g.DX_dyT * d_del2v));
+                    str = str + d_str;
+                }
+                s_xx[lj][li] = str * (s_h[lj][li] * g.red_xx);
+            }
+            if (li < HW - 1 && lj < MH - 1) {   // str_xy * hq at q point (i, j) (:954-1053)
+                const double dvdx = g.DY_dxBu * ((s_v[lj][li + 1] * g.IdyCv) - (s_v[lj][li] * g.IdyCv));
+                const double dudy = g.DX_dyBu * ((s_u[lj + 1][li] * g.IdxCu) - (s_u[lj][li] * g.IdxCu));
+                const double m = s_mBu[lj][li];
+                const double sh = (p.no_slip ? (2.0 - m) : m) * (dvdx + dudy);
+                double str = p.laplacian ? -(p.Kh * sh) : 0.0;
+                if (BIH) {
+                    const int qi = BIH ? li : 0, qj = BIH ? lj : 0;
+                    const double d_str =
+                        p.Ah * ((g.DY_dxBu * ((s_2v[qj][BIH ? qi + 1 : 0] * g.IdyCv) - (s_2v[qj][qi] * g.IdyCv))) +
+                                (g.DX_dyBu * ((s_2u[BIH ? qj + 1 : 0][qi] * g.IdxCu) - (s_2u[qj][qi] * g.IdxCu))));
+                    str = str + d_str;
+                }
+                const double hq = 0.25 * ((s_h[lj][li] + s_h[lj + 1][li + 1]) + (s_h[lj][li + 1] + s_h[lj + 1][li]));
+                s_xy[lj][li] = p.no_slip ? str * (hq * g.red_xy) : str * (hq * m * g.red_xy);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int i = i0 + tx, j = j0 + ty + 4 * r;
+            if (i > d.ied || j > d.jed) continue;
+            const int li = tx + 1, lj = ty + 4 * r + 1;
+            double du = 0.0, dv = 0.0;
+            if (i >= is - 1 && i <= ie && j >= js && j <= je) {   // (:1059-1067)
+                const double h_u = 0.5 * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj][li + 1] * s_h[lj][li + 1]);
+                du = ((g.IdxCu * ((g.dx2q * s_xy[lj - 1][li]) - (g.dx2q * s_xy[lj][li])) +
+                       g.IdyCu * ((g.dy2h * s_xx[lj][li]) - (g.dy2h * s_xx[lj][li + 1]))) *
+                      g.IareaCu) / (h_u + p.h_neglect);
+            }
+            if (i >= is && i <= ie && j >= js - 1 && j <= je) {   // (:1070-1078)
+                const double h_v = 0.5 * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj + 1][li] * s_h[lj + 1][li]);
+                dv = ((g.IdyCv * ((g.dy2q * s_xy[lj][li - 1]) - (g.dy2q * s_xy[lj][li])) -
+                       g.IdxCv * ((g.dx2h * s_xx[lj][li]) - (g.dx2h * s_xx[lj + 1][li]))) *
+                      g.IareaCv) / (h_v + p.h_neglect);
+            }
+            const long long c = IX3(d, i, j, k);
+            diffu[c] = du;
+            diffv[c] = dv;
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -471,6 +616,27 @@ static HV make_hv(const mom6h_hor_visc_cs *cs)
 
 static bool closures_on(const mom6h_hor_visc_cs *cs) { return cs->smagorinsky || cs->leith || cs->better_bound; }
 
+// the tiled kernel's launch: 8-row tiles over the data domain, the layers
+// split so that a block walks about 15 of them
+static void launch_tile(const mom6h_hor_visc_cs *cs, const HV &p, const double *u, const double *v,
+                        const double *h, double *diffu, double *diffv, hipStream_t s)
+{
+    const Dims &d = cs->d;
+    constexpr int TY = 8;
+    const int nkz = (d.nk + 14) / 15;
+    const dim3 grid((unsigned)((d.n1 + HT_X - 1) / HT_X), (unsigned)((d.n2 + TY - 1) / TY), (unsigned)nkz);
+    if (cs->biharmonic) k_hv_tile<true, TY><<<grid, 256, 0, s>>>(p, cs->g, u, v, h, diffu, diffv);
+    else k_hv_tile<false, TY><<<grid, 256, 0, s>>>(p, cs->g, u, v, h, diffu, diffv);
+}
+
+// every metric the stencil reads constant over is-2..ie+2 x js-2..je+2
+static bool hv_uniform(const Dims &d, std::initializer_list<std::pair<const double *, double *>> fields)
+{
+    for (const auto &f : fields)
+        if (!uniform_over(d, f.first, d.isc - 2, d.iec + 2, d.jsc - 2, d.jec + 2, f.second)) return false;
+    return true;
+}
+
 }  // namespace mom6h
 
 using namespace mom6h;
@@ -515,6 +681,13 @@ int mom6h_hor_visc_init(mom6h_hor_visc_cs **out, int isd, int ied, int jsd, int 
         mom6h_hor_visc_end(cs);
         return rc;
     }
+    HVUniform &g = cs->g;
+    cs->uniform = hv_uniform(cs->d, {{DY_dxT, &g.DY_dxT}, {DX_dyT, &g.DX_dyT}, {DY_dxBu, &g.DY_dxBu},
+                                     {DX_dyBu, &g.DX_dyBu}, {IdyCu, &g.IdyCu}, {IdxCu, &g.IdxCu},
+                                     {IdyCv, &g.IdyCv}, {IdxCv, &g.IdxCv}, {IareaCu, &g.IareaCu},
+                                     {IareaCv, &g.IareaCv}, {reduction_xx, &g.red_xx},
+                                     {reduction_xy, &g.red_xy}, {dy2h, &g.dy2h}, {dx2h, &g.dx2h},
+                                     {dy2q, &g.dy2q}, {dx2q, &g.dx2q}});
     *out = cs;
     return MOM6H_OK;
 }
@@ -539,6 +712,9 @@ int mom6h_hor_visc_set_options(mom6h_hor_visc_cs *cs, int laplacian, int biharmo
         rc = rc ? rc : dev_alloc(&cs->Del2u, n3, true);
         rc = rc ? rc : dev_alloc(&cs->Del2v, n3, true);
         if (rc) return rc;
+        HVUniform &g = cs->g;
+        cs->bih_uniform = hv_uniform(cs->d, {{Idx2dyCu, &g.Idx2dyCu}, {Idxdy2u, &g.Idxdy2u},
+                                             {Idx2dyCv, &g.Idx2dyCv}, {Idxdy2v, &g.Idxdy2v}});
     }
     cs->biharmonic = biharmonic;
     return MOM6H_OK;
@@ -564,6 +740,8 @@ int mom6h_hor_visc(mom6h_hor_visc_cs *cs, const double *u, const double *v, cons
         MOM6H_LAUNCH_CHECK();
         const int nring = (d.jec - d.jsc + 2) + (d.iec - d.isc + 1);
         k_hv_keep_ring<<<(nring + 63) / 64, 64, 0, s>>>(p, u, v, h);
+    } else if (cs->uniform && (!cs->biharmonic || cs->bih_uniform) && !cs->general) {
+        launch_tile(cs, p, u, v, h, diffu, diffv, s);
     } else {
         k_hv_main<<<g, b, 0, s>>>(p, u, v, h, diffu, diffv);
     }

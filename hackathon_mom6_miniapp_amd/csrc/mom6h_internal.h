@@ -43,6 +43,20 @@ __host__ __device__ inline long long IX3(const Dims &d, int i, int j, int k)
     return ((long long)(k - 1) * d.n2 + (j - d.jsd)) * d.n1 + (i - d.isd);
 }
 
+// true and the common value when the 2D host array a holds one value on
+// ilo..ihi x jlo..jhi (clipped to the data domain)
+inline bool uniform_over(const Dims &d, const double *a, int ilo, int ihi, int jlo, int jhi, double *v)
+{
+    ilo = ilo < d.isd ? d.isd : ilo; ihi = ihi > d.ied ? d.ied : ihi;
+    jlo = jlo < d.jsd ? d.jsd : jlo; jhi = jhi > d.jed ? d.jed : jhi;
+    const double a0 = a[IX2(d, ilo, jlo)];
+    for (int j = jlo; j <= jhi; ++j)
+        for (int i = ilo; i <= ihi; ++i)
+            if (!(a[IX2(d, i, j)] == a0)) return false;
+    *v = a0;
+    return true;
+}
+
 // Block-to-tile mapping for the 8 XCDs of MI355X.  The hardware hands
 // consecutive workgroups to the XCDs in turn, each XCD with its own L2; this
 // renumbers them so that XCD x works on one contiguous range of tiles, and
@@ -292,6 +306,12 @@ struct mom6h_coriolis_cs {
     double *areaT, *IareaBu, *CoriolisBu, *mask2dBu, *dyCv, *dxCu, *dyCu, *dxCv, *IdxCu, *IdyCv;
     double *Area_q;          // sum of the 4 surrounding cell areas
     double *q, *KE;          // 3D potential vorticity / kinetic energy scratch
+    // every metric constant over the points the stencil reads: the values
+    // (the tiled kernel), else the two general kernels
+    int uniform = 0;
+    int general = 0;         // use the general kernels even on a uniform grid
+    double u_areaT = 0, u_IareaBu = 0, u_dyCv = 0, u_dxCu = 0, u_dyCu = 0, u_dxCv = 0, u_IdxCu = 0, u_IdyCv = 0;
+    double u_Area_q = 0;
 };
 
 struct mom6h_barotropic_cs {
@@ -327,6 +347,15 @@ struct mom6h_vert_visc_cs {
     double *taux_bot, *tauy_bot;
 };
 
+namespace mom6h {
+// hor_visc's metric values on a grid whose metric arrays are constant
+struct HVUniform {
+    double DY_dxT, DX_dyT, DY_dxBu, DX_dyBu, IdyCu, IdxCu, IdyCv, IdxCv, IareaCu, IareaCv;
+    double red_xx, red_xy, dy2h, dx2h, dy2q, dx2q;
+    double Idx2dyCu, Idxdy2u, Idx2dyCv, Idxdy2v;
+};
+}  // namespace mom6h
+
 struct mom6h_hor_visc_cs {
     mom6h::Dims d;
     double Kh, Ah, h_neglect;
@@ -344,4 +373,10 @@ struct mom6h_hor_visc_cs {
     // q-point Kh and Ah of the last layer of the previous call, read by the
     // first layer's stress ring (zero before the first call)
     double *ring_Kh = nullptr, *ring_Ah = nullptr;
+    // every metric constant over the points the stencil reads (is-2..ie+2 x
+    // js-2..je+2): the values, for the tiled kernel; bih_uniform likewise for
+    // the four biharmonic metrics
+    int uniform = 0, bih_uniform = 0;
+    int general = 0;         // use the general kernel even on a uniform grid
+    mom6h::HVUniform g{};
 };

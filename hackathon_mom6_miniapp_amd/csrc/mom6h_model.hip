@@ -808,6 +808,11 @@ int mom6h_model_set_option(mom6h_model *m, const char *key, double value)
         }
         drop_graphs(m);   // the captured step holds the old launch shape
     }
+    else if (k == "general") {   // module hor_visc and CorAdCalc: the general kernels on any grid
+        if (m->hv) m->hv->general = value != 0.0;
+        if (m->cor) m->cor->general = value != 0.0;
+        drop_graphs(m);
+    }
     else if (k == "fused") {
         MOM6H_REQUIRE(!m->mixed || value != 0.0, "mom6h_model_set_option: the mixed-precision form is fused only");
         m->fused = value != 0.0;

@@ -151,6 +151,12 @@ module mom6_hip_capi
             import :: c_int, c_ptr
             type(c_ptr), value :: cs
         end function
+        integer(c_int) function mom6h_vert_visc_keep_coefficients(cs, keep) &
+                bind(c, name='mom6h_vert_visc_keep_coefficients')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: cs
+            integer(c_int), value :: keep
+        end function
 
         integer(c_int) function mom6h_hor_visc_init(cs, isd, ied, jsd, jed, isc, iec, jsc, jec, nz, Kh, &
                 h_neglect, DY_dxT, DX_dyT, DY_dxBu, DX_dyBu, IdyCu, IdxCu, IdyCv, IdxCv, IareaCu, IareaCv, &
@@ -560,6 +566,7 @@ module mom6_vert_visc_hip
     implicit none
     private
     public :: vert_visc_CS_hip, vert_visc_init_hip, vert_visc_cra_hip, vert_visc_end_hip
+    public :: vert_visc_keep_coefficients_hip
 
     type :: vert_visc_CS_hip
         type(c_ptr) :: handle = c_null_ptr
@@ -593,6 +600,18 @@ contains
         if (present(ray_v_p)) rv = ray_v_p
         if (present(stream)) st = stream
         call hip_check(mom6h_vert_visc(CS%handle, u_p, v_p, h_p, dt, taux_p, tauy_p, ru, rv, st), 'vert_visc_cra_hip')
+    end subroutine
+
+    !> keep = .false.: the coupling coefficients and effective thicknesses
+    !! (the control structure's a_u, a_v, h_u, h_v) are not stored, for callers
+    !! that never read them
+    subroutine vert_visc_keep_coefficients_hip(CS, keep)
+        type(vert_visc_CS_hip), intent(inout) :: CS
+        logical, intent(in) :: keep
+        integer(c_int) :: k
+        k = 0
+        if (keep) k = 1
+        call hip_check(mom6h_vert_visc_keep_coefficients(CS%handle, k), 'vert_visc_keep_coefficients_hip')
     end subroutine
 
     subroutine vert_visc_end_hip(CS)

@@ -168,6 +168,7 @@ program rk2_hip_driver
                              gtot, gtot, gtot, gtot, f_4, f_4, rem, rem, IareaT, Idxv, Idxv)
     call vert_visc_init_hip(visc_CS, isd, ied, jsd, jed, isc, iec, jsc, jec, nk, one, one, 1.0e-4_dp, 1.0e-2_dp, &
                             1.0e-2_dp, 50.0_dp, 10.0_dp)
+    call vert_visc_keep_coefficients_hip(visc_CS, .false.)   ! the driver never reads a_u, h_u, ...
     call hor_visc_init_hip(hvisc_CS, isd, ied, jsd, jed, isc, iec, jsc, jec, nk, 100.0_dp, 1.0e-3_dp, ratio, ratio, &
                            ratio, ratio, Idxv, Idxv, Idxv, Idxv, IareaT, IareaT, one, one, one, one, dx2, dx2, &
                            dx2, dx2)

@@ -11,7 +11,7 @@ from tests.cases import CASES, CLOSURE_CASES, COMPARE, COMPARE_FUSED, MERID_CASE
 pytestmark = pytest.mark.gpu
 
 
-def _run_pair(name, fused=True):
+def _run_pair(name, fused=True, general=False):
     from hackathon_mom6_miniapp_amd.model import Model
     from oracle.pyoracle import Oracle
 
@@ -19,6 +19,7 @@ def _run_pair(name, fused=True):
     case = make_case(cfg, init=init, seed=11)
     gpu = Model(cfg, case)
     gpu.set_option("fused", 1.0 if fused else 0.0)
+    gpu.set_option("general", 1.0 if general else 0.0)
     gpu.step(niter)
     gpu.synchronize()
     ora = Oracle(cfg, case)
@@ -26,13 +27,16 @@ def _run_pair(name, fused=True):
     return gpu, ora
 
 
-@pytest.mark.parametrize("mode", ["modules", "fused"])
+@pytest.mark.parametrize("mode", ["modules", "general", "fused"])
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_model_bit_exact(name, mode):
+    """modules: the module entry points (on these uniform grids hor_visc and
+    CorAdCalc take their tiled kernels); general: the same with the general
+    hor_visc and CorAdCalc kernels; fused: the fused momentum stencil."""
     print(f"[gpu parity] case={name} mode={mode}", flush=True)
-    gpu, ora = _run_pair(name, fused=(mode == "fused"))
+    gpu, ora = _run_pair(name, fused=(mode == "fused"), general=(mode == "general"))
     bad = []
-    fields = COMPARE if mode == "modules" else COMPARE_FUSED + ("uh",)
+    fields = COMPARE if mode != "fused" else COMPARE_FUSED + ("uh",)
     cfg = CASES[name][0]
     isd, ied, jsd, jed, isc, iec, jsc, jec = cfg.bounds
     for f in fields:

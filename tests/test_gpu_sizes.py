@@ -81,8 +81,9 @@ SEAM_CASES = {
 @pytest.mark.parametrize("mode", ["modules", "fused"])
 @pytest.mark.parametrize("name", sorted(SEAM_CASES))
 def test_tile_seams_bit_exact(name, mode, merid):
-    """200 x 140 columns: 4 momentum tiles, 4 continuity blocks and 4
-    barotropic tiles in x (5 in y), three steps."""
+    """200 x 140 columns: 4 momentum tiles (modules: 4 hor_visc and CorAdCalc
+    tiles), 4 continuity blocks and 4 barotropic tiles in x (5 or more in y),
+    three steps."""
     from hackathon_mom6_miniapp_amd.model import Model
     from oracle.pyoracle import Oracle
 
