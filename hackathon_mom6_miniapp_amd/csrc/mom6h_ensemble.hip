@@ -372,11 +372,10 @@ int mom6h_ensemble_step(mom6h_ensemble *e, int niter)
     }
     MOM6H_CHECK(hipEventRecord(e->e_start, e->s0));
     MOM6H_CHECK(hipStreamWaitEvent(e->s1, e->e_start, 0));
+    // enqueued in turn, so that s0 has work while the host still enqueues s1's
     for (int it = 0; it < niter; ++it) {
         if (e->use_graph) MOM6H_CHECK(hipGraphLaunch(e->gexec_bt, e->s1));
         else RC(enqueue_bt(e));
-    }
-    for (int it = 0; it < niter; ++it) {
         if (e->use_graph) MOM6H_CHECK(hipGraphLaunch(e->gexec, e->s0));
         else RC(enqueue_3d_step(e));
     }

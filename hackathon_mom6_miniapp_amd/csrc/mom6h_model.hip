@@ -726,8 +726,10 @@ int mom6h_model_step(mom6h_model *m, int niter)
         }
         MOM6H_CHECK(hipEventRecord(m->e_start, m->s0));
         MOM6H_CHECK(hipStreamWaitEvent(m->s1, m->e_start, 0));
-        for (int it = 0; it < niter; ++it) MOM6H_CHECK(hipGraphLaunch(m->gexec_bt, m->s1));
-        for (int it = 0; it < niter; ++it) MOM6H_CHECK(hipGraphLaunch(m->gexec, m->s0));
+        for (int it = 0; it < niter; ++it) {
+            MOM6H_CHECK(hipGraphLaunch(m->gexec_bt, m->s1));
+            MOM6H_CHECK(hipGraphLaunch(m->gexec, m->s0));
+        }
         MOM6H_CHECK(hipEventRecord(m->e_bt, m->s1));
         MOM6H_CHECK(hipStreamWaitEvent(m->s0, m->e_bt, 0));
         return MOM6H_OK;
@@ -742,10 +744,17 @@ int mom6h_model_step(mom6h_model *m, int niter)
     // The barotropic chain and the 3D chain share no array (the 2D fields the
     // barotropic solver reads and writes are never touched by the 3D stages),
     // so s1 runs all its steps without waiting for s0; the streams join once.
+    // The steps are enqueued in turn, a barotropic step then a 3D step: the
+    // host takes about as long to enqueue a barotropic step (some thirty
+    // launches) as the GPU to run it, so enqueueing all barotropic steps
+    // first would leave s0 idle until the last was enqueued, and the two
+    // chains would run one after the other instead of side by side.
     MOM6H_CHECK(hipEventRecord(m->e_start, m->s0));
     MOM6H_CHECK(hipStreamWaitEvent(m->s1, m->e_start, 0));
-    for (int it = 0; it < niter; ++it) RC(enqueue_barotropic(m));
-    for (int it = 0; it < niter; ++it) RC(enqueue_step(m));
+    for (int it = 0; it < niter; ++it) {
+        RC(enqueue_barotropic(m));
+        RC(enqueue_step(m));
+    }
     MOM6H_CHECK(hipEventRecord(m->e_bt, m->s1));
     MOM6H_CHECK(hipStreamWaitEvent(m->s0, m->e_bt, 0));
     return MOM6H_OK;

@@ -205,21 +205,48 @@ int dev_alloc_bytes(void **p, size_t bytes, bool zero)
 // --------------------------------------------------------------- transports
 // compute_transports, app/rk2_serial_driver.F90:544-561:
 //   uh = u*0.5*(h + h(min(i+1,ied)))*dyCu ; vh = v*0.5*(h + h(i,min(j+1,jed)))*dxCv
-__global__ void k_transports(Dims d, const double *__restrict__ u, const double *__restrict__ v,
-                             const double *__restrict__ h, const double *__restrict__ dyCu,
-                             const double *__restrict__ dxCv, double *__restrict__ uh,
-                             double *__restrict__ vh)
+// A block owns 64 x 8 columns (64 x 4 threads, two rows each) and walks a
+// range of layers: the 2D metrics of its columns are loaded once, and the
+// h(i, j+1) row it reads is the next thread row's own, so it comes from the
+// CU's cache instead of a second fetch by a block on another XCD.
+constexpr int TR_X = 64, TR_Y = 8;
+
+__global__ void __launch_bounds__(256) k_transports(Dims d, const double *__restrict__ u,
+                                                    const double *__restrict__ v, const double *__restrict__ h,
+                                                    const double *__restrict__ dyCu,
+                                                    const double *__restrict__ dxCv, double *__restrict__ uh,
+                                                    double *__restrict__ vh)
 {
-    const int i = d.isd + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = d.jsd + blockIdx.y;
-    const int k = 1 + blockIdx.z;
+    const int nxy = gridDim.x * gridDim.y;
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nxy * gridDim.z);
+    const int xy = tile % nxy, kz = tile / nxy;
+    const int per = (d.nk + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int k0 = 1 + kz * per, k1 = k0 + per - 1 < d.nk ? k0 + per - 1 : d.nk;
+    const int i = d.isd + (xy % (int)gridDim.x) * TR_X + (int)(threadIdx.x % TR_X);
+    const int jt = d.jsd + (xy / (int)gridDim.x) * TR_Y + (int)(threadIdx.x / TR_X);
     if (i > d.ied) return;
     const int ip = i + 1 < d.ied ? i + 1 : d.ied;
-    const int jp = j + 1 < d.jed ? j + 1 : d.jed;
-    const long long c = IX3(d, i, j, k);
-    const double hc = h[c];
-    uh[c] = u[c] * 0.5 * (hc + h[IX3(d, ip, j, k)]) * dyCu[IX2(d, i, j)];
-    vh[c] = v[c] * 0.5 * (hc + h[IX3(d, i, jp, k)]) * dxCv[IX2(d, i, j)];
+    double my[2] = {0.0, 0.0}, mx[2] = {0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int j = jt + 4 * r;
+        if (j <= d.jed) {
+            my[r] = dyCu[IX2(d, i, j)];
+            mx[r] = dxCv[IX2(d, i, j)];
+        }
+    }
+    for (int k = k0; k <= k1; ++k) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int j = jt + 4 * r;
+            if (j > d.jed) continue;
+            const int jp = j + 1 < d.jed ? j + 1 : d.jed;
+            const long long c = IX3(d, i, j, k);
+            const double hc = h[c];
+            uh[c] = u[c] * 0.5 * (hc + h[IX3(d, ip, j, k)]) * my[r];
+            vh[c] = v[c] * 0.5 * (hc + h[IX3(d, i, jp, k)]) * mx[r];
+        }
+    }
 }
 
 // ---------------------------------------------------------- velocity update
@@ -347,8 +374,9 @@ int mom6h_transports(int isd, int ied, int jsd, int jed, int nk, const double *u
     MOM6H_REQUIRE(ied >= isd && jed >= jsd && nk >= 1, "mom6h_transports: bad bounds");
     MOM6H_REQUIRE(u && v && h && dyCu && dxCv && uh && vh, "mom6h_transports: NULL array");
     Dims d = make_dims(isd, ied, jsd, jed, isd, ied, jsd, jed, nk);
-    dim3 b(128), g((d.n1 + 127) / 128, d.n2, nk);
-    k_transports<<<g, b, 0, (hipStream_t)stream>>>(d, u, v, h, dyCu, dxCv, uh, vh);
+    const dim3 g((unsigned)((d.n1 + TR_X - 1) / TR_X), (unsigned)((d.n2 + TR_Y - 1) / TR_Y),
+                 (unsigned)((nk + 14) / 15));   // about 15 layers per block
+    k_transports<<<g, 256, 0, (hipStream_t)stream>>>(d, u, v, h, dyCu, dxCv, uh, vh);
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
 }
