@@ -151,6 +151,12 @@ module mom6_hip_capi
             import :: c_int, c_ptr
             type(c_ptr), value :: cs
         end function
+        integer(c_int) function mom6h_vert_visc_ray3d(cs, u, v, h, dt, taux, tauy, ray_u, ray_v, stream) &
+                bind(c, name='mom6h_vert_visc_ray3d')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: cs, u, v, h, taux, tauy, ray_u, ray_v, stream
+            real(c_double), value :: dt
+        end function
         integer(c_int) function mom6h_vert_visc_keep_coefficients(cs, keep) &
                 bind(c, name='mom6h_vert_visc_keep_coefficients')
             import :: c_int, c_ptr
@@ -182,6 +188,13 @@ module mom6_hip_capi
         integer(c_int) function mom6h_hor_visc(cs, u, v, h, diffu, diffv, stream) bind(c, name='mom6h_hor_visc')
             import :: c_int, c_ptr
             type(c_ptr), value :: cs, u, v, h, diffu, diffv, stream
+        end function
+        integer(c_int) function mom6h_hor_visc_set_closures(cs, smagorinsky, leith, better_bound, areaT, dxBu, &
+                dyBu, IdxBu, IdyBu) bind(c, name='mom6h_hor_visc_set_closures')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: cs
+            integer(c_int), value :: smagorinsky, leith, better_bound
+            real(c_double), intent(in) :: areaT(*), dxBu(*), dyBu(*), IdxBu(*), IdyBu(*)
         end function
         integer(c_int) function mom6h_hor_visc_end(cs) bind(c, name='mom6h_hor_visc_end')
             import :: c_int, c_ptr
@@ -566,7 +579,7 @@ module mom6_vert_visc_hip
     implicit none
     private
     public :: vert_visc_CS_hip, vert_visc_init_hip, vert_visc_cra_hip, vert_visc_end_hip
-    public :: vert_visc_keep_coefficients_hip
+    public :: vert_visc_keep_coefficients_hip, vert_visc_cra_ray3d_hip
 
     type :: vert_visc_CS_hip
         type(c_ptr) :: handle = c_null_ptr
@@ -602,6 +615,20 @@ contains
         call hip_check(mom6h_vert_visc(CS%handle, u_p, v_p, h_p, dt, taux_p, tauy_p, ru, rv, st), 'vert_visc_cra_hip')
     end subroutine
 
+    !> vert_visc_cra_hip with the serial module's general Rayleigh drag: ray_u3d_p,
+    !! ray_v3d_p are 3D (visc%Ray_u, visc%Ray_v of every level, mom6_types.F90:120-125)
+    subroutine vert_visc_cra_ray3d_hip(u_p, v_p, h_p, dt, CS, taux_p, tauy_p, ray_u3d_p, ray_v3d_p, stream)
+        type(c_ptr), intent(in) :: u_p, v_p, h_p, taux_p, tauy_p, ray_u3d_p, ray_v3d_p
+        real(dp), intent(in) :: dt
+        type(vert_visc_CS_hip), intent(inout) :: CS
+        type(c_ptr), intent(in), optional :: stream
+        type(c_ptr) :: st
+        st = c_null_ptr
+        if (present(stream)) st = stream
+        call hip_check(mom6h_vert_visc_ray3d(CS%handle, u_p, v_p, h_p, dt, taux_p, tauy_p, ray_u3d_p, ray_v3d_p, &
+                                             st), 'vert_visc_cra_ray3d_hip')
+    end subroutine
+
     !> keep = .false.: the coupling coefficients and effective thicknesses
     !! (the control structure's a_u, a_v, h_u, h_v) are not stored, for callers
     !! that never read them
@@ -633,7 +660,7 @@ module mom6_hor_visc_hip
     use mom6_hip_common
     implicit none
     private
-    public :: hor_visc_CS_hip, hor_visc_init_hip, hor_visc_hip, hor_visc_end_hip
+    public :: hor_visc_CS_hip, hor_visc_init_hip, hor_visc_hip, hor_visc_end_hip, hor_visc_set_closures_hip
 
     type :: hor_visc_CS_hip
         type(c_ptr) :: handle = c_null_ptr
@@ -684,6 +711,21 @@ contains
         st = c_null_ptr
         if (present(stream)) st = stream
         call hip_check(mom6h_hor_visc(CS%handle, u_p, v_p, h_p, diffu_p, diffv_p, st), 'hor_visc_hip')
+    end subroutine
+
+    !> hor_visc_init's Smagorinsky=, Leith= and better_bound= options
+    !! (mom6_hor_visc_serial.F90:160-193): call after hor_visc_init_hip and
+    !! before the first hor_visc_hip; the q-point metrics are read by Leith only
+    subroutine hor_visc_set_closures_hip(CS, isd, ied, jsd, jed, smagorinsky, leith, better_bound, areaT, dxBu, &
+                                         dyBu, IdxBu, IdyBu)
+        type(hor_visc_CS_hip), intent(inout) :: CS
+        integer, intent(in) :: isd, ied, jsd, jed
+        logical, intent(in) :: smagorinsky, leith, better_bound
+        real(dp), intent(in), dimension(isd:ied, jsd:jed) :: areaT, dxBu, dyBu, IdxBu, IdyBu
+        call hip_check(mom6h_hor_visc_set_closures(CS%handle, merge(1_c_int, 0_c_int, smagorinsky), &
+                                                   merge(1_c_int, 0_c_int, leith), &
+                                                   merge(1_c_int, 0_c_int, better_bound), areaT, dxBu, dyBu, &
+                                                   IdxBu, IdyBu), 'hor_visc_set_closures_hip')
     end subroutine
 
     subroutine hor_visc_end_hip(CS)

@@ -2,11 +2,18 @@
 !! (:218-399) with every module on the GPU through the Fortran module API of
 !! mom6_hip.F90 — the counterpart of app/rk2_cuda_c_driver.F90 for gfx950.
 !!
-!! Usage: rk2_hip_driver [ni] [nj] [nk] [niter] [bt_nsteps] [diag_mode] [infile] [outfile]
+!! Usage: rk2_hip_driver [ni] [nj] [nk] [niter] [bt_nsteps] [diag_mode] [infile] [outfile] [options]
 !!        Defaults: 180 180 75 10 30 0 (as the reference drivers).
 !!   infile  (optional) raw FP64 u, v, h0 (3D), eta, ubt, vbt, CoriolisBu (2D)
 !!           over the data domain, Fortran order, replacing the analytic state
+!!           ('-' for the analytic state)
 !!   outfile (optional) receives raw FP64 u, v, h (3D), eta (2D) after the run
+!!           ('-' for none)
+!!   options (optional) comma-separated switches of the serial modules that
+!!           the reference driver leaves off: smagorinsky, leith, bound
+!!           (hor_visc_init's Smagorinsky=, Leith=, better_bound=) and ray3d
+!!           (Rayleigh drag at every level, Ray(k) = (1e-4 k)/nk, through the
+!!           3D vertvisc fields instead of the bottom layer's 2D coefficient)
 !!
 !! Configuration of the reference driver: static halos, the beta plane of
 !! init_ocean_grid at 45 deg, dx = 10 km, dt = 300 s, Kh = 100 m2/s, Kv = 1e-4,
@@ -34,18 +41,20 @@ program rk2_hip_driver
     integer :: isd, ied, jsd, jed, isc, iec, jsc, jec, n1, n2, i, j, k, iter
     integer(c_size_t) :: n3v
     real(dp) :: dt, dx, f0, beta
-    character(len=512) :: arg, infile, outfile
+    character(len=512) :: arg, infile, outfile, options
+    logical :: opt_smag, opt_leith, opt_bound, opt_ray3d
 
     ! host arrays: grid, forcing, state
     real(dp), allocatable, target :: one(:, :), areaT(:, :), IareaT(:, :), dxv(:, :), Idxv(:, :)
     real(dp), allocatable, target :: CoriolisBu(:, :), Datu(:, :), gtot(:, :), rem(:, :), f_4(:, :, :)
     real(dp), allocatable, target :: dx2(:, :), ratio(:, :), taux(:, :), tauy(:, :), ray(:, :)
     real(dp), allocatable, target :: u(:, :, :), v(:, :, :), h(:, :, :), h0(:, :, :)
-    real(dp), allocatable, target :: eta(:, :), ubt(:, :), vbt(:, :)
+    real(dp), allocatable, target :: eta(:, :), ubt(:, :), vbt(:, :), ray3(:, :, :)
 
     ! device arrays
     type(c_ptr) :: d_u, d_v, d_h, d_h0, d_htmp, d_uh, d_vh, d_CAu, d_CAv, d_up, d_vp, d_diffu, d_diffv
     type(c_ptr) :: d_eta, d_ubt, d_vbt, d_ubt_av, d_vbt_av, d_eta_av, d_taux, d_tauy, d_ray, d_dyCu, d_dxCv
+    type(c_ptr) :: d_ray3
     ! diagnostics: the shared 2D work array, KE, mass, reduction workspace, 4 statistics
     type(c_ptr) :: d_work2d, d_ke, d_mass, d_red, d_stats
 
@@ -61,7 +70,7 @@ program rk2_hip_driver
     real(dp) :: mass_init, mass_final, max_u, max_v, max_eta, ke
 
     ni = 180; nj = 180; nk = 75; niter = 10; bt_nsteps = 30; diag_mode = 0
-    infile = ''; outfile = ''
+    infile = ''; outfile = ''; options = ''
     if (command_argument_count() >= 1) then; call get_command_argument(1, arg); read (arg, *) ni; end if
     if (command_argument_count() >= 2) then; call get_command_argument(2, arg); read (arg, *) nj; end if
     if (command_argument_count() >= 3) then; call get_command_argument(3, arg); read (arg, *) nk; end if
@@ -70,6 +79,13 @@ program rk2_hip_driver
     if (command_argument_count() >= 6) then; call get_command_argument(6, arg); read (arg, *) diag_mode; end if
     if (command_argument_count() >= 7) call get_command_argument(7, infile)
     if (command_argument_count() >= 8) call get_command_argument(8, outfile)
+    if (command_argument_count() >= 9) call get_command_argument(9, options)
+    if (trim(infile) == '-') infile = ''
+    if (trim(outfile) == '-') outfile = ''
+    opt_smag = index(options, 'smagorinsky') > 0
+    opt_leith = index(options, 'leith') > 0
+    opt_bound = index(options, 'bound') > 0
+    opt_ray3d = index(options, 'ray3d') > 0
     if (ni /= nj) then
         ! the reference driver's htmp copy loop runs i over isd..jed (:375)
         print '(A)', 'rk2_hip_driver: ni must equal nj, as for the reference serial driver'
@@ -89,6 +105,7 @@ program rk2_hip_driver
     else
         print '(A)', 'Diagnostics: disabled'
     end if
+    if (len_trim(options) > 0) print '(A,A)', 'Options: ', trim(options)
 
     ! grid bounds (init_ocean_grid, src/common/mom6_types.F90:145-148)
     isd = 1; ied = ni + 2; jsd = 1; jed = nj + 2
@@ -172,6 +189,9 @@ program rk2_hip_driver
     call hor_visc_init_hip(hvisc_CS, isd, ied, jsd, jed, isc, iec, jsc, jec, nk, 100.0_dp, 1.0e-3_dp, ratio, ratio, &
                            ratio, ratio, Idxv, Idxv, Idxv, Idxv, IareaT, IareaT, one, one, one, one, dx2, dx2, &
                            dx2, dx2)
+    if (opt_smag .or. opt_leith .or. opt_bound) &
+        call hor_visc_set_closures_hip(hvisc_CS, isd, ied, jsd, jed, opt_smag, opt_leith, opt_bound, areaT, dxv, &
+                                       dxv, Idxv, Idxv)
 
     ! device state (up, vp and the work arrays start at zero, as the
     ! reference's never-written host arrays do)
@@ -185,6 +205,14 @@ program rk2_hip_driver
     d_ubt_av = alloc_zero_2d(n1, n2); d_vbt_av = alloc_zero_2d(n1, n2); d_eta_av = alloc_zero_2d(n1, n2)
     d_taux = alloc_copy_2d(taux, n1, n2); d_tauy = alloc_copy_2d(tauy, n1, n2); d_ray = alloc_copy_2d(ray, n1, n2)
     d_dyCu = alloc_copy_2d(dxv, n1, n2); d_dxCv = alloc_copy_2d(dxv, n1, n2)
+    d_ray3 = c_null_ptr
+    if (opt_ray3d) then   ! Ray(k) = (1e-4 k)/nk at every level, Ray_u = Ray_v
+        allocate (ray3(isd:ied, jsd:jed, nk))
+        do k = 1, nk
+            ray3(:, :, k) = (1.0e-4_dp * real(k, dp)) / real(nk, dp)
+        end do
+        d_ray3 = alloc_copy_3d(ray3, n1, n2, nk)
+    end if
     if (diag_mode > 0) then   ! diag_init (mom6_diag.F90:76-114): work_2d starts at zero
         d_work2d = alloc_zero_2d(n1, n2); d_ke = alloc_zero_3d(n1, n2, nk); d_mass = alloc_zero_2d(n1, n2)
         d_red = alloc_zero_2d(int(mom6h_diag_stats_work_doubles()), 1); d_stats = alloc_zero_2d(4, 1)
@@ -202,7 +230,11 @@ program rk2_hip_driver
         call CorAdCalc_hip(d_u, d_v, d_h0, d_uh, d_vh, d_CAu, d_CAv, cor_CS)
         call hip_check(mom6h_vel_update(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, dt, d_u, d_v, d_CAu, d_diffu, &
                                         d_CAv, d_diffv, d_up, d_vp, c_null_ptr), 'vel_update')
-        call vert_visc_cra_hip(d_up, d_vp, d_h0, dt, visc_CS, d_taux, d_tauy, d_ray, d_ray)
+        if (opt_ray3d) then
+            call vert_visc_cra_ray3d_hip(d_up, d_vp, d_h0, dt, visc_CS, d_taux, d_tauy, d_ray3, d_ray3)
+        else
+            call vert_visc_cra_hip(d_up, d_vp, d_h0, dt, visc_CS, d_taux, d_tauy, d_ray, d_ray)
+        end if
         call btstep_hip(d_eta, d_ubt, d_vbt, d_ubt_av, d_vbt_av, d_eta_av, bt_CS)
         call continuity_PPM_hip(d_up, d_h0, d_h, d_uh, dt, cont_CS, BT_cont=BT_cont, hin_halo_to_h=.true.)
         if (diag_mode > 0) then   ! predictor diagnostics (:300-308)
@@ -220,7 +252,11 @@ program rk2_hip_driver
         call CorAdCalc_hip(d_up, d_vp, d_h, d_uh, d_vh, d_CAu, d_CAv, cor_CS)
         call hip_check(mom6h_vel_update(isd, ied, jsd, jed, isc, iec, jsc, jec, nk, 0.5_dp * dt, d_u, d_v, d_CAu, &
                                         d_diffu, d_CAv, d_diffv, d_u, d_v, c_null_ptr), 'vel_update')
-        call vert_visc_cra_hip(d_u, d_v, d_h, 0.5_dp * dt, visc_CS, d_taux, d_tauy, d_ray, d_ray)
+        if (opt_ray3d) then
+            call vert_visc_cra_ray3d_hip(d_u, d_v, d_h, 0.5_dp * dt, visc_CS, d_taux, d_tauy, d_ray3, d_ray3)
+        else
+            call vert_visc_cra_hip(d_u, d_v, d_h, 0.5_dp * dt, visc_CS, d_taux, d_tauy, d_ray, d_ray)
+        end if
         call btstep_hip(d_eta_av, d_ubt_av, d_vbt_av, d_ubt_av, d_vbt_av, d_eta, bt_CS)
         call copy_d2d(d_htmp, d_h, n3v)
         call continuity_PPM_hip(d_u, d_htmp, d_h, d_uh, 0.5_dp * dt, cont_CS, BT_cont=BT_cont)
@@ -290,6 +326,7 @@ program rk2_hip_driver
     call hip_free(d_eta); call hip_free(d_ubt); call hip_free(d_vbt)
     call hip_free(d_ubt_av); call hip_free(d_vbt_av); call hip_free(d_eta_av)
     call hip_free(d_taux); call hip_free(d_tauy); call hip_free(d_ray); call hip_free(d_dyCu); call hip_free(d_dxCv)
+    call hip_free(d_ray3)
 
 contains
 
