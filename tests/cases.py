@@ -14,6 +14,11 @@ CASES = {
                                 coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, biharmonic=True, Ah=1e11),
                          "seeded", 2),
     "cw84_limiter": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, monotonic=True), "seeded", 2),
+    # no-slip walls with the biharmonic term: the tiled hor_visc kernel's
+    # no-slip and Del2 paths (the closure cases run the general kernel)
+    "noslip_biharmonic_walls": (Config(ni=70, nj=24, nk=5, bt_nsteps=12, bc_x=BC_PERIODIC, bc_y=BC_WALL,
+                                       coriolis="betaplane", f0=1e-4, beta=2e-11, Kh=1e3, biharmonic=True,
+                                       Ah=1e11, no_slip=True), "seeded", 2),
     "arakawa_hsu90": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, coriolis_scheme=2), "reference", 2),
     "arakawa_lamb81": (Config(ni=20, nj=20, nk=6, bt_nsteps=15, coriolis_scheme=3), "seeded", 2),
     "deep_column_64": (Config(ni=64, nj=48, nk=30, bt_nsteps=30), "reference", 2),
