@@ -7,8 +7,8 @@
 //   energy scheme) -> velocity update  ou = base + fdt*(CA + diff)
 // The transports, tendencies and potential vorticity never leave the chip: a
 // block stages u, v, h of its tile plus a one-cell ring in LDS for one layer,
-// forms q and KE on the tile's q/h points, then every thread finishes its
-// velocity points.  It walks all layers, so the 2D masks and Coriolis
+// forms q, KE and the viscous stresses on the staged q/h points (each once),
+// then every thread finishes its velocity points.  It walks all layers, so the 2D masks and Coriolis
 // parameter stay in L1/L2 for the whole layer walk.
 //
 // On a uniform grid every metric array holds one value; the kernels take those
@@ -98,6 +98,10 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
     constexpr int MH = TY + 2, RPT = TY / 4;   // staged rows; output rows per thread
     __shared__ T s_u[MH][MW], s_v[MH][MW], s_h[MH][MW];
     __shared__ T s_q[MH][MW], s_KE[MH][MW];
+    // the Laplacian (+ biharmonic) stresses str_xx*h at the staged h points and
+    // str_xy*hq at the staged q points, formed once per point (each is read by
+    // up to three output points)
+    __shared__ T s_sxx[MH][MW], s_sxy[MH][MW];
     __shared__ T s_d2u[BIH ? MH : 1][BIH ? MW : 1], s_d2v[BIH ? MH : 1][BIH ? MW : 1];
     // the 2D masks and Coriolis of the staged region, the same for every layer
     __shared__ T s_mT[MH][MW], s_mBu[MH][MW], s_cor[MH][MW];
@@ -150,6 +154,33 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
         }
         s_mT[lj][li] = mt; s_mBu[lj][li] = mb; s_cor[lj][li] = cor;
     }
+    // Laplacian stress pieces (mom6_hor_visc_serial.F90:503-547, 807-1053)
+    auto str_xx = [&](int pi, int pj) -> T {   // LDS coords of an h point
+        const T dudx = g.DY_dxT * ((g.IdyCu * s_u[pj][pi]) - (g.IdyCu * s_u[pj][pi - 1]));
+        const T dvdy = g.DX_dyT * ((g.IdxCv * s_v[pj][pi]) - (g.IdxCv * s_v[pj - 1][pi]));
+        T str = -(a.Kh * (dudx - dvdy));
+        if (BIH) {
+            const int qi = BIH ? pi : 0, qj = BIH ? pj : 0;
+            const T d_del2u = (g.IdyCu * s_d2u[qj][qi]) - (g.IdyCu * s_d2u[qj][BIH ? qi - 1 : 0]);
+            const T d_del2v = (g.IdxCv * s_d2v[qj][qi]) - (g.IdxCv * s_d2v[BIH ? qj - 1 : 0][qi]);
+            str = str + a.Ah * ((g.DY_dxT * d_del2u) - (g.DX_dyT * d_del2v));
+        }
+        return str * (s_h[pj][pi] * T(1.0));
+    };
+    auto str_xy = [&](int pi, int pj) -> T {   // q point at LDS (pi, pj)
+        const T dvdx = g.DY_dxBu * ((s_v[pj][pi + 1] * g.IdyCv) - (s_v[pj][pi] * g.IdyCv));
+        const T dudy = g.DX_dyBu * ((s_u[pj + 1][pi] * g.IdxCu) - (s_u[pj][pi] * g.IdxCu));
+        const T m = s_mBu[pj][pi];
+        const T sh = (a.no_slip ? (T(2.0) - m) : m) * (dvdx + dudy);
+        const T hq = T(0.25) * ((s_h[pj][pi] + s_h[pj + 1][pi + 1]) + (s_h[pj][pi + 1] + s_h[pj + 1][pi]));
+        T str = -(a.Kh * sh);
+        if (BIH) {
+            const int qi = BIH ? pi : 0, qj = BIH ? pj : 0;
+            str = str + a.Ah * ((g.DY_dxBu * ((s_d2v[qj][BIH ? qi + 1 : 0] * g.IdyCv) - (s_d2v[qj][qi] * g.IdyCv))) +
+                                (g.DX_dyBu * ((s_d2u[BIH ? qj + 1 : 0][qi] * g.IdxCu) - (s_d2u[qj][qi] * g.IdxCu))));
+        }
+        return a.no_slip ? str * (hq * T(1.0)) : str * (hq * m * T(1.0));
+    };
     load_layer(k_first);
     for (int k = k_first; k <= k_last; ++k) {
 #pragma unroll
@@ -182,6 +213,8 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
                 const T hArea_q = (hAu0 + hAu1) + (hAv0 + hAv1);
                 s_q[lj][li] = abs_vort * sdiv(g.Area_q, hArea_q + T(1.0e-20));
             }
+            if (li >= 1 && lj >= 1) s_sxx[lj][li] = str_xx(li, lj);
+            if (li < MW - 1 && lj < MH - 1) s_sxy[lj][li] = str_xy(li, lj);
             if (li >= 1 && lj >= 1 && i <= ie && j <= je) {
                 const T uc = s_u[lj][li], uw = s_u[lj][li - 1], vc = s_v[lj][li], vs = s_v[lj - 1][li];
                 s_KE[lj][li] = sdiv(T(0.25) * ((g.dyCu * (uc * uc) + g.dyCu * (uw * uw)) +
@@ -195,38 +228,11 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             if (i > ie || j > je) continue;
             const int li = i - bi, lj = j - bj;
             const long long c = IX3(d, i, j, k);
-            // Laplacian stress pieces (mom6_hor_visc_serial.F90:503-547, 807-1053)
-            auto str_xx = [&](int pi, int pj) -> T {   // LDS coords of an h point
-                const T dudx = g.DY_dxT * ((g.IdyCu * s_u[pj][pi]) - (g.IdyCu * s_u[pj][pi - 1]));
-                const T dvdy = g.DX_dyT * ((g.IdxCv * s_v[pj][pi]) - (g.IdxCv * s_v[pj - 1][pi]));
-                T str = -(a.Kh * (dudx - dvdy));
-                if (BIH) {
-                    const int qi = BIH ? pi : 0, qj = BIH ? pj : 0;
-                    const T d_del2u = (g.IdyCu * s_d2u[qj][qi]) - (g.IdyCu * s_d2u[qj][BIH ? qi - 1 : 0]);
-                    const T d_del2v = (g.IdxCv * s_d2v[qj][qi]) - (g.IdxCv * s_d2v[BIH ? qj - 1 : 0][qi]);
-                    str = str + a.Ah * ((g.DY_dxT * d_del2u) - (g.DX_dyT * d_del2v));
-                }
-                return str * (s_h[pj][pi] * T(1.0));
-            };
-            auto str_xy = [&](int pi, int pj) -> T {   // q point at LDS (pi, pj)
-                const T dvdx = g.DY_dxBu * ((s_v[pj][pi + 1] * g.IdyCv) - (s_v[pj][pi] * g.IdyCv));
-                const T dudy = g.DX_dyBu * ((s_u[pj + 1][pi] * g.IdxCu) - (s_u[pj][pi] * g.IdxCu));
-                const T m = s_mBu[pj][pi];
-                const T sh = (a.no_slip ? (T(2.0) - m) : m) * (dvdx + dudy);
-                const T hq = T(0.25) * ((s_h[pj][pi] + s_h[pj + 1][pi + 1]) + (s_h[pj][pi + 1] + s_h[pj + 1][pi]));
-                T str = -(a.Kh * sh);
-                if (BIH) {
-                    const int qi = BIH ? pi : 0, qj = BIH ? pj : 0;
-                    str = str + a.Ah * ((g.DY_dxBu * ((s_d2v[qj][BIH ? qi + 1 : 0] * g.IdyCv) - (s_d2v[qj][qi] * g.IdyCv))) +
-                                        (g.DX_dyBu * ((s_d2u[BIH ? qj + 1 : 0][qi] * g.IdxCu) - (s_d2u[qj][qi] * g.IdxCu))));
-                }
-                return a.no_slip ? str * (hq * T(1.0)) : str * (hq * m * T(1.0));
-            };
             if (i <= ie - 1) {
                 // diffu (:1059-1067)
                 const T h_u = T(0.5) * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj][li + 1] * s_h[lj][li + 1]);
-                const T diffu = sdiv((g.IdxCu * ((g.dx2q * str_xy(li, lj - 1)) - (g.dx2q * str_xy(li, lj))) +
-                                      g.IdyCu * ((g.dy2h * str_xx(li, lj)) - (g.dy2h * str_xx(li + 1, lj)))) *
+                const T diffu = sdiv((g.IdxCu * ((g.dx2q * s_sxy[lj - 1][li]) - (g.dx2q * s_sxy[lj][li])) +
+                                      g.IdyCu * ((g.dy2h * s_sxx[lj][li]) - (g.dy2h * s_sxx[lj][li + 1]))) *
                                          g.IareaCu, h_u + a.h_neglect);
                 // CAu, Sadourny (:208-216); vh from compute_transports
                 auto vh = [&](int pi, int pj) { return s_v[pj][pi] * T(0.5) * (s_h[pj][pi] + s_h[pj + 1][pi]) * g.dxCv; };
@@ -238,8 +244,8 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             if (j <= je - 1) {
                 // diffv (:1070-1078)
                 const T h_v = T(0.5) * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj + 1][li] * s_h[lj + 1][li]);
-                const T diffv = sdiv((g.IdyCv * ((g.dy2q * str_xy(li - 1, lj)) - (g.dy2q * str_xy(li, lj))) -
-                                      g.IdxCv * ((g.dx2h * str_xx(li, lj)) - (g.dx2h * str_xx(li, lj + 1)))) *
+                const T diffv = sdiv((g.IdyCv * ((g.dy2q * s_sxy[lj][li - 1]) - (g.dy2q * s_sxy[lj][li])) -
+                                      g.IdxCv * ((g.dx2h * s_sxx[lj][li]) - (g.dx2h * s_sxx[lj + 1][li]))) *
                                          g.IareaCv, h_v + a.h_neglect);
                 // CAv, Sadourny (:218-226); uh from compute_transports
                 auto uh = [&](int pi, int pj) { return s_u[pj][pi] * T(0.5) * (s_h[pj][pi] + s_h[pj][pi + 1]) * g.dyCu; };
