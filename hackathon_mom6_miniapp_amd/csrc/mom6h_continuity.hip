@@ -511,8 +511,9 @@ struct ColRegs {
     }
 };
 
-// LDS of one block: U = u of every level (levels past nk repeat level nk; u
-// lives here rather than in registers), S0/S1 = per-level terms of a pass,
+// LDS of one block: U = u of every level (rows past nk hold level nk or
+// nothing and are never read as values; u lives here rather than in
+// registers), S0/S1 = per-level terms of a pass,
 // R = results of the column sums (one row per quantity; rows 5..8 keep the
 // Newton iteration's starting point for a second iteration), F = the votes of
 // block_any.
@@ -665,7 +666,7 @@ __device__ __forceinline__ float col_sum(const float (*col)[FW], int nk, int lan
 // residual at the rounding level of the layer fluxes (4 ulp of sum |uh_k| of
 // the first pass); the FP64 tolerance of 1e-12 m lies below what FP32 sums
 // resolve, and the iteration would otherwise run to its 20-pass limit.
-template <typename T, bool Y, int CW, int K, int FW>
+template <typename T, bool Y, int CW, int K, int FW, bool EVEN>
 __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monotonic, unsigned bx, unsigned by)
 {
     extern __shared__ double smem_raw[];
@@ -690,8 +691,32 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     const int i = fi, j = fj;
     const int nk = d.nk;
     const long long s = d.s2;
-    const int k0 = grp * K;                                  // my levels: k0+1 .. k0+K
-    auto valid = [&](int kk) __attribute__((always_inline)) { return k0 + kk < nk; };
+    // my levels.  EVEN = false: k0+1 .. k0+K, every slot evaluated (a slot
+    // past nk evaluates level nk into an LDS row past nk, which no sum or
+    // output reads).  EVEN = true
+    // (one level group per wave and CW*(K-1) <= nk <= CW*K, which the
+    // launcher checks): the nk levels spread evenly over the waves, K of them
+    // for the first nk - CW*(K-1) waves and K-1 for the rest, so that the
+    // SIMDs evaluate about nk/4 levels each per pass instead of CW*K/4 (at nk
+    // = 75 with 12 waves of 7: 19 instead of 21); only a wave's last slot can
+    // be empty, skipped in a branch on one wave-uniform flag.
+    int k0;
+    bool tail = true;
+    if constexpr (EVEN) {
+        static_assert(FW == 64, "one level group per wave");
+        const int g = __builtin_amdgcn_readfirstlane(grp);
+        const int full = nk - CW * (K - 1);   // waves with K levels
+        tail = g < full;
+        k0 = g * (K - 1) + (tail ? g : full);
+    } else {
+        k0 = grp * K;
+    }
+    auto valid = [&](int kk) __attribute__((always_inline)) {
+        if constexpr (EVEN) return kk < K - 1 || tail;
+        else return k0 + kk < nk;
+    };
+    // slots whose terms are formed: EVEN skips its empty slot
+    auto formed = [&](int kk) __attribute__((always_inline)) { return !EVEN || valid(kk); };
 
     ColRegs<T, K> col;
     col.dt = (T)a.dt;
@@ -718,8 +743,9 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             S.P[5][lane] = (T)dxT1;
         }
     };
-    // PPM edges of cells 0 and 1 (zonal_edge_thickness :185-207); levels past
-    // nk repeat level nk.  x: each lane reconstructs its own cell 0 from the
+    // PPM edges of cells 0 and 1 (zonal_edge_thickness :185-207); the loads
+    // of a slot past the lane's levels read level min(k, nk) (in bounds) and
+    // nothing is formed from them.  x: each lane reconstructs its own cell 0 from the
     // 5-cell stencil and takes cell 1 from the next lane (whose cell 0 it is),
     // so every cell is reconstructed once.  y: both cells from the 6-cell
     // stencil along j.
@@ -742,6 +768,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         T wW[K], wE[K], w0[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
+            if (!formed(kk)) continue;
             S.U[k0 + kk][lane] = us[kk];
             const T *h5 = hs[kk];
             T sl[3];
@@ -752,6 +779,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         }
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
+            if (!formed(kk)) continue;
             ppm_limit(monotonic, w0[kk], (T)a.h_min, wW[kk], wE[kk]);
             col.set(kk, wW[kk], wE[kk], w0[kk], __shfl_down(wW[kk], 1, FW), __shfl_down(wE[kk], 1, FW),
                     __shfl_down(w0[kk], 1, FW));
@@ -773,6 +801,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         T wW[K], wE[K], w0[K], eW[K], eE[K], e0[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
+            if (!formed(kk)) continue;
             S.U[k0 + kk][lane] = us[kk];
             const T *h6 = hs[kk];
             T sl[4];
@@ -785,6 +814,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         }
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
+            if (!formed(kk)) continue;
             ppm_limit(monotonic, w0[kk], (T)a.h_min, wW[kk], wE[kk]);
             ppm_limit(monotonic, e0[kk], (T)a.h_min, eW[kk], eE[kk]);
             col.set(kk, wW[kk], wE[kk], w0[kk], eW[kk], eE[kk], e0[kk]);
@@ -796,8 +826,8 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // LDS.  S0 keeps each lane's transports of its last evaluation, the
     // result the face outputs and the convergence read.
     // Each level's u is read before the stores of the level above it, so the
-    // LDS read overlaps that level's arithmetic; rows past nk are written too
-    // (they lie inside the arrays and no sum or output reads them).
+    // LDS read overlaps that level's arithmetic (EVEN: the read for a wave's
+    // empty slot stays inside U and its value is not used).
     auto eval_pass = [&](T du, bool on) __attribute__((always_inline)) {
         if (!on) return;
         col.fence();
@@ -806,6 +836,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         for (int kk = 0; kk < K; ++kk) {
             const T u_k = u_next;
             if (kk + 1 < K) u_next = uk(kk + 1);
+            if (!formed(kk)) continue;
             T f, hm;
             col.eval(kk, u_k + du, f, hm);
             S.S0[k0 + kk][lane] = f;
@@ -987,12 +1018,13 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     if (writes_face) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
+            if (!valid(kk)) continue;
             T hm = S.S1[k0 + kk][lane];
             if (!last_at_du) {
                 T fc;
                 col.eval(kk, uk(kk) + du, fc, hm);
             }
-            if (valid(kk)) a.bt_h_u[cu + (long long)(k0 + kk) * s] = hm;
+            a.bt_h_u[cu + (long long)(k0 + kk) * s] = hm;
         }
     }
 
@@ -1060,12 +1092,12 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
                                 : (T(1.5) * (duR - du0)) * ((FAmt_R - FA_avg) / (FAmt_R - FA_0)));
 }
 
-template <typename T, bool Y, int CW, int K, int FW = CF>
+template <typename T, bool Y, int CW, int K, int FW = CF, bool EVEN = false>
 __global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(
     (sizeof(T) == 4 || FW == 32) ? 4 : (CW + 3) / 4, (CW >= 12 || sizeof(T) == 4 || FW == 32) ? 4 : 2)))
 k_continuity_blk(ContArgsT<T> a, int monotonic)
 {
-    continuity_body<T, Y, CW, K, FW>(a, monotonic, blockIdx.x, blockIdx.y);
+    continuity_body<T, Y, CW, K, FW, EVEN>(a, monotonic, blockIdx.x, blockIdx.y);
 }
 
 // every member of an ensemble in one launch (K for the deepest member)
@@ -1076,7 +1108,7 @@ k_continuity_batch(const ContArgsT<T> *__restrict__ args, BatchMap mp)
     int m, local;
     batch_locate(mp, blockIdx.x, m, local);
     const ContArgsT<T> a = args[m];
-    continuity_body<T, Y, CW, K, CF>(a, a.monotonic, local % mp.nbx[m], local / mp.nbx[m]);
+    continuity_body<T, Y, CW, K, CF, false>(a, a.monotonic, local % mp.nbx[m], local / mp.nbx[m]);
 }
 
 // the driver's `h = h0` reset for the cells the convergence does not write:
@@ -1182,10 +1214,10 @@ __global__ void k_convergence(Dims d, const double *hin, const double *__restric
 }
 
 // FP64 block shape beyond 32 levels: false = 64 faces x 12 waves of 7 levels
-// (1.29 ms at 720x720x75), true = 32 faces x 8 waves x 2 groups of 5 levels,
-// two blocks per CU (1.35 ms: 128 VGPRs with spills, and a block's latency
-// chains -- the column folds, the barriers, the loads -- do not shorten with
-// fewer faces)
+// (1.21 ms per call at 720x720x75), true = 32 faces x 8 waves x 2 groups of 5
+// levels, two blocks per CU (1.31 ms: 128 VGPRs with spills, and a block's
+// latency chains -- the column folds, the barriers, the loads -- do not
+// shorten with fewer faces)
 constexpr bool kFP64Narrow = false;
 
 // Blocks in x of a pass with FW faces per block (x: blocks overlap by two faces)
@@ -1195,25 +1227,40 @@ unsigned blk_nbx(const Dims &d, int FW)
     return Y ? (unsigned)((d.iec - d.isc + 1 + FW - 1) / FW) : (unsigned)((d.iec - d.isc + 1 + FW - 3) / (FW - 2));
 }
 
+// A block shape of CW waves of K level slots (64 faces): levels spread evenly
+// over the waves when CW*(K-1) <= nk (every shape of 1 or 2 slots, 8 x 4 from
+// 24 levels, 12 x 7 and 8 x 10 from 72)
+template <typename T, bool Y, int CW, int K>
+void launch_shape(const ContArgsT<T> &a, int mono, dim3 g, hipStream_t s)
+{
+    const dim3 b(64 * CW);
+    constexpr size_t lds = sizeof(ContShared<T, CW, K>);
+    if constexpr (K <= 2) {
+        k_continuity_blk<T, Y, CW, K, CF, true><<<g, b, lds, s>>>(a, mono);
+    } else {
+        if (a.d.nk >= CW * (K - 1)) k_continuity_blk<T, Y, CW, K, CF, true><<<g, b, lds, s>>>(a, mono);
+        else k_continuity_blk<T, Y, CW, K, CF, false><<<g, b, lds, s>>>(a, mono);
+    }
+}
+
 // One pass over nby rows of faces; the block shape follows nk and the type
 template <typename T, bool Y>
 int launch_blk(const ContArgsT<T> &a, int mono, unsigned nby, hipStream_t s)
 {
     const int nk = a.d.nk;
-    const dim3 b(64 * 8), g(blk_nbx<Y>(a.d, CF), nby);
-    if (nk <= 8) k_continuity_blk<T, Y, 8, 1><<<g, b, sizeof(ContShared<T, 8, 1>), s>>>(a, mono);
-    else if (nk <= 16) k_continuity_blk<T, Y, 8, 2><<<g, b, sizeof(ContShared<T, 8, 2>), s>>>(a, mono);
-    else if (nk <= 32) k_continuity_blk<T, Y, 8, 4><<<g, b, sizeof(ContShared<T, 8, 4>), s>>>(a, mono);
+    const dim3 g(blk_nbx<Y>(a.d, CF), nby);
+    if (nk <= 8) launch_shape<T, Y, 8, 1>(a, mono, g, s);
+    else if (nk <= 16) launch_shape<T, Y, 8, 2>(a, mono, g, s);
+    else if (nk <= 32) launch_shape<T, Y, 8, 4>(a, mono, g, s);
     // FP64 beyond 32 levels: 12 waves of 7 levels (three waves per SIMD); the
     // FP32 form: 8 waves of 10 levels (two blocks per CU)
-    else if (sizeof(T) == 8 && !kFP64Narrow)
-        k_continuity_blk<T, Y, 12, 7><<<g, dim3(64 * 12), sizeof(ContShared<T, 12, 7>), s>>>(a, mono);
+    else if (sizeof(T) == 8 && !kFP64Narrow) launch_shape<T, Y, 12, 7>(a, mono, g, s);
     // the alternative: blocks of 32 faces, 8 waves x 2 level groups of 5
     // levels (64 KB of LDS, two blocks per CU)
     else if (sizeof(T) == 8)
-        k_continuity_blk<T, Y, 8, 5, 32><<<dim3(blk_nbx<Y>(a.d, 32), nby), b, sizeof(ContShared<T, 8, 5, 32>), s>>>(
-            a, mono);
-    else k_continuity_blk<T, Y, 8, 10><<<g, b, sizeof(ContShared<T, 8, 10>), s>>>(a, mono);
+        k_continuity_blk<T, Y, 8, 5, 32><<<dim3(blk_nbx<Y>(a.d, 32), nby), dim3(64 * 8), sizeof(ContShared<T, 8, 5, 32>),
+                                            s>>>(a, mono);
+    else launch_shape<T, Y, 8, 10>(a, mono, g, s);
     MOM6H_LAUNCH_CHECK();
     return MOM6H_OK;
 }
@@ -1223,35 +1270,28 @@ int launch_blk(const ContArgsT<T> &a, int mono, unsigned nby, hipStream_t s)
 template <typename T>
 int set_blk_lds(long long max_lds)
 {
-    const void *kern[] = {(const void *)k_continuity_blk<T, false, 8, 1>, (const void *)k_continuity_blk<T, false, 8, 2>,
-                          (const void *)k_continuity_blk<T, false, 8, 4>, (const void *)k_continuity_blk<T, false, 8, 10>,
-                          (const void *)k_continuity_blk<T, true, 8, 1>, (const void *)k_continuity_blk<T, true, 8, 2>,
-                          (const void *)k_continuity_blk<T, true, 8, 4>, (const void *)k_continuity_blk<T, true, 8, 10>};
-    const int bytes[] = {(int)sizeof(ContShared<T, 8, 1>), (int)sizeof(ContShared<T, 8, 2>),
-                         (int)sizeof(ContShared<T, 8, 4>), (int)sizeof(ContShared<T, 8, 10>)};
-    const void *kb[] = {(const void *)k_continuity_batch<T, false, 8, 1>, (const void *)k_continuity_batch<T, false, 8, 2>,
-                        (const void *)k_continuity_batch<T, false, 8, 4>, (const void *)k_continuity_batch<T, false, 8, 10>,
-                        (const void *)k_continuity_batch<T, true, 8, 1>, (const void *)k_continuity_batch<T, true, 8, 2>,
-                        (const void *)k_continuity_batch<T, true, 8, 4>, (const void *)k_continuity_batch<T, true, 8, 10>};
-    if ((int)sizeof(ContShared<T, 12, 7>) <= max_lds) {
-        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, false, 12, 7>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
-        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, true, 12, 7>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(ContShared<T, 12, 7>)));
+    auto raise = [max_lds](const void *kern, long long bytes) -> hipError_t {
+        if (bytes > max_lds) return hipSuccess;
+        return hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    };
+#define MOM6H_BLK_LDS(Y, CW, K)                                                                                    \
+    MOM6H_CHECK(raise((const void *)k_continuity_blk<T, Y, CW, K, CF, true>, sizeof(ContShared<T, CW, K>)));        \
+    MOM6H_CHECK(raise((const void *)k_continuity_blk<T, Y, CW, K, CF, false>, sizeof(ContShared<T, CW, K>)));       \
+    MOM6H_CHECK(raise((const void *)k_continuity_batch<T, Y, CW, K>, sizeof(ContShared<T, CW, K>)));
+    MOM6H_BLK_LDS(false, 8, 1) MOM6H_BLK_LDS(false, 8, 2) MOM6H_BLK_LDS(false, 8, 4) MOM6H_BLK_LDS(false, 8, 10)
+    MOM6H_BLK_LDS(true, 8, 1) MOM6H_BLK_LDS(true, 8, 2) MOM6H_BLK_LDS(true, 8, 4) MOM6H_BLK_LDS(true, 8, 10)
+#undef MOM6H_BLK_LDS
+    for (const bool Y : {false, true}) {
+        const void *even = Y ? (const void *)k_continuity_blk<T, true, 12, 7, CF, true>
+                             : (const void *)k_continuity_blk<T, false, 12, 7, CF, true>;
+        const void *plain = Y ? (const void *)k_continuity_blk<T, true, 12, 7, CF, false>
+                              : (const void *)k_continuity_blk<T, false, 12, 7, CF, false>;
+        const void *narrow = Y ? (const void *)k_continuity_blk<T, true, 8, 5, 32>
+                               : (const void *)k_continuity_blk<T, false, 8, 5, 32>;
+        MOM6H_CHECK(raise(even, sizeof(ContShared<T, 12, 7>)));
+        MOM6H_CHECK(raise(plain, sizeof(ContShared<T, 12, 7>)));
+        MOM6H_CHECK(raise(narrow, sizeof(ContShared<T, 8, 5, 32>)));
     }
-    if ((int)sizeof(ContShared<T, 8, 5, 32>) <= max_lds) {
-        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, false, 8, 5, 32>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)sizeof(ContShared<T, 8, 5, 32>)));
-        MOM6H_CHECK(hipFuncSetAttribute((const void *)k_continuity_blk<T, true, 8, 5, 32>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)sizeof(ContShared<T, 8, 5, 32>)));
-    }
-    for (int q = 0; q < 8; ++q)
-        if (bytes[q % 4] <= max_lds) {
-            MOM6H_CHECK(hipFuncSetAttribute(kern[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q % 4]));
-            MOM6H_CHECK(hipFuncSetAttribute(kb[q], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[q % 4]));
-        }
     return MOM6H_OK;
 }
 
