@@ -339,12 +339,20 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         const C blend = (C(1.0) - bf) * h_harm + bf * h_arith;
         return (vk * h_delta < C(0.0)) ? blend : h_harm;
     };
-    // coupling coefficient a(K2) (:184-202) from z_top (already advanced),
-    // botfn(z_i(K2)) and hvel at K2-1 and K2
-    auto coefA = [&](C z_top, C bfK, C hv_m1, C hv_0) -> C {
-        const C topfn = C(1.0) - z_top / Hmix;
-        C Kv_tot = (z_top < Hmix) ? Kv + (Kv_ml - Kv) * topfn : Kv;
-        Kv_tot = Kv_tot + Kv_extra_bbl * bfK;
+    // Kv with the mixed-layer taper at z_top (:188-192).  The tapered value
+    // passes through an empty asm so that the compiler keeps the select: with
+    // a division among its operands it turns the select into a branch, one per
+    // level, which splits a chunk into a basic block per level and stops the
+    // interleaving of the levels' independent divisions.
+    auto kv_top = [&](C z_top) __attribute__((always_inline)) -> C {
+        C t = Kv + (Kv_ml - Kv) * (C(1.0) - z_top / Hmix);
+        asm("" : "+v"(t));
+        return (z_top < Hmix) ? t : Kv;
+    };
+    // coupling coefficient a(K2) (:184-202) from kv_top(z_top) (z_top
+    // already advanced), botfn(z_i(K2)) and hvel at K2-1 and K2
+    auto coefA = [&](C kvt, C bfK, C hv_m1, C hv_0) -> C {
+        const C Kv_tot = kvt + Kv_extra_bbl * bfK;
         return cdiv(Kv_tot, C(0.5) * (hv_0 + hv_m1 + hn));
     };
 
@@ -361,7 +369,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
             hv_n = hvel(p_a2, p_b2, v2, botfn(ZC(3)));
             vk_cur = (double)v2;
             z_top = z_top + hv1;
-            a_n = (double)coefA(z_top, botfn(ZC(2)), hv1, hv_n);
+            a_n = (double)coefA(kv_top(z_top), botfn(ZC(2)), hv1, hv_n);
         } else {
             vk_cur = 0.0;
             a_n = (double)((Kv + Kv_extra_bbl) / (C(0.5) * hv1 + hn));
@@ -415,17 +423,30 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
                 tb_ready = true;
             }
             // off-recurrence terms: bf[c] = botfn(z_i(k0+1+c)), hvel(k+1), z_top, a(k+1)
-            C bf[VV_C + 1], hvx[VV_C], ax[VV_C];
+            C bf[VV_C + 1], hvx[VV_C], ax[VV_C], hvk[VV_C], zt[VV_C], kvt[VV_C];
 #pragma unroll
             for (int c = 0; c <= VV_C; ++c) bf[c] = botfn(ZC(k0 + 1 + c));
 #pragma unroll
             for (int c = 0; c < VV_C; ++c) hvx[c] = hvel(ha[c], hb[c], vv[c], bf[c + 1]);
+            bool taper = false;
 #pragma unroll
             for (int c = 0; c < VV_C; ++c) {
-                const C hv_k = (c == 0) ? hv_n : hvx[c - 1];
-                z_top = z_top + hv_k;
-                ax[c] = coefA(z_top, bf[c], hv_k, hvx[c]);
+                hvk[c] = (c == 0) ? hv_n : hvx[c - 1];
+                z_top = z_top + hvk[c];
+                zt[c] = z_top;
+                taper = taper || (z_top < Hmix);
             }
+            // the taper only in a chunk where some column has a level above
+            // Hmix (kv_top is Kv at every other level)
+            if (__any(taper)) {
+#pragma unroll
+                for (int c = 0; c < VV_C; ++c) kvt[c] = kv_top(zt[c]);
+            } else {
+#pragma unroll
+                for (int c = 0; c < VV_C; ++c) kvt[c] = Kv;
+            }
+#pragma unroll
+            for (int c = 0; c < VV_C; ++c) ax[c] = coefA(kvt[c], bf[c], hvk[c], hvx[c]);
 #pragma unroll
             for (int c = 0; c < VV_C; ++c) B_serial(k0 + c, (double)ax[c], hvx[c], (double)vv[c]);
         }
@@ -440,7 +461,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         }
         const C hvx = hvel(ha_, hb_, vnx, botfn(ZC(k0 + 2)));
         z_top = z_top + hv_n;
-        const C ax = coefA(z_top, botfn(ZC(k0 + 1)), hv_n, hvx);
+        const C ax = coefA(kv_top(z_top), botfn(ZC(k0 + 1)), hv_n, hvx);
         B_serial(k0, (double)ax, hvx, (double)vnx);
     }
     if (nk >= 2) {   // bottom level: a(nk+1) from the bottom formula (:205-206), Rayleigh drag
