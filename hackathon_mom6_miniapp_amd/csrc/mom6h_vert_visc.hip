@@ -245,13 +245,12 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         }
     }
     const double h_neglect = 1.0e-10;
-    if (!(m > 0.0)) {  // masked column: coefficients only (:213-221)
-        if (KEEP) {
-            for (int k = 0; k < nk; ++k) hu_out[k * s] = h_neglect;
-            for (int k = 0; k <= nk; ++k) a_out[k * s] = 0.0;
-        }
-        return;
-    }
+    // A masked column (:213-221) keeps its velocity and takes the coefficients
+    // hu = h_neglect and a = 0.  It runs the sweeps like the others (on its
+    // own in-bounds data) and its stores write those values (selects, no
+    // branch), so that no load waits for the mask: returning early would hold
+    // every column's first loads behind the mask's memory round trip.
+    const bool act = m > 0.0;
     using ZT = VVSlot<T>;
     auto Z = [&](int k) -> ZT & { return Zs[(k - 1) * 64 + lane]; };   // level k = 1..nk+1
     // C: the arithmetic type of the coefficients (z_i, bottom taper, hvel, the
@@ -376,27 +375,27 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         }
         if (KEEP) a_out[0] = 0.0;
         const double hu_k = (double)hv1 + h_neglect;
-        if (KEEP) hu_out[0] = hu_k;
+        if (KEEP) hu_out[0] = act ? hu_k : h_neglect;
         const double Ray = RAY3 ? ray3[0] : ((nk == 1) ? Ray_bot : 0.0);
         const double b_denom_1 = hu_k + dt * (Ray + 0.0);
         b1 = 1.0 / (b_denom_1 + dt * a_n);
         d1 = b_denom_1 * b1;
         vprev = b1 * (hu_k * (double)v1 + sfc);
-        vel[0] = (T)vprev;
+        vel[0] = act ? (T)vprev : v1;
     }
     // one interior level k (2..nk-1): the recurrence, given a(k+1) and hvel(k+1)
     auto B_serial = [&](int k, double a_next, C hv_next, double vk_next) {
         const double a_k = a_n;
-        if (KEEP) a_out[(k - 1) * s] = a_k;
+        if (KEEP) a_out[(k - 1) * s] = act ? a_k : 0.0;
         const C hv_k = hv_n;
         const double hu_k = (double)hv_k + h_neglect;
-        if (KEEP) hu_out[(k - 1) * s] = hu_k;
+        if (KEEP) hu_out[(k - 1) * s] = act ? hu_k : h_neglect;
         Z(k) = (ZT)(dt * a_k * b1);                        // c1(k); z_i(k) is no longer read
         const double b_denom_1 = hu_k + dt * ((RAY3 ? ray3[(k - 1) * s] : 0.0) + a_k * d1);
         b1 = 1.0 / (b_denom_1 + dt * a_next);
         d1 = b_denom_1 * b1;
         vprev = (hu_k * vk_cur + dt * a_k * vprev) * b1;
-        vel[(k - 1) * s] = (T)vprev;
+        vel[(k - 1) * s] = act ? (T)vprev : (T)vk_cur;
         a_n = a_next;
         hv_n = hv_next;
         vk_cur = vk_next;
@@ -467,21 +466,22 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
     if (nk >= 2) {   // bottom level: a(nk+1) from the bottom formula (:205-206), Rayleigh drag
         const int k = nk;
         const double a_k = a_n;
-        if (KEEP) a_out[(k - 1) * s] = a_k;
+        if (KEEP) a_out[(k - 1) * s] = act ? a_k : 0.0;
         const C hv_k = hv_n;
         a_n = (double)((Kv + Kv_extra_bbl) / (C(0.5) * hv_k + hn));
         const double hu_k = (double)hv_k + h_neglect;
-        if (KEEP) hu_out[(k - 1) * s] = hu_k;
+        if (KEEP) hu_out[(k - 1) * s] = act ? hu_k : h_neglect;
         Z(k) = (ZT)(dt * a_k * b1);
         const double b_denom_1 = hu_k + dt * ((RAY3 ? ray3[(k - 1) * s] : Ray_bot) + a_k * d1);
         b1 = 1.0 / (b_denom_1 + dt * a_n);
         d1 = b_denom_1 * b1;
         vprev = (hu_k * vk_cur + dt * a_k * vprev) * b1;
-        vel[(k - 1) * s] = (T)vprev;
+        vel[(k - 1) * s] = act ? (T)vprev : (T)vk_cur;
     }
-    if (KEEP) a_out[nk * s] = a_n;
+    if (KEEP) a_out[nk * s] = act ? a_n : 0.0;
 
-    // ---- sweep C, bottom-up back substitution (:447-449) ----
+    // ---- sweep C, bottom-up back substitution (:447-449); a masked column
+    // stores back the velocity sweep B left in place (its input) ----
     double vnext = vprev;
     k0 = nk - 1;
     double tf[VV_T];   // the first VV_T levels after the last whole chunk (clamped to level 1)
@@ -508,7 +508,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
 #pragma unroll
             for (int c = 0; c < VV_A; ++c) {
                 vnext = fv[c] + Z(k0 - c + 1) * vnext;
-                vel[(k0 - c - 1) * s] = (T)vnext;
+                vel[(k0 - c - 1) * s] = act ? (T)vnext : (T)fv[c];
             }
         }
     }
@@ -517,14 +517,15 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
         for (int c = 0; c < VV_T; ++c) {
             if (k0 - c >= 1) {
                 vnext = tf[c] + Z(k0 - c + 1) * vnext;
-                vel[(k0 - c - 1) * s] = (T)vnext;
+                vel[(k0 - c - 1) * s] = act ? (T)vnext : (T)tf[c];
             }
         }
         k0 -= VV_T;
     }
     for (; k0 >= 1; --k0) {
-        vnext = vel[(k0 - 1) * s] + Z(k0 + 1) * vnext;
-        vel[(k0 - 1) * s] = (T)vnext;
+        const double f = vel[(k0 - 1) * s];
+        vnext = f + Z(k0 + 1) * vnext;
+        vel[(k0 - 1) * s] = act ? (T)vnext : (T)f;
     }
     // bottom stress (:452-458)
     const double RHO_0 = 1035.0;
@@ -534,7 +535,7 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
     } else {
         t = t + RHO_0 * Ray_bot * vprev;
     }
-    *tb_out = t;
+    if (act) *tb_out = t;
 }
 
 // one grid, tiles renumbered per XCD
