@@ -436,10 +436,17 @@ __device__ __forceinline__ void vv_body(const VVArgs<T> &p, int bx, int row, VVS
                 taper = taper || (z_top < Hmix);
             }
             // the taper only in a chunk where some column has a level above
-            // Hmix (kv_top is Kv at every other level)
+            // Hmix (kv_top is Kv at every other level).  Its inputs pass
+            // through a volatile asm inside the block, so that the divisions
+            // stay behind the one scalar branch per chunk instead of being
+            // hoisted above it and evaluated in every chunk.
             if (__any(taper)) {
 #pragma unroll
-                for (int c = 0; c < VV_C; ++c) kvt[c] = kv_top(zt[c]);
+                for (int c = 0; c < VV_C; ++c) {
+                    C z = zt[c];
+                    asm volatile("" : "+v"(z));
+                    kvt[c] = kv_top(z);
+                }
             } else {
 #pragma unroll
                 for (int c = 0; c < VV_C; ++c) kvt[c] = Kv;
