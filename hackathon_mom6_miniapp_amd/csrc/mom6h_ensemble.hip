@@ -333,7 +333,7 @@ int mom6h_ensemble_prepare(mom6h_ensemble *e)
     for (mom6h_model *m : e->members) {
         RC(mom6h_model_prepare(m));
         MOM6H_REQUIRE(fused_ok(m), "mom6h_ensemble_prepare: every member needs the fused stencils (uniform metrics, "
-                                   "Sadourny Coriolis, Laplacian viscosity)");
+                                   "0/1 masks, Coriolis constant along rows, Sadourny Coriolis, Laplacian viscosity)");
         MOM6H_REQUIRE(m->bt_cs->uniform, "mom6h_ensemble_prepare: members need uniform barotropic coefficients");
         e->max_lds = m->vv->max_lds;
     }

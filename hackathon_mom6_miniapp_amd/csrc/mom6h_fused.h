@@ -71,4 +71,24 @@ inline bool uniform_value(const double *a, size_t n, double *v)
     return true;
 }
 
+// true when every element of a[0..n) is 0 or 1 (a land/sea mask the fused
+// momentum stencil can stage as bytes)
+inline bool binary_mask(const double *a, size_t n)
+{
+    for (size_t q = 0; q < n; ++q)
+        if (!(a[q] == 0.0 || a[q] == 1.0)) return false;
+    return true;
+}
+
+// true when the 2D field a (ni x nj, i fastest) is constant along every row
+// j (f-plane and beta-plane Coriolis: the fused momentum stencil stages one
+// value per row)
+inline bool rows_uniform(const double *a, size_t ni, size_t nj)
+{
+    for (size_t j = 0; j < nj; ++j)
+        for (size_t i = 1; i < ni; ++i)
+            if (!(a[j * ni + i] == a[j * ni])) return false;
+    return true;
+}
+
 }  // namespace mom6h

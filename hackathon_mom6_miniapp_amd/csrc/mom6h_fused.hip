@@ -103,8 +103,13 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
     // up to three output points)
     __shared__ T s_sxx[MH][MW], s_sxy[MH][MW];
     __shared__ T s_d2u[BIH ? MH : 1][BIH ? MW : 1], s_d2v[BIH ? MH : 1][BIH ? MW : 1];
-    // the 2D masks and Coriolis of the staged region, the same for every layer
-    __shared__ T s_mT[MH][MW], s_mBu[MH][MW], s_cor[MH][MW];
+    // the 2D masks and Coriolis of the staged region, the same for every
+    // layer: the masks as bytes (the host checks that they hold 0 or 1) and
+    // Coriolis as one value per row (the host checks that CoriolisBu is
+    // constant along every row), which leaves a 64 x 8 tile in FP64 38 KB of
+    // LDS, four blocks per CU instead of three
+    __shared__ unsigned char s_mT[MH][MW], s_mBu[MH][MW];
+    __shared__ T s_cor[MH];
     const Dims &d = a.d;
     const UMetT<T> g(a.g);
     const int is = d.isc, ie = d.iec, js = d.jsc, je = d.jec;
@@ -152,7 +157,9 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             mb = (T)a.mask2dBu[c2];
             cor = (T)a.CoriolisBu[c2];
         }
-        s_mT[lj][li] = mt; s_mBu[lj][li] = mb; s_cor[lj][li] = cor;
+        s_mT[lj][li] = (unsigned char)(mt != T(0.0));
+        s_mBu[lj][li] = (unsigned char)(mb != T(0.0));
+        if (li == 0) s_cor[lj] = cor;   // i = i0-1: inside the data domain
     }
     // Laplacian stress pieces (mom6_hor_visc_serial.F90:503-547, 807-1053)
     auto str_xx = [&](int pi, int pj) -> T {   // LDS coords of an h point
@@ -170,7 +177,7 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
     auto str_xy = [&](int pi, int pj) -> T {   // q point at LDS (pi, pj)
         const T dvdx = g.DY_dxBu * ((s_v[pj][pi + 1] * g.IdyCv) - (s_v[pj][pi] * g.IdyCv));
         const T dudy = g.DX_dyBu * ((s_u[pj + 1][pi] * g.IdxCu) - (s_u[pj][pi] * g.IdxCu));
-        const T m = s_mBu[pj][pi];
+        const T m = (T)s_mBu[pj][pi];
         const T sh = (a.no_slip ? (T(2.0) - m) : m) * (dvdx + dudy);
         const T hq = T(0.25) * ((s_h[pj][pi] + s_h[pj + 1][pi + 1]) + (s_h[pj][pi + 1] + s_h[pj + 1][pi]));
         T str = -(a.Kh * sh);
@@ -208,8 +215,8 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
                 const T hAu1 = T(0.5) * ((g.areaT * s_h[lj + 1][li]) + (g.areaT * s_h[lj + 1][li + 1]));
                 const T hAv0 = (i < is) ? T(0.0) : T(0.5) * ((g.areaT * s_h[lj][li]) + (g.areaT * s_h[lj + 1][li]));
                 const T hAv1 = T(0.5) * ((g.areaT * s_h[lj][li + 1]) + (g.areaT * s_h[lj + 1][li + 1]));
-                const T rel_vort = s_mBu[lj][li] * (dvdx - dudy) * g.IareaBu;
-                const T abs_vort = s_cor[lj][li] + rel_vort;
+                const T rel_vort = (T)s_mBu[lj][li] * (dvdx - dudy) * g.IareaBu;
+                const T abs_vort = s_cor[lj] + rel_vort;
                 const T hArea_q = (hAu0 + hAu1) + (hAv0 + hAv1);
                 s_q[lj][li] = abs_vort * sdiv(g.Area_q, hArea_q + T(1.0e-20));
             }
@@ -230,7 +237,7 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             const long long c = IX3(d, i, j, k);
             if (i <= ie - 1) {
                 // diffu (:1059-1067)
-                const T h_u = T(0.5) * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj][li + 1] * s_h[lj][li + 1]);
+                const T h_u = T(0.5) * ((T)s_mT[lj][li] * s_h[lj][li] + (T)s_mT[lj][li + 1] * s_h[lj][li + 1]);
                 const T diffu = sdiv((g.IdxCu * ((g.dx2q * s_sxy[lj - 1][li]) - (g.dx2q * s_sxy[lj][li])) +
                                       g.IdyCu * ((g.dy2h * s_sxx[lj][li]) - (g.dy2h * s_sxx[lj][li + 1]))) *
                                          g.IareaCu, h_u + a.h_neglect);
@@ -243,7 +250,7 @@ __device__ __forceinline__ void momentum_body(const MomArgsT<T> &a, int tile_x, 
             }
             if (j <= je - 1) {
                 // diffv (:1070-1078)
-                const T h_v = T(0.5) * (s_mT[lj][li] * s_h[lj][li] + s_mT[lj + 1][li] * s_h[lj + 1][li]);
+                const T h_v = T(0.5) * ((T)s_mT[lj][li] * s_h[lj][li] + (T)s_mT[lj + 1][li] * s_h[lj + 1][li]);
                 const T diffv = sdiv((g.IdyCv * ((g.dy2q * s_sxy[lj][li - 1]) - (g.dy2q * s_sxy[lj][li])) -
                                       g.IdxCv * ((g.dx2h * s_sxx[lj][li]) - (g.dx2h * s_sxx[lj + 1][li]))) *
                                          g.IareaCv, h_v + a.h_neglect);

@@ -40,6 +40,7 @@ struct mom6h_model {
     int serialize = 0;       // barotropic chain on the 3D stream (per-kernel timing without overlap)
     int fused = 1;           // fused momentum stencil (module-by-module when 0)
     bool uniform = false;    // every metric array constant (fused kernels allowed)
+    bool fused_rows = false; // masks 0/1 and Coriolis constant along rows (the fused stencil's staging)
     UMetrics ug{};
     hipStream_t s0 = nullptr, s1 = nullptr;
     hipEvent_t e_start = nullptr, e_bt = nullptr;

@@ -639,9 +639,12 @@ int mom6h_model_prepare(mom6h_model *m)
             g.Idxdy2v = g.IdxCv * g.IdyCv * g.IdyCv;
         }
         m->uniform = ok;
+        m->fused_rows = binary_mask(mT.data(), n) && binary_mask(mBu.data(), n) &&
+                        rows_uniform(fq.data(), (size_t)d.n1, (size_t)d.n2);
     }
-    MOM6H_REQUIRE(!m->mixed || m->uniform,
-                  "mom6h_model_prepare: the mixed-precision form needs a grid with uniform metrics");
+    MOM6H_REQUIRE(!m->mixed || (m->uniform && m->fused_rows),
+                  "mom6h_model_prepare: the mixed-precision form needs a grid with uniform metrics, 0/1 masks "
+                  "and Coriolis constant along each row");
     // periodic fill of the initial state (rk2_mpi_serial_driver.F90:191-197)
     if (!m->mixed)
         RC(fill_halos(m, {{m->u, nk}, {m->v, nk}, {m->h, nk}, {m->h0, nk}}));
@@ -972,7 +975,7 @@ int mom6h_model_destroy(mom6h_model *m)
 namespace mom6h {
 bool fused_ok(const mom6h_model *m)
 {
-    return m->fused && m->uniform && m->c.laplacian && m->c.coriolis_scheme == 1 && !m->c.smagorinsky &&
-           !m->c.leith && !m->c.better_bound;
+    return m->fused && m->uniform && m->fused_rows && m->c.laplacian && m->c.coriolis_scheme == 1 &&
+           !m->c.smagorinsky && !m->c.leith && !m->c.better_bound;
 }
 }  // namespace mom6h
