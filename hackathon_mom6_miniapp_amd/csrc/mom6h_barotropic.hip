@@ -281,20 +281,41 @@ struct BTMulti {
     int n0, ns;                            // first substep, number of substeps (1 .. TS)
 };
 
+// M(w) = ceil(2^20 / w) for widths w < 256, so that t / w = (t * M(w)) >> 20
+// exactly for t < 4096; read from constant memory with a uniform index (a
+// scalar load) instead of an integer division per call.
+struct MulShift {
+    unsigned v[256];
+};
+constexpr MulShift make_mulshift()
+{
+    MulShift t{};
+    for (unsigned w = 1; w < 256; ++w) t.v[w] = ((1u << 20) + w - 1) / w;
+    return t;
+}
+__constant__ MulShift kMulShift = make_mulshift();
+
 // Calls f(x0 + t % w, y0 + t / w) for the positions t < w*h of a rectangle of
 // width w >= 8 and at most NT*MAXI positions (<= 4096: t / w as an exact
-// multiply-shift), t spread over the block's NT threads.
+// multiply-shift), t spread over the block's NT threads.  The products fit in
+// 24-bit operands (t < 2^12, M <= 2^17, q < 2^9, w < 2^12), so they take the
+// full-rate 24-bit multiply rather than the quarter-rate 32-bit one.
 template <int NT, int MAXI, typename F>
 __device__ __forceinline__ void rect_each(int x0, int x1, int y0, int y1, int tid, F &&f)
 {
+    static_assert(NT * MAXI <= 4096, "positions beyond the exact multiply-shift range");
     const unsigned w = (unsigned)(x1 - x0 + 1), n = w * (unsigned)(y1 - y0 + 1);
-    const unsigned M = ((1u << 20) + w - 1) / w;
+    const unsigned M = kMulShift.v[w];   // 8 <= w < 256
 #pragma unroll
     for (int r = 0; r < MAXI; ++r) {
         const unsigned t = (unsigned)tid + (unsigned)NT * r;
         if (t < n) {
-            const unsigned q = (t * M) >> 20;
-            f(x0 + (int)(t - q * w), y0 + (int)q);
+            const unsigned q = __umul24(t, M) >> 20;
+            const int li = x0 + (int)(t - __umul24(q, w)), lj = y0 + (int)q;
+            // staged positions: small and non-negative, so that their LDS
+            // offsets take 24-bit multiplies
+            __builtin_assume(li >= 0 && li < 4096 && lj >= 0 && lj < 4096);
+            f(li, lj);
         }
     }
 }
@@ -311,7 +332,7 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
     constexpr int NOUT = (TX * TY + NT - 1) / NT;       // output points per thread (t < TX*TY)
     constexpr int NR = ((TX + 3 * (TS - 1)) * (TY + 3 * (TS - 1)) + NT - 1) / NT;          // valid region
     constexpr int NP = ((TX + 3 * (TS - 1) + 2) * (TY + 3 * (TS - 1) + 2) + NT - 1) / NT;  // update regions
-    static_assert(RW * RH <= 4096, "staged region too large for the multiply-shift index");
+    static_assert(RW * RH <= 4096 && RW + 2 < 256, "staged region too large for the multiply-shift index");
     __shared__ double s_e[RH][RW], s_u[RH][RW], s_v[RH][RW];
     __shared__ double s_dv[RH][ROW ? 1 : RW], s_f4[RH][ROW ? 1 : RW];
     __shared__ double s_ep[ROW ? RH : 1][ROW ? RW : 1];   // ROW: the predicted eta
@@ -406,11 +427,12 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
         const bool v_first = ((n + p.first_direction) % 2) == 1;
         // this thread's points of R: t = tid + NT r, (A0 + t % rw, B0 + t / rw)
         const unsigned rw = (unsigned)(A1 - A0 + 1), nR = rw * (unsigned)(B1 - B0 + 1);
-        const unsigned Mr = ((1u << 20) + rw - 1) / rw;
+        const unsigned Mr = kMulShift.v[rw];
         auto r_pos = [&](int r, int &li, int &lj) -> bool {
-            const unsigned t = (unsigned)tid + (unsigned)NT * r, q = (t * Mr) >> 20;
-            li = A0 + (int)(t - q * rw);
+            const unsigned t = (unsigned)tid + (unsigned)NT * r, q = __umul24(t, Mr) >> 20;
+            li = A0 + (int)(t - __umul24(q, rw));
             lj = B0 + (int)q;
+            __builtin_assume(li >= 0 && li < 4096 && lj >= 0 && lj < 4096);
             return t < nR;
         };
         // old values the transports of these points need
