@@ -1100,15 +1100,22 @@ k_continuity_blk(ContArgsT<T> a, int monotonic)
     continuity_body<T, Y, CW, K, FW, EVEN>(a, monotonic, blockIdx.x, blockIdx.y);
 }
 
-// every member of an ensemble in one launch (K for the deepest member)
-template <typename T, bool Y, int CW, int K>
+// the members of an ensemble that one batched launch covers: entry q of its
+// BatchMap is member v[q]
+struct MemberIndex {
+    int v[kMaxMembers];
+};
+
+// the members of an ensemble whose depth takes the block shape CW x K, in one
+// launch (EVEN: every one of them spreads its levels evenly)
+template <typename T, bool Y, int CW, int K, bool EVEN>
 __global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, (CW >= 16 || sizeof(T) == 4) ? 4 : 2)))
-k_continuity_batch(const ContArgsT<T> *__restrict__ args, BatchMap mp)
+k_continuity_batch(const ContArgsT<T> *__restrict__ args, BatchMap mp, MemberIndex mi)
 {
     int m, local;
     batch_locate(mp, blockIdx.x, m, local);
-    const ContArgsT<T> a = args[m];
-    continuity_body<T, Y, CW, K, CF, false>(a, a.monotonic, local % mp.nbx[m], local / mp.nbx[m]);
+    const ContArgsT<T> a = args[mi.v[m]];
+    continuity_body<T, Y, CW, K, CF, EVEN>(a, a.monotonic, local % mp.nbx[m], local / mp.nbx[m]);
 }
 
 // the driver's `h = h0` reset for the cells the convergence does not write:
@@ -1277,9 +1284,16 @@ int set_blk_lds(long long max_lds)
 #define MOM6H_BLK_LDS(Y, CW, K)                                                                                    \
     MOM6H_CHECK(raise((const void *)k_continuity_blk<T, Y, CW, K, CF, true>, sizeof(ContShared<T, CW, K>)));        \
     MOM6H_CHECK(raise((const void *)k_continuity_blk<T, Y, CW, K, CF, false>, sizeof(ContShared<T, CW, K>)));       \
-    MOM6H_CHECK(raise((const void *)k_continuity_batch<T, Y, CW, K>, sizeof(ContShared<T, CW, K>)));
+    MOM6H_CHECK(raise((const void *)k_continuity_batch<T, Y, CW, K, true>, sizeof(ContShared<T, CW, K>)));
+#define MOM6H_BATCH_PLAIN_LDS(Y, K)                                                                                \
+    MOM6H_CHECK(raise((const void *)k_continuity_batch<T, Y, 8, K, false>, sizeof(ContShared<T, 8, K>)));
     MOM6H_BLK_LDS(false, 8, 1) MOM6H_BLK_LDS(false, 8, 2) MOM6H_BLK_LDS(false, 8, 4) MOM6H_BLK_LDS(false, 8, 10)
     MOM6H_BLK_LDS(true, 8, 1) MOM6H_BLK_LDS(true, 8, 2) MOM6H_BLK_LDS(true, 8, 4) MOM6H_BLK_LDS(true, 8, 10)
+    MOM6H_BATCH_PLAIN_LDS(false, 4) MOM6H_BATCH_PLAIN_LDS(false, 7) MOM6H_BATCH_PLAIN_LDS(false, 10)
+    MOM6H_BATCH_PLAIN_LDS(true, 4) MOM6H_BATCH_PLAIN_LDS(true, 7) MOM6H_BATCH_PLAIN_LDS(true, 10)
+    MOM6H_CHECK(raise((const void *)k_continuity_batch<T, false, 8, 7, true>, sizeof(ContShared<T, 8, 7>)));
+    MOM6H_CHECK(raise((const void *)k_continuity_batch<T, true, 8, 7, true>, sizeof(ContShared<T, 8, 7>)));
+#undef MOM6H_BATCH_PLAIN_LDS
 #undef MOM6H_BLK_LDS
     for (const bool Y : {false, true}) {
         const void *even = Y ? (const void *)k_continuity_blk<T, true, 12, 7, CF, true>
@@ -1375,16 +1389,51 @@ int continuity_batch(const ContArgsT<T> *d_args, const ContArgsT<T> *h_args, int
         nk_max = d.nk > nk_max ? d.nk : nk_max;
     }
     MOM6H_REQUIRE(nk_max <= 80 && (long long)sizeof(ContShared<T, 8, 10>) <= max_lds, "continuity batch: nk <= 80");
-    const unsigned nb = (unsigned)mp.first[n];
-    const dim3 b(64 * 8);
-#define CB(Y)                                                                                                  \
-    if (nk_max <= 8) k_continuity_batch<T, Y, 8, 1><<<nb, b, sizeof(ContShared<T, 8, 1>), s>>>(d_args, mp);     \
-    else if (nk_max <= 16) k_continuity_batch<T, Y, 8, 2><<<nb, b, sizeof(ContShared<T, 8, 2>), s>>>(d_args, mp); \
-    else if (nk_max <= 32) k_continuity_batch<T, Y, 8, 4><<<nb, b, sizeof(ContShared<T, 8, 4>), s>>>(d_args, mp); \
-    else k_continuity_batch<T, Y, 8, 10><<<nb, b, sizeof(ContShared<T, 8, 10>), s>>>(d_args, mp);
-    if (y_pass) { CB(true) } else { CB(false) }
+    // one launch per block shape: 8 waves of K level slots, K from the
+    // member's depth (a member of 25 levels takes 8 x 4 = 32 slots rather than
+    // the 80 of the deepest member's 8 x 10); the levels spread evenly when
+    // every member of the launch has nk >= 8*(K-1)
+    const int shapes[5] = {1, 2, 4, 7, 10};
+    auto shape_of = [](int nk) { return nk <= 8 ? 1 : nk <= 16 ? 2 : nk <= 32 ? 4 : nk <= 56 ? 7 : 10; };
+    for (const int K : shapes) {
+        BatchMap g{};
+        MemberIndex mi{};
+        bool even = true;
+        for (int m = 0; m < n; ++m) {
+            const int nk = h_args[m].d.nk;
+            if (shape_of(nk) != K) continue;
+            const int q = g.n++;
+            mi.v[q] = m;
+            g.nbx[q] = mp.nbx[m];
+            g.nby[q] = mp.nby[m];
+            g.first[q + 1] = g.first[q] + g.nbx[q] * g.nby[q];
+            even = even && nk >= 8 * (K - 1);
+        }
+        if (g.n == 0) continue;
+        const unsigned nb = (unsigned)g.first[g.n];
+        const dim3 b(64 * 8);
+#define CB(Y, KK)                                                                                             \
+    if (even) k_continuity_batch<T, Y, 8, KK, true><<<nb, b, sizeof(ContShared<T, 8, KK>), s>>>(d_args, g, mi); \
+    else k_continuity_batch<T, Y, 8, KK, false><<<nb, b, sizeof(ContShared<T, 8, KK>), s>>>(d_args, g, mi);
+#define CBY(KK) \
+    if (y_pass) { CB(true, KK) } else { CB(false, KK) }
+        if (K == 1) {   // shapes of 1 and 2 slots always spread evenly
+            if (y_pass) k_continuity_batch<T, true, 8, 1, true><<<nb, b, sizeof(ContShared<T, 8, 1>), s>>>(d_args, g, mi);
+            else k_continuity_batch<T, false, 8, 1, true><<<nb, b, sizeof(ContShared<T, 8, 1>), s>>>(d_args, g, mi);
+        } else if (K == 2) {
+            if (y_pass) k_continuity_batch<T, true, 8, 2, true><<<nb, b, sizeof(ContShared<T, 8, 2>), s>>>(d_args, g, mi);
+            else k_continuity_batch<T, false, 8, 2, true><<<nb, b, sizeof(ContShared<T, 8, 2>), s>>>(d_args, g, mi);
+        } else if (K == 4) {
+            CBY(4)
+        } else if (K == 7) {
+            CBY(7)
+        } else {
+            CBY(10)
+        }
+#undef CBY
 #undef CB
-    MOM6H_LAUNCH_CHECK();
+        MOM6H_LAUNCH_CHECK();
+    }
     if (y_pass) {
         BatchMap mc{};
         mc.n = n;

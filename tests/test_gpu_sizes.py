@@ -226,6 +226,37 @@ def test_deep_ensemble_members_bit_exact_with_oracle():
     assert not bad, "\n".join(bad)
 
 
+RAGGED_MEMBERS = [
+    Config(ni=70, nj=30, nk=20, bt_nsteps=12, bc_x=BC_PERIODIC, bc_y=BC_WALL, coriolis="betaplane", f0=1e-4,
+           beta=2e-11, Kh=1e3),
+    Config(ni=40, nj=28, nk=40, bt_nsteps=12, bc_x=BC_PERIODIC, bc_y=BC_PERIODIC, coriolis="fplane", f0=1e-4),
+    Config(ni=36, nj=24, nk=60, bt_nsteps=12, bc_x=BC_PERIODIC, bc_y=BC_WALL, coriolis="betaplane", f0=1e-4,
+           beta=2e-11, Kh=1e3),
+]
+
+
+def test_ragged_ensemble_members_bit_exact_with_oracle():
+    """Members whose depths fall below their block shape's even spread (20 of
+    8 x 4 slots, 40 of 8 x 7, 60 of 8 x 10): the batched continuity runs one
+    launch per shape with the plain level mapping; every FP64 member against
+    the oracle directly."""
+    from hackathon_mom6_miniapp_amd.model import Ensemble
+    from oracle.pyoracle import Oracle
+
+    cases = [make_case(c, init="seeded", seed=700 + q) for q, c in enumerate(RAGGED_MEMBERS)]
+    print("[gpu ragged ensemble] fp64, 2 steps", flush=True)
+    ens = Ensemble(RAGGED_MEMBERS, cases)
+    ens.step(2)
+    ens.synchronize()
+    bad = []
+    for q, (cfg, case) in enumerate(zip(RAGGED_MEMBERS, cases)):
+        ora = Oracle(cfg, case)
+        ora.step(2)
+        bad += [f"member {q}: {m}" for m in _compare(ens.members[q], ora, cfg, fused=True)]
+    ens.close()
+    assert not bad, "\n".join(bad)
+
+
 def test_deep_ensemble_mixed_within_bound():
     """The same members in the mixed form (one launch per stage) against the
     FP64 ensemble after 100 steps."""
