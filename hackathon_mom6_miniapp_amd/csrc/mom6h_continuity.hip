@@ -357,14 +357,24 @@ __device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
 template <typename T>
 __device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
 
+// max/min as single instructions.  The reference's MAX/MIN differ from them
+// only for NaN operands and for a pair of zeros of opposite sign; the PPM
+// slope below applies them to thicknesses (never -0: a layer thickness is
+// positive or the +0 that continuity's max(h, 0) leaves) and to differences
+// x - y with x >= y and absolute values (>= +0, never -0).
+__device__ __forceinline__ double hmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double hmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float hmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float hmin(float a, float b) { return __builtin_fminf(a, b); }
+
 // PPM_reconstruction_x (:1124-1247) slope from the three cells, branch-free
 template <typename T>
 __device__ __forceinline__ T ppm_slope_v(T mL, T m0, T mR, T hm, T h0, T hp)
 {
     const T s = T(0.5) * (hp - hm);
-    const T dMx = vmax(vmax(hp, hm), h0) - h0;
-    const T dMn = h0 - vmin(vmin(hp, hm), h0);
-    const T slp = vsign1(s) * vmin(vabs(s), T(2.0) * vmin(dMx, dMn));
+    const T dMx = hmax(hmax(hp, hm), h0) - h0;
+    const T dMn = h0 - hmin(hmin(hp, hm), h0);
+    const T slp = vsign1(s) * hmin(vabs(s), T(2.0) * hmin(dMx, dMn));
     return ((mL * m0 * mR) == T(0.0)) ? T(0.0) : slp;
 }
 
