@@ -994,19 +994,26 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     }
 
     // outputs at the face: uh, u_cor, du_cor; a face shared with the western
-    // neighbour block is written by that block
+    // neighbour block is written by that block.  u_cor = u + du needs only u
+    // (in LDS to the end) and du: with the BT_cont outputs its stores wait for
+    // the CFL folds of BT_cont, where the other waves are idle, instead of
+    // joining the burst of the other outputs (1.8% of the kernel)
     const bool writes_face = live && (Y || ((lane > 0 || bx == 0) && lane < FW - 1));
+    auto store_u_cor = [&]() __attribute__((always_inline)) {
+        if (a.u_cor && writes_face) {
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+                if (valid(kk)) a.u_cor[cu + (long long)(k0 + kk) * s] = uk(kk) + du;
+        }
+    };
     if (writes_face) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            if (valid(kk)) {
-                const long long o = cu + (long long)(k0 + kk) * s;
-                a.uh[o] = S.S0[k0 + kk][lane];
-                if (a.u_cor) a.u_cor[o] = uk(kk) + du;
-            }
+            if (valid(kk)) a.uh[cu + (long long)(k0 + kk) * s] = S.S0[k0 + kk][lane];
         }
         if (a.du_cor && grp == 0) a.du_cor[c2] = (double)du;
     }
+    if (!a.bt_h_u) store_u_cor();
     // continuity_zonal_convergence (:148-176) for cell i = I (faces I-1 and I)
     if (!Y) {
         const T dtIareaT = S.P[0][lane];   // dt*IareaT(i, j): cell 0 of the face
@@ -1024,8 +1031,12 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     if (!a.bt_h_u) return;   // block-uniform
 
     // h_u = marginal thickness at u_cor = u + du (zonal_flux_thickness :602-735):
-    // the last Newton pass's when that was at u + du, else evaluated here
-    if (writes_face) {
+    // the last Newton pass's (in S1, each wave's own rows) when that was at
+    // u + du, else evaluated here.  S1 holds until the next pass, so unless a
+    // pass comes before the duL pass (the zero-target iteration, the du0
+    // pass) the stores wait for the CFL folds, like u_cor's
+    auto store_h_u = [&]() __attribute__((always_inline)) {
+        if (!writes_face) return;
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
             if (!valid(kk)) continue;
@@ -1036,13 +1047,16 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             }
             a.bt_h_u[cu + (long long)(k0 + kk) * s] = hm;
         }
-    }
+    };
+    bool h_u_stored = false;   // block-uniform
 
     // set_zonal_BT_cont (:920-1121).  The zero-transport correction equals du
     // whenever the target is zero (identical Newton iterations).
     const bool bt_any = a.uhbt && block_any<CW>(S.P[2][lane] != T(0.0), S.F[2], w, lane);
     T du0 = du, du0_eval = S.P[6][lane], FAmt_0 = S.P[7][lane];
     if (bt_any) {   // without uhbt, bt = 0 everywhere
+        store_h_u();
+        h_u_stored = true;
         const T bt = S.P[2][lane];
         T de, s0;
         const Start again{S.R[5][lane], S.R[6][lane], S.R[7][lane], S.R[8][lane]};
@@ -1053,6 +1067,8 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // when that was at du0 (the same terms in the same order), else a pass of its own
     const bool have0 = (du0_eval == du0);
     if (block_any<CW>(!have0, S.F[3], w, lane)) {
+        if (!h_u_stored) store_h_u();
+        h_u_stored = true;
         eval_pass(du0, !have0);
         sum_pass(T(0.0), T(0.0), !have0);
         if (!have0) FAmt_0 = S.R[1][lane];
@@ -1062,6 +1078,10 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     const T CFL_min = (T)1e-6f;
     const T du_CFL = (CFL_min * Idt) * S.P[1][lane];
     lds_barrier();
+    if (w >= 2) {   // while waves 0 and 1 fold
+        store_u_cor();
+        if (!h_u_stored) store_h_u();
+    }
     if (w == 0) {
         S.R[2][lane] = lds_fold<L>(S.U, nk, lane, vmin(T(0.0), du0 - du_CFL), [&](T x, T uu) {
             return (uu + x > -du_CFL) ? -(uu + du_CFL) : x;
@@ -1072,6 +1092,10 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         });
     }
     lds_barrier();
+    if (w < 2) {
+        store_u_cor();
+        if (!h_u_stored) store_h_u();
+    }
     const T duR = S.R[2][lane], duL = S.R[3][lane];
     // FAmt_L, uhtot_L at u + duL; FAmt_R, uhtot_R at u + duR
     eval_pass(duL, true);
