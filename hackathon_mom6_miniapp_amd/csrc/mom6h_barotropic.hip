@@ -337,6 +337,12 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
     __shared__ double s_dv[RH][ROW ? 1 : RW], s_f4[RH][ROW ? 1 : RW];
     __shared__ double s_ep[ROW ? RH : 1][ROW ? RW : 1];   // ROW: the predicted eta
     __shared__ int s_cx[RW], s_cy[RH];
+    // range tests of each staged column (x) and row (y) on its canonical
+    // index, one bit each, formed once per launch: bit 0 inside the compute
+    // domain (is..ie / js..je), bit 1 inside it but for the last face
+    // (..ie-1 / ..je-1: the u / v update ranges), bit 2 inside the u / v
+    // transport range (iu0..iu1 / jv0..jv1), bit 3 the same for the index - 1
+    __shared__ int s_fx[RW], s_fy[RH];
     auto DV = [&](int lj, int li) -> double { return s_dv[lj][ROW ? 0 : li]; };
     auto F4 = [&](int lj, int li) -> double { return s_f4[lj][ROW ? 0 : li]; };
     auto EP = [&](int lj, int li) -> double { return ROW ? s_ep[ROW ? lj : 0][ROW ? li : 0] : s_e[lj][li]; };
@@ -346,14 +352,24 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
     const int bi = i0 - 2 * TS, bj = j0 - 2 * TS;   // LDS origin
     const int tid = threadIdx.x;
     const double dtbt = p.dtbt;
-    // canonical index of each staged column and row
+    const int iu0 = p.extended ? is - 1 : is, iu1 = p.extended ? ie : ie - 1;
+    const int jv0 = p.extended ? js - 1 : js, jv1 = p.extended ? je : je - 1;
+    auto flags = [](int c, int lo, int hi, int t0, int t1) -> int {
+        return ((c >= lo && c <= hi) ? 1 : 0) | ((c >= lo && c <= hi - 1) ? 2 : 0) |
+               ((c >= t0 && c <= t1) ? 4 : 0) | ((c - 1 >= t0 && c - 1 <= t1) ? 8 : 0);
+    };
+    // canonical index of each staged column and row, and its range bits
     for (int t = tid; t < RW + RH; t += NT) {
         if (t < RW) {
             const int x = bi + t, P = ie - is + 1;
-            s_cx[t] = p.px ? is + ((x - is) % P + P) % P : x;
+            const int c = p.px ? is + ((x - is) % P + P) % P : x;
+            s_cx[t] = c;
+            s_fx[t] = flags(c, is, ie, iu0, iu1);
         } else {
             const int y = bj + (t - RW), Q = je - js + 1;
-            s_cy[t - RW] = p.py ? js + ((y - js) % Q + Q) % Q : y;
+            const int c = p.py ? js + ((y - js) % Q + Q) % Q : y;
+            s_cy[t - RW] = c;
+            s_fy[t - RW] = flags(c, js, je, jv0, jv1);
         }
     }
     __syncthreads();
@@ -392,9 +408,9 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
     // pressure forces and the velocity updates (:234-260) of the canonical
     // cell (i, j) at staged position (li, lj); the coefficients are uniform
     // but for Datv and f4
-    auto new_v = [&](int lj, int li, int i, int j) -> double {
+    auto new_v = [&](int lj, int li) -> double {
         double vn = s_v[lj][li];
-        if (i >= is && i <= ie && j >= js && j <= je - 1) {
+        if ((s_fx[li] & 1) && (s_fy[lj] & 2)) {   // is..ie x js..je-1
             const double f4 = F4(lj, li);
             const double Cor = -1.0 * (((f4 * s_u[lj][li - 1]) + (f4 * s_u[lj + 1][li])) +
                                        ((f4 * s_u[lj][li]) + (f4 * s_u[lj + 1][li - 1])));
@@ -403,9 +419,9 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
         }
         return vn;
     };
-    auto new_u = [&](int lj, int li, int i, int j) -> double {
+    auto new_u = [&](int lj, int li) -> double {
         double un = s_u[lj][li];
-        if (i >= is && i <= ie - 1 && j >= js && j <= je) {
+        if ((s_fx[li] & 2) && (s_fy[lj] & 1)) {   // is..ie-1 x js..je
             const double f4 = F4(lj, li);
             const double Cor = (((f4 * s_v[lj][li + 1]) + (f4 * s_v[lj - 1][li])) +
                                 ((f4 * s_v[lj][li]) + (f4 * s_v[lj - 1][li + 1])));
@@ -416,8 +432,6 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
     };
     const double trans_wt1 = 1.0 + p.bebt, trans_wt2 = -p.bebt;
     const double inv_nstep = 1.0 / (double)p.nstep;
-    const int iu0 = p.extended ? is - 1 : is, iu1 = p.extended ? ie : ie - 1;
-    const int jv0 = p.extended ? js - 1 : js, jv1 = p.extended ? je : je - 1;
     for (int s = 0; s < st.ns; ++s) {
         const int n = st.n0 + s;
         const int k = st.ns - 1 - s;   // substeps of this launch after this one
@@ -450,8 +464,7 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
         // eta predictor (:224-231): in place, or (ROW) into s_ep, which then
         // holds eta itself where no prediction applies
         rect_each<NT, NP>(A0 - 1, A1 + 1, B0 - 1, B1 + 1, tid, [&](int li, int lj) {
-            const int i = s_cx[li], j = s_cy[lj];
-            const bool in = !(i < is || i > ie || j < js || j > je);
+            const bool in = (s_fx[li] & s_fy[lj] & 1) != 0;
             if (!ROW && !in) return;
             const double e0 = s_e[lj][li];
             const double ep = in ? e0 + (dtbt * m.IareaT) *
@@ -465,21 +478,21 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
         // the first velocity update
         if (v_first)
             rect_each<NT, NP>(A0 - 1, A1 + 1, B0 - 1, B1, tid, [&](int li, int lj) {
-                s_v[lj][li] = new_v(lj, li, s_cx[li], s_cy[lj]);
+                s_v[lj][li] = new_v(lj, li);
             });
         else
             rect_each<NT, NP>(A0 - 1, A1, B0 - 1, B1 + 1, tid, [&](int li, int lj) {
-                s_u[lj][li] = new_u(lj, li, s_cx[li], s_cy[lj]);
+                s_u[lj][li] = new_u(lj, li);
             });
         __syncthreads();
         // the second velocity update
         if (v_first)
             rect_each<NT, NP>(A0 - 1, A1, B0, B1, tid, [&](int li, int lj) {
-                s_u[lj][li] = new_u(lj, li, s_cx[li], s_cy[lj]);
+                s_u[lj][li] = new_u(lj, li);
             });
         else
             rect_each<NT, NP>(A0, A1, B0 - 1, B1, tid, [&](int li, int lj) {
-                s_v[lj][li] = new_v(lj, li, s_cx[li], s_cy[lj]);
+                s_v[lj][li] = new_v(lj, li);
             });
         __syncthreads();
         // time averages (:281-291) of the output points: the array faces
@@ -498,13 +511,13 @@ __device__ __forceinline__ void bt_multi_body(const BT &p, const BTUniform &m, c
         for (int r = 0; r < NR; ++r) {
             int li, lj;
             if (!r_pos(r, li, lj)) continue;
-            const int i = s_cx[li], j = s_cy[lj];
-            if (i < is || i > ie || j < js || j > je) continue;
+            const int fx = s_fx[li], fy = s_fy[lj];
+            if (!(fx & fy & 1)) continue;   // outside is..ie x js..je
             const double un = s_u[lj][li], uw = s_u[lj][li - 1], vn = s_v[lj][li], vs = s_v[lj - 1][li];
-            const double uh_e = (i >= iu0 && i <= iu1) ? m.Datu * (trans_wt1 * un + trans_wt2 * u_old[r]) : 0.0;
-            const double uh_w = (i - 1 >= iu0 && i - 1 <= iu1) ? m.Datu * (trans_wt1 * uw + trans_wt2 * uw_old[r]) : 0.0;
-            const double vh_n = (j >= jv0 && j <= jv1) ? DV(lj, li) * (trans_wt1 * vn + trans_wt2 * v_old[r]) : 0.0;
-            const double vh_s = (j - 1 >= jv0 && j - 1 <= jv1) ? DV(lj - 1, li) * (trans_wt1 * vs + trans_wt2 * vs_old[r]) : 0.0;
+            const double uh_e = (fx & 4) ? m.Datu * (trans_wt1 * un + trans_wt2 * u_old[r]) : 0.0;
+            const double uh_w = (fx & 8) ? m.Datu * (trans_wt1 * uw + trans_wt2 * uw_old[r]) : 0.0;
+            const double vh_n = (fy & 4) ? DV(lj, li) * (trans_wt1 * vn + trans_wt2 * v_old[r]) : 0.0;
+            const double vh_s = (fy & 8) ? DV(lj - 1, li) * (trans_wt1 * vs + trans_wt2 * vs_old[r]) : 0.0;
             s_e[lj][li] = e_old[r] - dtbt * m.IareaT * ((uh_e - uh_w) + (vh_n - vh_s));
         }
         __syncthreads();
