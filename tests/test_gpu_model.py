@@ -51,6 +51,20 @@ def test_model_bit_exact(name, mode):
     assert not bad, "\n".join(bad)
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_vert_visc_coefficients_bit_exact(name):
+    """The module path keeps vert_visc's coupling coefficients a and thickness
+    h_u (the u columns of the last call); masked columns (the walls) take a = 0
+    and h_u = h_neglect without a solve of their own."""
+    print(f"[gpu vv coefficients] case={name}", flush=True)
+    gpu, ora = _run_pair(name, fused=False)
+    for f in ("vv_a_u", "vv_h_u"):
+        g, o = gpu.get(f), ora.field(f)
+        assert g.shape == o.shape, (f, g.shape, o.shape)
+        assert np.array_equal(g, o), f"{f}: max|diff| = {np.abs(g - o).max():.3e}"
+    gpu.close()
+
+
 def test_fused_stencil_staging_checks():
     """The fused stencil stages the masks as bytes and Coriolis as one value
     per row, which the model checks on the host.  On the beta-plane case the
