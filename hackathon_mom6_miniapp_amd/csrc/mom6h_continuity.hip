@@ -765,16 +765,70 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         T m5[5], hs[K][5], us[K];
 #pragma unroll
         for (int q = 0; q < 5; ++q) m5[q] = (T)a.mask2dT[ccell(q)];
+        // STAGE: each level's row of thicknesses is loaded once, the lanes'
+        // own cells (position t = lane holds cell f_raw(lane 0) - 2 + t) and
+        // the four beyond the last lane (t = 64..67, loaded by lanes 0-3),
+        // and each lane reads its 5-cell stencil t = lane .. lane+4 back from
+        // LDS: one load per level and lane instead of five.  The rows are
+        // staged in the wave's own rows of S0 (levels 0..KA-1) and S1 (the
+        // rest), which no other wave touches before the first pass's barrier
+        // and this wave's first pass overwrites only after these reads.  A
+        // cell index past iec+3 (the last one the lanes of the unstaged form
+        // read) is clamped there: only lanes past iec+1, which are evaluated
+        // and never written, read a clamped cell.
+        constexpr bool STAGE = FW == 64 && K >= 3;
+        constexpr int KA = K / 2, RL = FW + 4;
+        static_assert(!STAGE || (KA * RL <= (K - 1) * FW && (K - KA) * RL <= K * FW &&
+                                 (K - 1 - KA) * RL <= (K - 1) * FW),
+                      "staged rows exceed the wave's own rows");
+        if constexpr (STAGE) {
+            const int c_lo = f_raw - 2, c_hi = d.iec + 3;
+            const long long rA = IX2(d, c_lo < c_hi ? c_lo : c_hi, fj);
+            const long long rB = IX2(d, c_lo + FW < c_hi ? c_lo + FW : c_hi, fj);
+            T hA[K], hB[K];
 #pragma unroll
-        for (int kk = 0; kk < K; ++kk) {
-            const int kr = k0 + kk + 1;
-            const int k = kr < nk ? kr : nk;
-            const long long o = (long long)(k - 1) * s;
-            us[kk] = a.u[cu + o];
+            for (int kk = 0; kk < K; ++kk) {
+                const int kr = k0 + kk + 1;
+                const int k = kr < nk ? kr : nk;
+                const long long o = (long long)(k - 1) * s;
+                us[kk] = a.u[cu + o];
+                if (!formed(kk)) continue;
+                hA[kk] = a.hin[rA + o];
+                if (lane < 4) hB[kk] = a.hin[rB + o];
+            }
+            park();
+            auto row = [&](int kk) __attribute__((always_inline)) -> T * {
+                return kk < KA ? &S.S0[k0][0] + kk * RL : &S.S1[k0][0] + (kk - KA) * RL;
+            };
 #pragma unroll
-            for (int q = 0; q < 5; ++q) hs[kk][q] = a.hin[ccell(q) + o];
+            for (int kk = 0; kk < K; ++kk) {
+                if (!formed(kk)) continue;
+                row(kk)[lane] = hA[kk];
+                if (lane < 4) row(kk)[FW + lane] = hB[kk];
+            }
+            // one wave's LDS operations complete in order: the fences only
+            // keep the compiler from moving the reads above the stores
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) {
+                if (!formed(kk)) continue;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) hs[kk][q] = row(kk)[lane + q];
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) {
+                const int kr = k0 + kk + 1;
+                const int k = kr < nk ? kr : nk;
+                const long long o = (long long)(k - 1) * s;
+                us[kk] = a.u[cu + o];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) hs[kk][q] = a.hin[ccell(q) + o];
+            }
+            park();
         }
-        park();
         T wW[K], wE[K], w0[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
