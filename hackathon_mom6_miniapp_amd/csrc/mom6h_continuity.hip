@@ -732,17 +732,32 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     col.dt = (T)a.dt;
     const long long c2 = IX2(d, fi, fj);
     col.dy = (T)a.dy_Cu[c2];
-    col.IdxT0 = (T)a.IdxT[cell(2)];
-    col.IdxT1 = (T)a.IdxT[cell(3)];
+    if constexpr (!Y && FW == 64) {
+        // cell 1's IdxT is the next lane's cell 0's, read at cell
+        // min(f_raw, iec+1): the lane after the last live one holds cell
+        // iec+1 (a lane past iec evaluates values that are never written)
+        const int ci0 = f_raw < d.iec + 1 ? f_raw : d.iec + 1;
+        col.IdxT0 = (T)a.IdxT[IX2(d, ci0, fj)];
+        col.IdxT1 = __shfl_down(col.IdxT0, 1, FW);
+    } else {
+        col.IdxT0 = (T)a.IdxT[cell(2)];
+        col.IdxT1 = (T)a.IdxT[cell(3)];
+    }
     const T mCu = (T)a.mask2dCu[c2];
     const long long cu = IX3(d, fi, fj, 1);
-    // the 2D values read after the column passes, issued here with the column
+    // the 2D values read after the column passes, loaded by wave 0 alone
+    // (which parks them in LDS for every wave) and issued with the column
     // loads: a load issued where its value is first used costs the whole
     // block a memory round trip at that point
-    const double IareaT0 = a.IareaT[cell(2)], IareaT1 = a.IareaT[cell(3)];
-    const double dxT0 = a.dxT[cell(2)], dxT1 = a.dxT[cell(3)];
-    const double uhbt_f = a.uhbt ? a.uhbt[c2] : 0.0;
-    const double dxCu_f = a.dxCu[c2];
+    double IareaT0 = 0.0, IareaT1 = 0.0, dxT0 = 0.0, dxT1 = 0.0, uhbt_f = 0.0, dxCu_f = 0.0;
+    if (w == 0) {
+        IareaT0 = a.IareaT[cell(2)];
+        IareaT1 = a.IareaT[cell(3)];
+        dxT0 = a.dxT[cell(2)];
+        dxT1 = a.dxT[cell(3)];
+        uhbt_f = a.uhbt ? a.uhbt[c2] : 0.0;
+        dxCu_f = a.dxCu[c2];
+    }
     auto park = [&]() __attribute__((always_inline)) {
         if (w == 0) {
             if (!Y) S.P[0][lane] = (T)(a.dt * IareaT0);
@@ -762,9 +777,6 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     if (!Y) {
         const int ci = f_raw < d.iec + 1 ? f_raw : d.iec + 1;   // this lane's cell (in bounds)
         auto ccell = [&](int q) __attribute__((always_inline)) -> long long { return IX2(d, ci - 2 + q, fj); };
-        T m5[5], hs[K][5], us[K];
-#pragma unroll
-        for (int q = 0; q < 5; ++q) m5[q] = (T)a.mask2dT[ccell(q)];
         // STAGE: each level's row of thicknesses is loaded once, the lanes'
         // own cells (position t = lane holds cell f_raw(lane 0) - 2 + t) and
         // the four beyond the last lane (t = 64..67, loaded by lanes 0-3),
@@ -775,16 +787,28 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         // and this wave's first pass overwrites only after these reads.  A
         // cell index past iec+3 (the last one the lanes of the unstaged form
         // read) is clamped there: only lanes past iec+1, which are evaluated
-        // and never written, read a clamped cell.
+        // and never written, read a clamped cell.  STAGE_M: the row of masks
+        // likewise, after the thickness rows in S1 where they fit.
         constexpr bool STAGE = FW == 64 && K >= 3;
         constexpr int KA = K / 2, RL = FW + 4;
         static_assert(!STAGE || (KA * RL <= (K - 1) * FW && (K - KA) * RL <= K * FW &&
                                  (K - 1 - KA) * RL <= (K - 1) * FW),
                       "staged rows exceed the wave's own rows");
+        constexpr bool STAGE_M = STAGE && (K - KA + 1) * RL <= (K - 1) * FW;
+        T m5[5], hs[K][5], us[K];
+        if constexpr (!STAGE_M) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) m5[q] = (T)a.mask2dT[ccell(q)];
+        }
         if constexpr (STAGE) {
             const int c_lo = f_raw - 2, c_hi = d.iec + 3;
             const long long rA = IX2(d, c_lo < c_hi ? c_lo : c_hi, fj);
             const long long rB = IX2(d, c_lo + FW < c_hi ? c_lo + FW : c_hi, fj);
+            T mA = T(0.0), mB = T(0.0);
+            if constexpr (STAGE_M) {
+                mA = (T)a.mask2dT[rA];
+                if (lane < 4) mB = (T)a.mask2dT[rB];
+            }
             T hA[K], hB[K];
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) {
@@ -800,6 +824,11 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             auto row = [&](int kk) __attribute__((always_inline)) -> T * {
                 return kk < KA ? &S.S0[k0][0] + kk * RL : &S.S1[k0][0] + (kk - KA) * RL;
             };
+            T *const mrow = &S.S1[k0][0] + (K - KA) * RL;
+            if constexpr (STAGE_M) {
+                mrow[lane] = mA;
+                if (lane < 4) mrow[FW + lane] = mB;
+            }
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) {
                 if (!formed(kk)) continue;
@@ -811,6 +840,10 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+            if constexpr (STAGE_M) {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) m5[q] = mrow[lane + q];
+            }
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) {
                 if (!formed(kk)) continue;
