@@ -743,7 +743,8 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         col.IdxT0 = (T)a.IdxT[cell(2)];
         col.IdxT1 = (T)a.IdxT[cell(3)];
     }
-    const T mCu = (T)a.mask2dCu[c2];
+    T mCu = T(0.0);   // read by the CFL folds of waves 2 and 3 only
+    if (w == 2 || w == 3) mCu = (T)a.mask2dCu[c2];
     const long long cu = IX3(d, fi, fj, 1);
     // the 2D values read after the column passes, loaded by wave 0 alone
     // (which parks them in LDS for every wave) and issued with the column
@@ -779,9 +780,11 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
         auto ccell = [&](int q) __attribute__((always_inline)) -> long long { return IX2(d, ci - 2 + q, fj); };
         // STAGE: each level's row of thicknesses is loaded once, the lanes'
         // own cells (position t = lane holds cell f_raw(lane 0) - 2 + t) and
-        // the four beyond the last lane (t = 64..67, loaded by lanes 0-3),
-        // and each lane reads its 5-cell stencil t = lane .. lane+4 back from
-        // LDS: one load per level and lane instead of five.  The rows are
+        // the four beyond the last lane (t = 64..67, those of all the wave's
+        // levels in one load), and each lane reads its 5-cell stencil t =
+        // lane .. lane+4 back from LDS: one load per level and lane instead
+        // of five (a load with few active lanes costs about what a full one
+        // does, so the extensions share one).  The rows are
         // staged in the wave's own rows of S0 (levels 0..KA-1) and S1 (the
         // rest), which no other wave touches before the first pass's barrier
         // and this wave's first pass overwrites only after these reads.  A
@@ -809,7 +812,12 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
                 mA = (T)a.mask2dT[rA];
                 if (lane < 4) mB = (T)a.mask2dT[rB];
             }
-            T hA[K], hB[K];
+            T hA[K], hX = T(0.0);
+            // the four cells past the last lane of every level in one load:
+            // lane 4*kk + x holds cell 64 + x of slot kk
+            const int xk = lane >> 2;
+            const int cx = f_raw - lane - 2 + FW + (lane & 3);
+            const long long rX = IX2(d, cx < c_hi ? cx : c_hi, fj);
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) {
                 const int kr = k0 + kk + 1;
@@ -818,7 +826,10 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
                 us[kk] = a.u[cu + o];
                 if (!formed(kk)) continue;
                 hA[kk] = a.hin[rA + o];
-                if (lane < 4) hB[kk] = a.hin[rB + o];
+            }
+            if (lane < 4 * K) {
+                const int kr = k0 + xk + 1;
+                hX = a.hin[rX + (long long)((kr < nk ? kr : nk) - 1) * s];
             }
             park();
             auto row = [&](int kk) __attribute__((always_inline)) -> T * {
@@ -833,7 +844,10 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             for (int kk = 0; kk < K; ++kk) {
                 if (!formed(kk)) continue;
                 row(kk)[lane] = hA[kk];
-                if (lane < 4) row(kk)[FW + lane] = hB[kk];
+            }
+            if (lane < 4 * K && formed(xk)) {
+                T *const r = xk < KA ? &S.S0[k0][0] + xk * RL : &S.S1[k0][0] + (xk - KA) * RL;
+                r[FW + (lane & 3)] = hX;
             }
             // one wave's LDS operations complete in order: the fences only
             // keep the compiler from moving the reads above the stores
