@@ -677,7 +677,8 @@ __device__ __forceinline__ float col_sum(const float (*col)[FW], int nk, int lan
 // the first pass); the FP64 tolerance of 1e-12 m lies below what FP32 sums
 // resolve, and the iteration would otherwise run to its 20-pass limit.
 template <typename T, bool Y, int CW, int K, int FW, bool EVEN>
-__device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monotonic, unsigned bx, unsigned by)
+__device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monotonic, unsigned bx, unsigned by,
+                                                bool pf_on = false)
 {
     extern __shared__ double smem_raw[];
     constexpr bool F32 = sizeof(T) == 4;
@@ -972,6 +973,40 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     const T CFL_dt = (T)a.CFL_limit_adjust / (T)a.dt;
     eval_pass(T(0.0), true);
     lds_barrier();
+    // Prefetch (pf_on: a launch whose blocks run in blockIdx order, one per
+    // CU): the u and h rows of the block 256 further on, which the
+    // round-robin dispatch over the 8 XCDs sends to this block's XCD about
+    // one block lifetime later, one 4-byte load per lane, each touching one
+    // 128-byte line of a row (9 lines per level in FP64, 5 in FP32, as many
+    // levels as the block's lanes cover).  A block's set-up waits on its
+    // loads (up to 14 K cycles, the lines of its own rows arriving a few
+    // hundred at a time per CU); lines already in cache arrive sooner.  The
+    // loads are issued here, after this block's own, and their value is
+    // consumed by an empty asm at the end (so that they are kept); they read
+    // inside the arrays, at indices the block they prefetch for reads.
+    int pf = 0;
+    if constexpr (!Y && FW == 64) {
+        const unsigned gx = gridDim.x, b2 = bx + gx * by + 256;
+        if (pf_on && b2 < gx * gridDim.y) {
+            constexpr int LU = (64 * (int)sizeof(T) + 127) / 128, LH = (68 * (int)sizeof(T) + 127) / 128;
+            constexpr int ST = 128 / (int)sizeof(T);
+            const int bx2 = (int)(b2 % gx), by2 = (int)(b2 / gx);
+            const int q = w * 64 + (int)(threadIdx.x & 63);
+            const int lvl = q / (LU + LH), part = q % (LU + LH);
+            if (lvl < nk) {
+                const int f0 = d.isc - 1 + bx2 * (FW - 2), fj2 = d.jsc + by2;
+                const int *pp;
+                if (part < LU) {
+                    const int f = f0 + part * ST;
+                    pp = (const int *)(a.u + IX3(d, f < d.iec ? f : d.iec, fj2, lvl + 1));
+                } else {
+                    const int c = f0 - 2 + (part - LU) * ST;
+                    pp = (const int *)(a.hin + IX2(d, c < d.iec + 3 ? c : d.iec + 3, fj2) + (long long)lvl * s);
+                }
+                pf = *pp;
+            }
+        }
+    }
     const T dx_W = S.P[4][lane], dx_E = S.P[5][lane];
     if (w < 2) {
         S.R[w][lane] = (w == 0) ? col_sum<L>(S.S0, nk, lane, T(0.0)) : sum_dy(T(0.0));
@@ -1129,7 +1164,10 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             }
         }
     }
-    if (!a.bt_h_u) return;   // block-uniform
+    if (!a.bt_h_u) {   // block-uniform
+        asm volatile("" ::"v"(pf));
+        return;
+    }
 
     // h_u = marginal thickness at u_cor = u + du (zonal_flux_thickness :602-735):
     // the last Newton pass's (in S1, each wave's own rows) when that was at
@@ -1205,6 +1243,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     eval_pass(duR, true);
     sum_pass(T(0.0), T(0.0), true);
     const T uhtot_R = S.R[0][lane], FAmt_R = S.R[1][lane];
+    asm volatile("" ::"v"(pf));
     if (!writes_face || grp != 0) return;
     T FA_0 = FAmt_0, FA_avg = FAmt_0;
     if ((duL - du0) != T(0.0)) FA_avg = uhtot_L / (duL - du0);
@@ -1232,7 +1271,7 @@ __global__ void __launch_bounds__(64 * CW) __attribute__((amdgpu_waves_per_eu(
     (sizeof(T) == 4 || FW == 32) ? 4 : (CW + 3) / 4, (CW >= 12 || sizeof(T) == 4 || FW == 32) ? 4 : 2)))
 k_continuity_blk(ContArgsT<T> a, int monotonic)
 {
-    continuity_body<T, Y, CW, K, FW, EVEN>(a, monotonic, blockIdx.x, blockIdx.y);
+    continuity_body<T, Y, CW, K, FW, EVEN>(a, monotonic, blockIdx.x, blockIdx.y, true);
 }
 
 // the members of an ensemble that one batched launch covers: entry q of its
