@@ -525,8 +525,7 @@ struct ColRegs {
 // nothing and are never read as values; u lives here rather than in
 // registers), S0/S1 = per-level terms of a pass,
 // R = results of the column sums (one row per quantity; rows 5..8 keep the
-// Newton iteration's starting point for a second iteration), F = the votes of
-// block_any.
+// Newton iteration's starting point for a second iteration).
 template <typename T>
 constexpr int walk_batch = 4;   // rows per batch of a column walk (two batches in flight)
 // FW = faces per block (64: one level group per wave; 32: two, lanes 0-31
@@ -542,9 +541,6 @@ struct ContShared {
     // dxT of cells 0 and 1; P[6], P[7] = du_eval and sum_dyhm of the face's
     // Newton iteration, kept over the output stores for BT_cont
     T P[8][FW];
-    // per-wave votes of the block-wide "any" tests: 0, 1 alternate over the
-    // Newton iterations, 2 = the barotropic-transport test, 3 = the du0 pass test
-    int F[4][16];
 };
 
 // Visits rows 0..nk-1 (nk <= L) of one LDS column of ContShared in k order: step(x_k).
@@ -603,21 +599,17 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Block-wide "any lane has pred" with one barrier (the runtime's
-// __syncthreads_or takes three): each wave stores its vote in its own word of
-// F, then every thread reads the CW words.  The caller passes a buffer that no
-// other wave can still be reading: consecutive uses of one buffer must be
-// separated by another barrier.
-template <int CW>
-__device__ __forceinline__ bool block_any(bool pred, int *F, int w, int lane)
+// Block-wide "any face has pred", for the predicates of the Newton iteration
+// and of BT_cont: every wave holds all the block's faces (FW = 32: twice) and
+// evaluates pred for each from the same operands (the column sums and parked
+// 2D values in LDS, and what every wave derives from them by the same
+// instructions), so one wave's vote is the block's and needs no barrier (an
+// LDS vote and a barrier per test measured 4% of the kernel).  The caller's
+// LDS reads and writes around a test are ordered by the barriers of the
+// column sums.
+__device__ __forceinline__ bool block_any(bool pred)
 {
-    const bool vote = __any(pred);
-    if (lane == 0) F[w] = vote ? 1 : 0;
-    lds_barrier();
-    int r = 0;
-#pragma unroll
-    for (int q = 0; q < CW; ++q) r |= F[q];
-    return r != 0;
+    return __any(pred);
 }
 
 // Sequential fold acc = op(acc, x_k) over rows 0..nk-1 of one LDS column
@@ -1091,8 +1083,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
             }
             // a finished column keeps its du for the rest of the reference's row loop
             const bool ev = do_I && ((itt < max_itts) || keep);
-            // buffers alternate by iteration: the sum's two barriers lie between two uses
-            if (!block_any<CW>(ev, S.F[itt & 1], w, lane)) break;
+            if (!block_any(ev)) break;
             eval_pass(du, ev);
             const bool summed = ev && itt < max_itts;
             sum_pass(-uhbt, T(0.0), summed);
@@ -1191,7 +1182,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
 
     // set_zonal_BT_cont (:920-1121).  The zero-transport correction equals du
     // whenever the target is zero (identical Newton iterations).
-    const bool bt_any = a.uhbt && block_any<CW>(S.P[2][lane] != T(0.0), S.F[2], w, lane);
+    const bool bt_any = a.uhbt && block_any(S.P[2][lane] != T(0.0));
     T du0 = du, du0_eval = S.P[6][lane], FAmt_0 = S.P[7][lane];
     if (bt_any) {   // without uhbt, bt = 0 everywhere
         store_h_u();
@@ -1205,7 +1196,7 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     // FAmt_0 = sum dy*hm(u + du0): the sum of the iteration's last evaluation
     // when that was at du0 (the same terms in the same order), else a pass of its own
     const bool have0 = (du0_eval == du0);
-    if (block_any<CW>(!have0, S.F[3], w, lane)) {
+    if (block_any(!have0)) {
         if (!h_u_stored) store_h_u();
         h_u_stored = true;
         eval_pass(du0, !have0);
