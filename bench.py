@@ -81,6 +81,16 @@ STAGE_CALLS = {"transports": 2, "hor_visc": 2, "coriolis": 2, "vel_update": 2, "
 ROOFLINE_KERNELS = {"momentum": "k_momentum", "vert_visc": "k_vert_visc_chunked",
                     "continuity": "k_continuity_blk"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_720x720x75.md")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters_720x720x75.md")
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
+# what binds each roofline kernel (from its counters and per-wave time stamps)
+ROOFLINE_LIMIT = {
+    "k_continuity_blk": "latency: one 12-wave block per CU, its set-up waiting on its loads, "
+                        "serial k-order column folds and the barriers around them",
+    "k_momentum": "VALU issue (bit-exact FP64 divisions)",
+    "k_vert_visc_chunked": "forward-sweep VALU issue and the memory queueing of the load-heavy "
+                           "sweeps at one wave per SIMD",
+}
 
 
 def measured_traffic(kernel: str):
@@ -92,6 +102,21 @@ def measured_traffic(kernel: str):
                 parts = line.split()
                 if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
                     return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
+def measured_counters(kernel: str):
+    """(FP64 FLOPs per launch, VALU busy fraction) of a kernel at 720x720x75
+    from the committed counter summary (lines "counters <kernel> <flops>
+    <busy>"), or None."""
+    try:
+        with open(COUNTERS_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "counters" and parts[1] == kernel:
+                    return float(parts[2]), float(parts[3])
     except OSError:
         pass
     return None
@@ -468,7 +493,14 @@ def main():
             roofline = {"bound": "hbm", "kernel": kern, "stage": dominant, "achieved": round(ach, 1),
                         "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
                         "traffic": traffic, "algorithmic_bytes_per_launch": alg,
-                        "mean_launch_ms": per_stage[dominant]["ms_per_call"]}
+                        "mean_launch_ms": per_stage[dominant]["ms_per_call"],
+                        "limit": ROOFLINE_LIMIT.get(kern)}
+            ctr = measured_counters(kern) if (cfg.ni, cfg.nj, cfg.nk) == (720, 720, 75) else None
+            if ctr:
+                tf = ctr[0] / (per_stage[dominant]["ms_per_call"] * 1e-3) / 1e12
+                roofline.update({"valu_busy": ctr[1], "fp64_flops_per_launch": ctr[0],
+                                 "fp64_tflops": round(tf, 2), "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                                 "fp64_frac": round(tf / FP64_PEAK_TFLOPS, 4)})
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
