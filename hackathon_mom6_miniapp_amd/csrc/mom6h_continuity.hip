@@ -965,17 +965,20 @@ __device__ __forceinline__ void continuity_body(const ContArgsT<T> &a, int monot
     const T CFL_dt = (T)a.CFL_limit_adjust / (T)a.dt;
     eval_pass(T(0.0), true);
     lds_barrier();
-    // Prefetch (pf_on: a launch whose blocks run in blockIdx order, one per
-    // CU): the u and h rows of the block 256 further on, which the
-    // round-robin dispatch over the 8 XCDs sends to this block's XCD about
-    // one block lifetime later, one 4-byte load per lane, each touching one
-    // 128-byte line of a row (9 lines per level in FP64, 5 in FP32, as many
-    // levels as the block's lanes cover).  A block's set-up waits on its
-    // loads (up to 14 K cycles, the lines of its own rows arriving a few
-    // hundred at a time per CU); lines already in cache arrive sooner.  The
-    // loads are issued here, after this block's own, and their value is
-    // consumed by an empty asm at the end (so that they are kept); they read
-    // inside the arrays, at indices the block they prefetch for reads.
+    // Prefetch (pf_on: the single-model launch, whose blocks are dispatched
+    // in blockIdx order, round-robin over the 8 XCDs): the u and h rows of
+    // block b + 256, which runs on this block's XCD (256 is a multiple of 8)
+    // about one block lifetime later at one block per CU.  One 4-byte load per
+    // lane touches one 128-byte line of a row (9 lines per level in FP64, 5 in
+    // FP32, as many levels as the block's lanes cover).  A block's set-up
+    // waits on its own loads (the last wave of each SIMD receives its rows
+    // about 14 K cycles after the block starts); lines this prefetch brought
+    // in arrive sooner (0.98 against 1.017 ms per FP64 launch at
+    // 720x720x75; the FP32 launch gains 4%, and a distance of 512 measured no
+    // better for it).  The loads are issued after this block's first pass,
+    // their value is consumed by an empty asm at the end so that they are
+    // kept, and they read inside the arrays, at indices that the block they
+    // prefetch for reads.
     int pf = 0;
     if constexpr (!Y && FW == 64) {
         const unsigned gx = gridDim.x, b2 = bx + gx * by + 256;
